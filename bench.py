@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X kernel-sum hot path (one GPU).
+
+Workload (the N-body kernel-sum drop-in): 262,144 points, sources = targets,
+seeded uniform in [-4,4]^2 x [0.05,4] (numpy default_rng(2)), forces i.i.d.
+N(0,1), eps = 0.02, mu = 1.  One step = one fl.evaluate of the fused
+WallStokeslet3D kernel sum in FP64 over all 6.87e10 target-source pairs, with
+inputs resident in HBM.  value = pair evaluations per second.
+
+Also reported in the same JSON line: the other kernels of the workload
+(Stokeslet3D, Brinkmanlet3D(4), Stokeslet3D with mu = 1+0.5i) in FP64 and
+FP32C with their errors against FP64, the roofline of the dominant kernel
+(FP64 flops per pair / its average launch time, measured with HIP events on
+the launch stream) and a CPU baseline: the oracle's C/OpenMP restatement of
+the reference loop timed on this host's cores over a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--solve]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+N_POINTS = 262144
+EPS = 0.02
+SEED = 2
+# FP64 flops per pair (add/sub/mul = 1, FMA = 2, sqrt/rsqrt/div/exp = 1;
+# per-source constants amortised; real strengths).
+FLOPS_PER_PAIR = {"wall": 116, "stokeslet": 32, "brinkman": 50}
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and FP64 matrix) peak
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_inputs():
+    rng = np.random.default_rng(SEED)
+    X = rng.uniform([-4, -4, 0.05], [4, 4, 4], (N_POINTS, 3))
+    F = rng.standard_normal((N_POINTS, 3))
+    return X, F
+
+
+def profile_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated),
+    or None when no summary is present."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(X, F, n_targets: int):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O  # the oracle is the CPU baseline, not the product
+    cores = len(os.sched_getaffinity(0))
+    threads = min(cores, 16)
+    O.set_threads(threads)
+    O.evaluate("wall", X[:8], X[:2048], F[:2048], EPS)  # warm the library
+    t0 = time.perf_counter()
+    O.evaluate("wall", X[:n_targets], X, F, EPS)
+    dt = time.perf_counter() - t0
+    return {"value": n_targets * N_POINTS / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+            "sample": f"WallStokeslet3D FP64, first {n_targets} targets x all {N_POINTS} sources "
+                      f"({n_targets * N_POINTS:.3e} pairs, {dt:.1f} s); C/OpenMP restatement of "
+                      f"core/matvec.py:_apply with the reference's pow()-based block, -O3, no fast-math"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-targets", type=int, default=4096)
+    ap.add_argument("--no-table", action="store_true", help="skip the other kernels of the workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(the kernel sum shards by targets with an all-gather of sources; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fluidlets_amd as fl
+    from fluidlets_amd import _device as D
+    from fluidlets_amd.core.matvec import evaluate_device
+
+    dev = D.device()
+    X, F = c2_inputs()
+    x = D.soa(X, dev)
+    f = D.soa(F, dev)
+    eps_t, stride = D.eps_arg(EPS, N_POINTS, dev)
+    wall = fl.WallStokeslet3D()
+    u = torch.empty((3, N_POINTS), dtype=torch.float64, device=dev)
+    pairs = float(N_POINTS) * N_POINTS
+
+    for _ in range(args.warmup):
+        evaluate_device(wall, x, x, eps_t, stride, f, u_re=u)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        evaluate_device(wall, x, x, eps_t, stride, f, u_re=u)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t0
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    ms_per_step = 1e3 * wall_s / args.steps
+    value = pairs / (wall_s / args.steps)
+    achieved = pairs * FLOPS_PER_PAIR["wall"] / (kernel_ms * 1e-3) / 1e12
+    traffic = profile_traffic("nbody_wall_c2_traffic.txt")
+
+    result = {
+        "metric": "pair-evals/s (FP64)",
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded uniform points, N(0,1) forces)",
+        "config": {"workload": "N-body kernel-sum drop-in: WallStokeslet3D FP64, 262,144 sources = targets "
+                               "uniform in [-4,4]^2x[0.05,4], eps=0.02, mu=1 (seed 2)",
+                   "points": N_POINTS, "pairs_per_step": pairs, "kernel": "WallStokeslet3D", "eps": EPS},
+        "roofline": {"bound": "valu", "pipe": "FP64 vector ALU (compute bound; HBM traffic is ~56 B per point)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "fl::nbody64<WallStokeslet3D>", "kernel_ms": kernel_ms,
+                     "flops_per_pair": FLOPS_PER_PAIR["wall"]},
+    }
+
+    if not args.no_table:
+        table = {}
+        ref = {}
+        cases = [("wall", fl.WallStokeslet3D(), 1.0), ("stokeslet", fl.Stokeslet3D(), 1.0),
+                 ("brinkman4", fl.Brinkmanlet3D(4.0), 1.0), ("stokeslet_mu1+0.5i", fl.Stokeslet3D(), 1 + 0.5j)]
+        for name, K, mu in cases:
+            for prec in ["fp64", "fp32c"]:
+                evaluate_device(K, x, x, eps_t, stride, f, mu=mu, precision=prec)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                ur, ui = evaluate_device(K, x, x, eps_t, stride, f, mu=mu, precision=prec)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1)
+                uu = ur if ui is None else torch.complex(ur, ui)
+                row = {"pair_evals_per_s": pairs / (ms * 1e-3), "ms": ms}
+                kfam = "wall" if name == "wall" else ("brinkman" if name.startswith("brinkman") else "stokeslet")
+                row["tflops_fp64_convention"] = pairs * FLOPS_PER_PAIR[kfam] / (ms * 1e-3) / 1e12
+                if prec == "fp64":
+                    ref[name] = uu.clone()
+                else:
+                    row["rel_err_vs_fp64"] = (torch.linalg.norm(uu - ref[name]) / torch.linalg.norm(ref[name])).item()
+                table[f"{name}/{prec}"] = row
+        result["c2_table"] = table
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(X, F, args.cpu_targets)
+        result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

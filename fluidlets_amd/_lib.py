@@ -1,0 +1,77 @@
+"""ctypes binding of libfluidlets_hip.so (the C ABI in include/fluidlets_hip.h).
+
+The library is built in-tree (``python -m fluidlets_amd.build``).  There is
+no fallback: if the library is missing, importing a compute function raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfluidlets_hip.so"
+
+FL_OK, FL_EINVAL, FL_EHIP, FL_ENOCONV, FL_ESINGULAR, FL_EWORKSPACE = 0, -1, -2, -3, -4, -5
+FL_FP64, FL_FP32C = 0, 1
+
+
+class fl_kernel(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("reserved", c_int32), ("alpha", c_double)]
+
+
+_SIGNATURES = {
+    "fl_version": (c_int, []),
+    "fl_get_last_error": (c_int, [ctypes.c_char_p, c_size_t]),
+    "fl_device_cus": (c_int, []),
+    "fl_eval_workspace_size": (c_size_t, [c_int64, c_int64, c_int, c_int]),
+    "fl_eval": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                        c_void_p, c_double, c_double, c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
+    "fl_assemble": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_double,
+                            c_double, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p]),
+    "fl_assemble_nearest": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                    c_int64, c_void_p, c_void_p, c_double, c_void_p, c_int64, c_void_p, c_int64,
+                                    c_void_p]),
+    "fl_nearest": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH.name} is not built; run `python -m fluidlets_amd.build` "
+                              "(fluidlets_amd has no CPU fallback)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    lib().fl_get_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int, what: str) -> None:
+    """Raise the Python exception the reference raises for this kind of failure."""
+    if status == FL_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if status == FL_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def kernel_struct(kernel) -> fl_kernel:
+    return fl_kernel(int(kernel.kind), 0, float(getattr(kernel, "alpha", 0.0)))

@@ -1,0 +1,121 @@
+"""Dense kernel matrices on the GPU.
+
+Drop-in for src/fluidlets/core/assemble.py: ``assemble`` (27-46),
+``assemble_nearest`` (68-101) and ``nearest_neighbors`` (61-65).  Entry layout
+follows the [point, component] convention: row p*i + a is component a at
+target i, column q*j + b is component b of source j, so
+``K @ forces.ravel()`` gives ``velocities.ravel()``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _device as D
+from .._lib import check, kernel_struct, lib
+from ._common import blob_widths, field_dtype, points
+
+
+def _eps_device(eps, n, dev):
+    e = eps if not (n > 1 and np.all(eps == eps[0])) else eps[:1]
+    return D.eps_arg(e, n, dev)
+
+
+def assemble_device(kernel, tgt, src, eps_t, eps_stride, mu=1.0, fp64=True, fp32=False, stream=None):
+    """Device assembly from (3, M) / (3, N) tensors.  Returns (K64, K32);
+    either may be None.  For complex mu returns (Kc, None) with Kc complex128."""
+    t = D.torch()
+    M, N = int(tgt.shape[1]), int(src.shape[1])
+    mu = complex(mu)
+    k = kernel_struct(kernel)
+    L = lib()
+    if mu.imag != 0.0:
+        Kc = t.empty((3 * M, 3 * N), dtype=t.complex128, device=tgt.device)
+        st = L.fl_assemble(k, M, N, D.ptr(tgt), D.ptr(src), D.ptr(eps_t), eps_stride, mu.real, mu.imag,
+                           None, 3 * N, None, 3 * N, D.ptr(Kc), D.stream_handle(stream))
+        check(st, "fl_assemble")
+        return Kc, None
+    K64 = t.empty((3 * M, 3 * N), dtype=t.float64, device=tgt.device) if fp64 else None
+    K32 = t.empty((3 * M, 3 * N), dtype=t.float32, device=tgt.device) if fp32 else None
+    st = L.fl_assemble(k, M, N, D.ptr(tgt), D.ptr(src), D.ptr(eps_t), eps_stride, mu.real, 0.0,
+                       D.ptr(K64), 3 * N, D.ptr(K32), 3 * N, None, D.stream_handle(stream))
+    check(st, "fl_assemble")
+    return K64, K32
+
+
+def assemble(kernel, targets, sources, eps, mu=1.0) -> np.ndarray:
+    """Dense (p M) x (q N) matrix of ``kernel`` from N sources to M targets
+    (reference signature and result; a complex mu gives a complex matrix)."""
+    targets = points(targets, kernel.dim)
+    sources = points(sources, kernel.dim)
+    M, N = targets.shape[0], sources.shape[0]
+    eps = blob_widths(eps, N)
+    p, q = kernel.out_shape
+    dtype = field_dtype(mu)
+    if M == 0 or N == 0:
+        return np.zeros((p * M, q * N), dtype=dtype)
+    dev = D.device()
+    eps_t, stride = _eps_device(eps, N, dev)
+    K, _ = assemble_device(kernel, D.soa(targets, dev), D.soa(sources, dev), eps_t, stride, mu)
+    out = K.cpu().numpy()
+    return out if out.dtype == dtype else out.astype(dtype)
+
+
+def nearest_groups(nearest: np.ndarray, n_force: int):
+    """Quadrature indices grouped by force point, increasing within each group
+    (the order in which core/assemble.py:53-58 accumulates them)."""
+    nearest = np.ascontiguousarray(nearest, dtype=np.int64)
+    if nearest.size and (nearest.min() < 0 or nearest.max() >= n_force):
+        raise ValueError("nearest indices must lie in [0, n_force)")
+    order = np.argsort(nearest, kind="stable").astype(np.int64)
+    start = np.searchsorted(nearest[order], np.arange(n_force + 1)).astype(np.int64)
+    return order, start
+
+
+def assemble_nearest_device(kernel, tgt, quad, eps_t, eps_stride, order_t, start_t, n_force, mu=1.0,
+                            K64=None, fp32=False, stream=None):
+    t = D.torch()
+    M, Q = int(tgt.shape[1]), int(quad.shape[1])
+    if K64 is None:
+        K64 = t.empty((3 * M, 3 * n_force), dtype=t.float64, device=tgt.device)
+    ld = int(K64.stride(0))
+    K32 = t.empty((3 * M, 3 * n_force), dtype=t.float32, device=tgt.device) if fp32 else None
+    st = lib().fl_assemble_nearest(kernel_struct(kernel), M, Q, int(n_force), D.ptr(tgt), D.ptr(quad), D.ptr(eps_t),
+                                   eps_stride, D.ptr(order_t), D.ptr(start_t), float(mu), D.ptr(K64), ld,
+                                   D.ptr(K32), 3 * n_force, D.stream_handle(stream))
+    check(st, "fl_assemble_nearest")
+    return K64, K32
+
+
+def assemble_nearest(kernel, targets, quad_points, nearest, n_force, eps, mu=1.0) -> np.ndarray:
+    """Kernel matrix with forces on a coarse set and quadrature on a fine set
+    (Gallagher & Smith nearest-neighbour discretization):
+    K[i, j] = sum over k with nearest[k] = j of G(x_i, X_k) / mu."""
+    targets = points(targets, kernel.dim)
+    quad = points(quad_points, kernel.dim)
+    Q = quad.shape[0]
+    eps = blob_widths(eps, Q)
+    nearest = np.ascontiguousarray(nearest, dtype=np.int64)
+    p, q = kernel.out_shape
+    n_force = int(n_force)
+    dtype = field_dtype(mu)
+    M = targets.shape[0]
+    if M == 0 or n_force == 0:
+        return np.zeros((p * M, q * n_force), dtype=dtype)
+    order, start = nearest_groups(nearest, n_force)
+    dev = D.device()
+    t = D.torch()
+    eps_t, stride = _eps_device(eps, Q, dev)
+    mu_c = complex(mu)
+    K, _ = assemble_nearest_device(kernel, D.soa(targets, dev), D.soa(quad, dev), eps_t, stride,
+                                   t.from_numpy(order).to(dev), t.from_numpy(start).to(dev), n_force,
+                                   mu=mu_c.real if mu_c.imag == 0 else 1.0)
+    out = K.cpu().numpy()
+    if mu_c.imag != 0:
+        return out.astype(dtype) / mu
+    return out.astype(dtype) if out.dtype != dtype else out
+
+
+def nearest_neighbors(quad_points, force_points) -> np.ndarray:
+    """Index of the nearest force point for every quadrature point, shape (Q,)."""
+    from .. import _nearest
+    return _nearest.nearest_neighbors(quad_points, force_points)

@@ -1,0 +1,149 @@
+// extern "C" entry points of libfluidlets_hip.so: argument checks, error slot
+// and dispatch to the HIP kernels.  The declarations (with the reference call
+// each one replaces) are in include/fluidlets_hip.h.
+#include "fl_internal.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace fl {
+
+namespace {
+thread_local int g_status = FL_OK;
+thread_local char g_msg[512] = "";
+}  // namespace
+
+int set_error(int status, const char* fmt, ...)
+{
+    g_status = status;
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+void clear_error()
+{
+    g_status = FL_OK;
+    g_msg[0] = '\0';
+}
+
+int device_cus()
+{
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+    if (cache[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cache[dev] = cus;
+    }
+    return cache[dev];
+}
+
+int assemble_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt, const double* src,
+                    const double* eps, int64_t eps_stride, double mu_re, double mu_im, double* K64, int64_t ld,
+                    float* K32, int64_t ld32, double* Kc, hipStream_t stream);
+int assemble_nearest_launch(int kind, double alpha, int64_t M, int64_t Q, int64_t n_force, const double* tgt,
+                            const double* quad, const double* eps, int64_t eps_stride, const int64_t* order,
+                            const int64_t* start, double mu_re, double* K64, int64_t ld, float* K32, int64_t ld32,
+                            hipStream_t stream);
+
+int nearest_launch(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* out,
+                   hipStream_t stream);
+
+}  // namespace fl
+
+using namespace fl;
+
+namespace {
+
+int check_kernel(const fl_kernel* k)
+{
+    if (!k) return set_error(FL_EINVAL, "kernel is NULL");
+    if (k->kind < FL_STOKESLET3D || k->kind > FL_BRINKMANLET3D)
+        return set_error(FL_EINVAL, "unknown kernel kind %d", k->kind);
+    if (k->kind == FL_BRINKMANLET3D && !(k->alpha >= 0.0))
+        return set_error(FL_EINVAL, "alpha must be real and non-negative");
+    return FL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fl_version(void) { return FL_ABI_VERSION; }
+
+int fl_get_last_error(char* buf, size_t n)
+{
+    if (buf && n > 0) {
+        std::strncpy(buf, g_msg, n - 1);
+        buf[n - 1] = '\0';
+    }
+    return g_status;
+}
+
+int fl_device_cus(void) { return device_cus(); }
+
+size_t fl_eval_workspace_size(int64_t M, int64_t N, int complex_strengths, int precision)
+{
+    return eval_workspace_size(M, N, complex_strengths ? 2 : 1, precision);
+}
+
+int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N, const double* tgt, const double* src, const double* eps,
+            int64_t eps_stride, const double* f_re, const double* f_im, double mu_re, double mu_im, double* u_re,
+            double* u_im, int precision, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    FL_CHECK(mu_re != 0.0 || mu_im != 0.0, "mu must be nonzero");
+    const double s = kernel_scale(kernel->kind);
+    double w_re, w_im;
+    if (mu_im == 0.0) {
+        w_re = s / mu_re;
+        w_im = 0.0;
+    } else {
+        const double den = mu_re * mu_re + mu_im * mu_im;
+        w_re = s * (mu_re / den);
+        w_im = -s * (mu_im / den);
+    }
+    return eval_launch(kernel->kind, kernel->alpha, M, N, tgt, src, eps, eps_stride, f_re, f_im, w_re, w_im, u_re,
+                       u_im, precision, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_assemble(const fl_kernel* kernel, int64_t M, int64_t N, const double* tgt, const double* src,
+                const double* eps, int64_t eps_stride, double mu_re, double mu_im, double* K64, int64_t ld,
+                float* K32, int64_t ld32, double* Kc, void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    return assemble_launch(kernel->kind, kernel->alpha, M, N, tgt, src, eps, eps_stride, mu_re, mu_im, K64, ld, K32,
+                           ld32, Kc, as_stream(stream));
+}
+
+int fl_assemble_nearest(const fl_kernel* kernel, int64_t M, int64_t Q, int64_t n_force, const double* tgt,
+                        const double* quad, const double* eps, int64_t eps_stride, const int64_t* order,
+                        const int64_t* start, double mu_re, double* K64, int64_t ld, float* K32, int64_t ld32,
+                        void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    return assemble_nearest_launch(kernel->kind, kernel->alpha, M, Q, n_force, tgt, quad, eps, eps_stride, order,
+                                   start, mu_re, K64, ld, K32, ld32, as_stream(stream));
+}
+
+int fl_nearest(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* nearest, void* stream)
+{
+    clear_error();
+    return nearest_launch(Q, N, quad, force, nearest, as_stream(stream));
+}
+
+}  // extern "C"

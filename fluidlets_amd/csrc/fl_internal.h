@@ -1,0 +1,58 @@
+// Internal helpers shared by the translation units of libfluidlets_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "../../include/fluidlets_hip.h"
+
+namespace fl {
+
+// Thread-local error slot behind fl_get_last_error().
+int set_error(int status, const char* fmt, ...);
+void clear_error();
+
+#define FL_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::fl::set_error(FL_EHIP, "%s failed: %s (%s:%d)", #call,                 \
+                                   hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+#define FL_CHECK(cond, ...)                                                                \
+    do {                                                                                   \
+        if (!(cond)) return ::fl::set_error(FL_EINVAL, __VA_ARGS__);                        \
+    } while (0)
+
+// Checks the launch that was just enqueued.
+#define FL_LAUNCHED()                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess)                                                              \
+            return ::fl::set_error(FL_EHIP, "kernel launch failed: %s (%s:%d)",             \
+                                   hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int device_cus();
+
+// Kernel-sum entry used by the C ABI and by the solvers (GMRES, refinement).
+// Arrays are SoA (3, M) / (3, N); fi (nullable) holds imaginary strengths;
+// u_re receives the scaled real part, u_im (nullable) the imaginary part.
+// ws must hold eval_workspace_size() bytes.
+size_t eval_workspace_size(int64_t M, int64_t N, int nf, int precision);
+int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt, const double* src,
+                const double* eps, int64_t eps_stride, const double* f, const double* fi, double w_re, double w_im,
+                double* u_re, double* u_im, int precision, void* ws, size_t ws_bytes, hipStream_t stream);
+
+// Scale applied to the raw kernel sums: 1/(8 pi) for the Stokeslet families,
+// 1/(4 pi) for the Brinkmanlet.
+double kernel_scale(int kind);
+
+}  // namespace fl

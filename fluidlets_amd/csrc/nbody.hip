@@ -1,0 +1,613 @@
+// Tiled N-body kernel sums of the regularized Stokeslet family on gfx950.
+//
+// One target per lane (TPL targets per lane when register-blocked), 256-lane
+// workgroups, source tiles of 256 staged in LDS from coalesced
+// structure-of-arrays loads and read back as wave-wide broadcasts.  Every lane
+// walks the sources in index order, so each target's sum has the same order as
+// the reference loop (src/fluidlets/core/matvec.py:15-21); when the source
+// range is split across workgroups (to fill 256 CUs at small M), each chunk
+// keeps that order and the chunk sums are combined in chunk order by a second
+// kernel, so results are deterministic.
+//
+// Pair formulas (mu = 1; constant factors 1/(8 pi) or 1/(4 pi) applied once
+// per target in the epilogue):
+//   Stokeslet   kernels/stokeslet3d.py:22-50
+//   Wall        kernels/images.py:26-89, free + image fused in one pass that
+//               shares d0, d1, d0^2 + d1^2 and the source tile
+//   Brinkmanlet kernels/brinkmanlet3d.py:52-104, with 1/z = rsqrt(Q)/alpha
+//               and the 24-term series below z = 0.5 in FP64
+#include "fl_internal.h"
+
+#include <algorithm>
+#include <cstdlib>
+
+namespace fl {
+namespace {
+
+constexpr int WG = 256;
+constexpr int TILE = 256;
+constexpr int NTERMS = 24;
+
+struct SeriesCoef {
+    double c1[NTERMS];
+    double c2[NTERMS];
+};
+
+// c1_n = (-1)^n [1/(n+2)! - 1/(n+1)! + 1/n!],
+// c2_n = (-1)^(n+1) [3/(n+2)! - 3/(n+1)! + 1/n!]  (Taylor coefficients of
+// e^{-z}(1 + 1/z + 1/z^2) - 1/z^2 and 3/z^2 - e^{-z}(1 + 3/z + 3/z^2)).
+constexpr SeriesCoef make_series()
+{
+    SeriesCoef c{};
+    double fact[NTERMS + 3] = {};
+    fact[0] = 1.0;
+    for (int k = 1; k < NTERMS + 3; ++k) fact[k] = fact[k - 1] * k;
+    for (int n = 0; n < NTERMS; ++n) {
+        double s = (n % 2 == 0) ? 1.0 : -1.0;
+        c.c1[n] = s * (1.0 / fact[n + 2] - 1.0 / fact[n + 1] + 1.0 / fact[n]);
+        c.c2[n] = -s * (3.0 / fact[n + 2] - 3.0 / fact[n + 1] + 1.0 / fact[n]);
+    }
+    return c;
+}
+
+__constant__ SeriesCoef kSeries = make_series();
+
+__device__ __forceinline__ void brinkman_series(double z, double& f1, double& f2)
+{
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int n = NTERMS - 1; n >= 0; --n) {
+        s1 = s1 * z + kSeries.c1[n];
+        s2 = s2 * z + kSeries.c2[n];
+    }
+    f1 = s1;
+    f2 = s2;
+}
+
+// ---------------------------------------------------------------- FP64 pairs
+// acc[3r + a] += (G f_r)_a in units of the per-kernel scale.
+
+template <int NF>
+__device__ __forceinline__ void pair_stokeslet(double d0, double d1, double d2, double e2, const double* f,
+                                               double* acc)
+{
+    double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    double Q = r2 + e2;
+    double rs = rsqrt(Q);
+    double s3 = rs * rs * rs;
+    double h1 = (Q + e2) * s3;  // (r^2 + 2 eps^2) / Q^{3/2}
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        double fd = f[3 * r] * d0 + f[3 * r + 1] * d1 + f[3 * r + 2] * d2;
+        double t = s3 * fd;
+        acc[3 * r] += h1 * f[3 * r] + t * d0;
+        acc[3 * r + 1] += h1 * f[3 * r + 1] + t * d1;
+        acc[3 * r + 2] += h1 * f[3 * r + 2] + t * d2;
+    }
+}
+
+// Free Stokeslet + no-slip image system for a source at height h0 (hh = h0^2).
+// d0, d1: in-plane displacement; dzf = x3 - h0; dzi = x3 + h0 (from the image).
+// Image coefficients are doubled so that both parts share the 1/(8 pi) scale:
+//   A1 = 2 H1, s3 = 2 H2, P1 = 2 Hdp1, P2 = 2 Hdp2, B1..B3 = 2 Hdb1..3, R1 = 2 Hr1.
+// Applied to f with w = d0 f0 + d1 f1, g.d = dzi f2 - w (the dipole's sign
+// flip on the in-plane columns), the 3x3 image block of images.py:48-74 is
+//   rows 0,1:  (-A1 - P1 + dzi (R1 - B2)) f_a + d_a [P2 g.d - s3 f.d + B2 f2 + B3 dzi g.d]
+//   row 2:     (-A1 + P1 + 2 dzi B2) f2 + dzi [P2 g.d - s3 f.d] + (B1 + dzi^2 B3) g.d - R1 w
+template <int NF>
+__device__ __forceinline__ void pair_wall(double d0, double d1, double dzf, double dzi, double e2, double h0,
+                                          double hh, const double* f, double* acc)
+{
+    double rho2 = d0 * d0 + d1 * d1;
+    double Qf = rho2 + dzf * dzf + e2;
+    double rf = rsqrt(Qf);
+    double s3f = rf * rf * rf;
+    double h1f = (Qf + e2) * s3f;
+
+    double Qi = rho2 + dzi * dzi + e2;
+    double ri = rsqrt(Qi);
+    double ri2 = ri * ri;
+    double s3 = ri2 * ri;
+    double s5 = s3 * ri2;
+    double e2s5 = e2 * s5;
+    double A1 = ri + e2 * s3;
+    double P1 = -2.0 * hh * (s3 - 3.0 * e2s5);
+    double P2 = 6.0 * hh * s5;
+    double B1 = -2.0 * h0 * (3.0 * e2s5 + s3);
+    double B2 = 2.0 * h0 * s3;
+    double B3 = -6.0 * h0 * s5;
+    double R1 = 6.0 * h0 * e2s5;
+    double cxy = h1f - A1 - P1 + dzi * (R1 - B2);
+    double cz = h1f - A1 + P1 + 2.0 * dzi * B2;
+    double bz = B1 + dzi * dzi * B3;
+    double b3z = B3 * dzi;
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        double f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
+        double w = d0 * f0 + d1 * f1;
+        double tf = s3f * (w + dzf * f2);
+        double gd = dzi * f2 - w;
+        double bp = P2 * gd - s3 * (w + dzi * f2);
+        double bv = bp + B2 * f2 + b3z * gd + tf;
+        acc[3 * r] += cxy * f0 + d0 * bv;
+        acc[3 * r + 1] += cxy * f1 + d1 * bv;
+        acc[3 * r + 2] += cz * f2 + dzf * tf + dzi * bp + bz * gd - R1 * w;
+    }
+}
+
+template <int NF>
+__device__ __forceinline__ void pair_brinkman(double d0, double d1, double d2, double e2, double alpha,
+                                              double inv_alpha, const double* f, double* acc)
+{
+    double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    double Q = r2 + e2;
+    double rs = rsqrt(Q);
+    double z = alpha * (Q * rs);
+    double f1, f2;
+    if (z < 0.5) {
+        brinkman_series(z, f1, f2);
+    } else {
+        double E = exp(-z);
+        double iz = rs * inv_alpha;
+        double iz2 = iz * iz;
+        f1 = E * (1.0 + iz + iz2) - iz2;
+        f2 = 3.0 * iz2 - E * (1.0 + 3.0 * iz + 3.0 * iz2);
+    }
+    double h2 = f2 * (rs * rs * rs);
+    double H1 = f1 * rs + e2 * h2;
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        double fd = f[3 * r] * d0 + f[3 * r + 1] * d1 + f[3 * r + 2] * d2;
+        double t = h2 * fd;
+        acc[3 * r] += H1 * f[3 * r] + t * d0;
+        acc[3 * r + 1] += H1 * f[3 * r + 1] + t * d1;
+        acc[3 * r + 2] += H1 * f[3 * r + 2] + t * d2;
+    }
+}
+
+// ---------------------------------------------------------------- FP32 pairs
+// Displacements are formed in FP64 and rounded once; the Brinkmanlet series
+// branch (z < 0.5, where the closed form cancels) runs in FP64.
+
+template <int NF>
+__device__ __forceinline__ void pair_stokeslet32(float d0, float d1, float d2, float e2, const float* f, float* acc)
+{
+    float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    float Q = r2 + e2;
+    float rs = rsqrtf(Q);
+    float s3 = rs * rs * rs;
+    float h1 = (Q + e2) * s3;
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        float fd = f[3 * r] * d0 + f[3 * r + 1] * d1 + f[3 * r + 2] * d2;
+        float t = s3 * fd;
+        acc[3 * r] += h1 * f[3 * r] + t * d0;
+        acc[3 * r + 1] += h1 * f[3 * r + 1] + t * d1;
+        acc[3 * r + 2] += h1 * f[3 * r + 2] + t * d2;
+    }
+}
+
+template <int NF>
+__device__ __forceinline__ void pair_wall32(float d0, float d1, float dzf, float dzi, float e2, float h0, float hh,
+                                            const float* f, float* acc)
+{
+    float rho2 = d0 * d0 + d1 * d1;
+    float Qf = rho2 + dzf * dzf + e2;
+    float rf = rsqrtf(Qf);
+    float s3f = rf * rf * rf;
+    float h1f = (Qf + e2) * s3f;
+    float Qi = rho2 + dzi * dzi + e2;
+    float ri = rsqrtf(Qi);
+    float ri2 = ri * ri;
+    float s3 = ri2 * ri;
+    float s5 = s3 * ri2;
+    float e2s5 = e2 * s5;
+    float A1 = ri + e2 * s3;
+    float P1 = -2.0f * hh * (s3 - 3.0f * e2s5);
+    float P2 = 6.0f * hh * s5;
+    float B1 = -2.0f * h0 * (3.0f * e2s5 + s3);
+    float B2 = 2.0f * h0 * s3;
+    float B3 = -6.0f * h0 * s5;
+    float R1 = 6.0f * h0 * e2s5;
+    float cxy = h1f - A1 - P1 + dzi * (R1 - B2);
+    float cz = h1f - A1 + P1 + 2.0f * dzi * B2;
+    float bz = B1 + dzi * dzi * B3;
+    float b3z = B3 * dzi;
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        float f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
+        float w = d0 * f0 + d1 * f1;
+        float tf = s3f * (w + dzf * f2);
+        float gd = dzi * f2 - w;
+        float bp = P2 * gd - s3 * (w + dzi * f2);
+        float bv = bp + B2 * f2 + b3z * gd + tf;
+        acc[3 * r] += cxy * f0 + d0 * bv;
+        acc[3 * r + 1] += cxy * f1 + d1 * bv;
+        acc[3 * r + 2] += cz * f2 + dzf * tf + dzi * bp + bz * gd - R1 * w;
+    }
+}
+
+template <int NF>
+__device__ __forceinline__ void pair_brinkman32(double dd0, double dd1, double dd2, float e2, double e2d, float alpha,
+                                                float inv_alpha, double alpha_d, const float* f, float* acc)
+{
+    float d0 = (float)dd0, d1 = (float)dd1, d2 = (float)dd2;
+    float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    float Q = r2 + e2;
+    float rs = rsqrtf(Q);
+    float z = alpha * (Q * rs);
+    float H1, h2;
+    if (z < 0.5f) {
+        // FP64 island: the series and the radial factors in double precision.
+        double Qd = dd0 * dd0 + dd1 * dd1 + dd2 * dd2 + e2d;
+        double rsd = rsqrt(Qd);
+        double f1, f2;
+        brinkman_series(alpha_d * (Qd * rsd), f1, f2);
+        double h2d = f2 * (rsd * rsd * rsd);
+        H1 = (float)(f1 * rsd + e2d * h2d);
+        h2 = (float)h2d;
+    } else {
+        float E = expf(-z);
+        float iz = rs * inv_alpha;
+        float iz2 = iz * iz;
+        float f1 = E * (1.0f + iz + iz2) - iz2;
+        float f2 = 3.0f * iz2 - E * (1.0f + 3.0f * iz + 3.0f * iz2);
+        h2 = f2 * (rs * rs * rs);
+        H1 = f1 * rs + e2 * h2;
+    }
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+        float fd = f[3 * r] * d0 + f[3 * r + 1] * d1 + f[3 * r + 2] * d2;
+        float t = h2 * fd;
+        acc[3 * r] += H1 * f[3 * r] + t * d0;
+        acc[3 * r + 1] += H1 * f[3 * r + 1] + t * d1;
+        acc[3 * r + 2] += H1 * f[3 * r + 2] + t * d2;
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+
+struct NbArgs {
+    const double* tgt;   // (3, M)
+    const double* src;   // (3, N)
+    const double* eps;   // eps[j * eps_stride]
+    const double* f;     // (3, N) strengths (real part)
+    const double* fi;    // (3, N) imaginary part when nf == 2
+    double* u_re;        // (3, M)  (nsplit == 1)
+    double* u_im;        // (3, M) or null
+    double* part;        // (nsplit, nf, 3, M) raw chunk sums (nsplit > 1)
+    int64_t M, N, eps_stride, chunk;
+    double alpha, inv_alpha;
+    double w_re, w_im;   // kernel scale / mu
+    int nsplit;
+};
+
+template <int NF>
+__device__ __forceinline__ void store_result(const NbArgs& a, int64_t i, const double* s)
+{
+    const int64_t M = a.M;
+    if (a.nsplit > 1) {
+        double* p = a.part + (int64_t)blockIdx.y * NF * 3 * M;
+#pragma unroll
+        for (int k = 0; k < 3 * NF; ++k) p[k * M + i] = s[k];
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (NF == 1) {
+            a.u_re[c * M + i] = a.w_re * s[c];
+            if (a.u_im) a.u_im[c * M + i] = a.w_im * s[c];
+        } else {
+            a.u_re[c * M + i] = a.w_re * s[c] - a.w_im * s[3 + c];
+            a.u_im[c * M + i] = a.w_re * s[3 + c] + a.w_im * s[c];
+        }
+    }
+}
+
+// LDS record of one source for the FP64 kernels: x, y, z, eps^2, h0^2, f (3*NF).
+template <int NF>
+struct Src64 {
+    static constexpr int W = (5 + 3 * NF + 1) / 2 * 2;  // doubles, even for 16-byte reads
+};
+
+template <int KIND, int NF, int TPL>
+__global__ __launch_bounds__(WG) void nbody64(NbArgs a)
+{
+    constexpr int W = Src64<NF>::W;
+    __shared__ __attribute__((aligned(16))) double lds[TILE * W];
+    const int tid = threadIdx.x;
+    const int64_t M = a.M, N = a.N;
+
+    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
+        int64_t ic = i < M ? i : M - 1;
+        tx[t] = a.tgt[ic];
+        ty[t] = a.tgt[M + ic];
+        tz[t] = a.tgt[2 * M + ic];
+#pragma unroll
+        for (int k = 0; k < 3 * NF; ++k) acc[t][k] = 0.0;
+    }
+
+    const int64_t jb = (int64_t)blockIdx.y * a.chunk;
+    const int64_t je = min(N, jb + a.chunk);
+    for (int64_t j0 = jb; j0 < je; j0 += TILE) {
+        const int64_t j = j0 + tid;
+        double rec[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) rec[k] = 0.0;
+        if (j < je) {
+            rec[0] = a.src[j];
+            rec[1] = a.src[N + j];
+            rec[2] = a.src[2 * N + j];
+            double e = a.eps[j * a.eps_stride];
+            rec[3] = e * e;
+            rec[4] = rec[2] * rec[2];
+#pragma unroll
+            for (int k = 0; k < 3 * NF; ++k) rec[5 + k] = k < 3 ? a.f[(int64_t)k * N + j] : a.fi[(int64_t)(k - 3) * N + j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < W; k += 2)
+            *reinterpret_cast<double2*>(&lds[tid * W + k]) = make_double2(rec[k], rec[k + 1]);
+        __syncthreads();
+        const int nt = (int)min((int64_t)TILE, je - j0);
+        for (int s = 0; s < nt; ++s) {
+            double r[W];
+#pragma unroll
+            for (int k = 0; k < W; k += 2) {
+                double2 v = *reinterpret_cast<const double2*>(&lds[s * W + k]);
+                r[k] = v.x;
+                r[k + 1] = v.y;
+            }
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+                if (KIND == FL_STOKESLET3D)
+                    pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + 5, acc[t]);
+                else if (KIND == FL_WALL_STOKESLET3D)
+                    pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t] + r[2], r[3], r[2], r[4], r + 5,
+                                  acc[t]);
+                else
+                    pair_brinkman<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], a.alpha, a.inv_alpha, r + 5,
+                                      acc[t]);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
+        if (i < M) store_result<NF>(a, i, acc[t]);
+    }
+}
+
+// LDS record for the FP32C kernels: FP64 coordinates, FP32 constants and strengths.
+template <int NF>
+struct Src32 {
+    double x, y, z;
+    float e2, hh;
+    float f[3 * NF];
+    float pad[(3 * NF + 2) % 4 == 0 ? 0 : 4 - (3 * NF + 2) % 4];
+};
+
+template <int KIND, int NF, int TPL>
+__global__ __launch_bounds__(WG) void nbody32c(NbArgs a)
+{
+    __shared__ Src32<NF> lds[TILE];
+    const int tid = threadIdx.x;
+    const int64_t M = a.M, N = a.N;
+    const float alpha = (float)a.alpha, inv_alpha = (float)a.inv_alpha;
+
+    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
+        int64_t ic = i < M ? i : M - 1;
+        tx[t] = a.tgt[ic];
+        ty[t] = a.tgt[M + ic];
+        tz[t] = a.tgt[2 * M + ic];
+#pragma unroll
+        for (int k = 0; k < 3 * NF; ++k) acc[t][k] = 0.0;
+    }
+    const int64_t jb = (int64_t)blockIdx.y * a.chunk;
+    const int64_t je = min(N, jb + a.chunk);
+    for (int64_t j0 = jb; j0 < je; j0 += TILE) {
+        const int64_t j = j0 + tid;
+        Src32<NF> rec{};
+        if (j < je) {
+            rec.x = a.src[j];
+            rec.y = a.src[N + j];
+            rec.z = a.src[2 * N + j];
+            double e = a.eps[j * a.eps_stride];
+            rec.e2 = (float)(e * e);
+            rec.hh = (float)(rec.z * rec.z);
+#pragma unroll
+            for (int k = 0; k < 3 * NF; ++k)
+                rec.f[k] = (float)(k < 3 ? a.f[(int64_t)k * N + j] : a.fi[(int64_t)(k - 3) * N + j]);
+        }
+        __syncthreads();
+        lds[tid] = rec;
+        __syncthreads();
+        const int nt = (int)min((int64_t)TILE, je - j0);
+        float part[TPL][3 * NF];
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+#pragma unroll
+            for (int k = 0; k < 3 * NF; ++k) part[t][k] = 0.0f;
+        for (int s = 0; s < nt; ++s) {
+            const Src32<NF> r = lds[s];
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+                double dd0 = tx[t] - r.x, dd1 = ty[t] - r.y, dd2 = tz[t] - r.z;
+                if (KIND == FL_STOKESLET3D)
+                    pair_stokeslet32<NF>((float)dd0, (float)dd1, (float)dd2, r.e2, r.f, part[t]);
+                else if (KIND == FL_WALL_STOKESLET3D)
+                    pair_wall32<NF>((float)dd0, (float)dd1, (float)dd2, (float)(tz[t] + r.z), r.e2, (float)r.z,
+                                    r.hh, r.f, part[t]);
+                else
+                    pair_brinkman32<NF>(dd0, dd1, dd2, r.e2, (double)r.e2, alpha, inv_alpha, a.alpha, r.f, part[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+#pragma unroll
+            for (int k = 0; k < 3 * NF; ++k) acc[t][k] += (double)part[t][k];
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
+        if (i < M) store_result<NF>(a, i, acc[t]);
+    }
+}
+
+// Sums the chunk partials in chunk order and applies the scale.
+template <int NF>
+__global__ __launch_bounds__(WG) void nbody_reduce(NbArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
+    const int64_t M = a.M;
+    if (i >= M) return;
+    double s[3 * NF];
+#pragma unroll
+    for (int k = 0; k < 3 * NF; ++k) s[k] = 0.0;
+    for (int y = 0; y < a.nsplit; ++y) {
+        const double* p = a.part + (int64_t)y * NF * 3 * M;
+#pragma unroll
+        for (int k = 0; k < 3 * NF; ++k) s[k] += p[k * M + i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (NF == 1) {
+            a.u_re[c * M + i] = a.w_re * s[c];
+            if (a.u_im) a.u_im[c * M + i] = a.w_im * s[c];
+        } else {
+            a.u_re[c * M + i] = a.w_re * s[c] - a.w_im * s[3 + c];
+            a.u_im[c * M + i] = a.w_re * s[3 + c] + a.w_im * s[c];
+        }
+    }
+}
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+}
+
+// Targets per lane: 1, or 2 when there are enough targets to give every CU
+// two register-blocked workgroups anyway.  FL_NBODY_TPL=1|2 overrides (tuning).
+int pick_tpl(int64_t M)
+{
+    const int forced = env_int("FL_NBODY_TPL", 0);
+    if (forced == 1 || forced == 2) return forced;
+    return M >= (int64_t)WG * 2 * 2 * device_cus() ? 2 : 1;
+}
+
+// Number of source chunks: enough workgroups to fill the device (>= 4 per CU)
+// while every chunk keeps at least 2048 sources.
+int pick_split(int64_t M, int64_t N, int tpl)
+{
+    int64_t wgx = (M + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl);
+    int64_t want = 4LL * device_cus();
+    int64_t s = (want + wgx - 1) / wgx;
+    int64_t smax = std::max<int64_t>(1, N / 2048);
+    s = std::min<int64_t>(s, smax);
+    s = std::min<int64_t>(s, 256);
+    return (int)std::max<int64_t>(1, s);
+}
+
+template <int KIND, int NF, int TPL>
+void launch_main(const NbArgs& a, dim3 grid, int precision, hipStream_t stream)
+{
+    if (precision == FL_FP64)
+        hipLaunchKernelGGL((nbody64<KIND, NF, TPL>), grid, dim3(WG), 0, stream, a);
+    else
+        hipLaunchKernelGGL((nbody32c<KIND, NF, TPL>), grid, dim3(WG), 0, stream, a);
+}
+
+template <int NF, int TPL>
+void launch_kind(int kind, const NbArgs& a, dim3 grid, int precision, hipStream_t stream)
+{
+    if (kind == FL_STOKESLET3D) launch_main<FL_STOKESLET3D, NF, TPL>(a, grid, precision, stream);
+    else if (kind == FL_WALL_STOKESLET3D) launch_main<FL_WALL_STOKESLET3D, NF, TPL>(a, grid, precision, stream);
+    else launch_main<FL_BRINKMANLET3D, NF, TPL>(a, grid, precision, stream);
+}
+
+}  // namespace
+
+double kernel_scale(int kind)
+{
+    const double pi = 3.141592653589793;
+    return kind == FL_BRINKMANLET3D ? 1.0 / (4.0 * pi) : 1.0 / (8.0 * pi);
+}
+
+size_t eval_workspace_size(int64_t M, int64_t N, int nf, int precision)
+{
+    (void)precision;
+    if (M <= 0 || N <= 0) return 0;
+    int tpl = pick_tpl(M);
+    int s = pick_split(M, N, tpl);
+    if (s <= 1) return 0;
+    return (size_t)s * nf * 3 * (size_t)M * sizeof(double);
+}
+
+int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt, const double* src,
+                const double* eps, int64_t eps_stride, const double* f, const double* fi, double w_re, double w_im,
+                double* u_re, double* u_im, int precision, void* ws, size_t ws_bytes, hipStream_t stream)
+{
+    const int nf = fi ? 2 : 1;
+    FL_CHECK(kind >= 0 && kind <= 2, "unknown kernel kind %d", kind);
+    FL_CHECK(precision == FL_FP64 || precision == FL_FP32C, "unknown precision %d", precision);
+    FL_CHECK(M >= 0 && N >= 0, "negative size");
+    FL_CHECK(eps_stride == 0 || eps_stride == 1, "eps_stride must be 0 or 1");
+    FL_CHECK(!(nf == 2 || w_im != 0.0) || u_im != nullptr, "complex result needs u_im");
+    if (M == 0) return FL_OK;
+    if (N == 0) {
+        FL_HIP(hipMemsetAsync(u_re, 0, 3 * M * sizeof(double), stream));
+        if (u_im) FL_HIP(hipMemsetAsync(u_im, 0, 3 * M * sizeof(double), stream));
+        return FL_OK;
+    }
+    FL_CHECK(tgt && src && eps && f && u_re, "null pointer argument");
+    const int tpl = pick_tpl(M);
+    const int nsplit = pick_split(M, N, tpl);
+    const size_t need = eval_workspace_size(M, N, nf, precision);
+    if (ws_bytes < need || (need > 0 && ws == nullptr))
+        return set_error(FL_EWORKSPACE, "fl_eval needs %zu workspace bytes, got %zu", need, ws_bytes);
+
+    NbArgs a;
+    a.tgt = tgt;
+    a.src = src;
+    a.eps = eps;
+    a.f = f;
+    a.fi = fi;
+    a.u_re = u_re;
+    a.u_im = u_im;
+    a.part = static_cast<double*>(ws);
+    a.M = M;
+    a.N = N;
+    a.eps_stride = eps_stride;
+    a.chunk = (N + nsplit - 1) / nsplit;
+    a.chunk = (a.chunk + TILE - 1) / TILE * TILE;
+    a.nsplit = (int)((N + a.chunk - 1) / a.chunk);
+    a.alpha = alpha;
+    a.inv_alpha = alpha > 0.0 ? 1.0 / alpha : 0.0;
+    a.w_re = w_re;
+    a.w_im = w_im;
+    dim3 grid((unsigned)((M + (int64_t)WG * tpl - 1) / ((int64_t)WG * tpl)), (unsigned)a.nsplit);
+    if (nf == 1) {
+        if (tpl == 2) launch_kind<1, 2>(kind, a, grid, precision, stream);
+        else launch_kind<1, 1>(kind, a, grid, precision, stream);
+    } else {
+        if (tpl == 2) launch_kind<2, 2>(kind, a, grid, precision, stream);
+        else launch_kind<2, 1>(kind, a, grid, precision, stream);
+    }
+    FL_LAUNCHED();
+    if (a.nsplit > 1) {
+        dim3 rg((unsigned)((M + WG - 1) / WG));
+        if (nf == 1) hipLaunchKernelGGL(nbody_reduce<1>, rg, dim3(WG), 0, stream, a);
+        else hipLaunchKernelGGL(nbody_reduce<2>, rg, dim3(WG), 0, stream, a);
+        FL_LAUNCHED();
+    }
+    return FL_OK;
+}
+
+}  // namespace fl

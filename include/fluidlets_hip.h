@@ -1,0 +1,141 @@
+/*
+ * fluidlets_hip.h -- C ABI of libfluidlets_hip.so, the MI355X (gfx950) backend
+ * of the Fluidlets regularized-Stokeslet hot path.
+ *
+ * The reference's boundary is a Python plugin API whose hot loops are Numba
+ * functions and whose dense algebra is LAPACK through scipy/numpy.  Each entry
+ * point below replaces one of those calls; the file:line of the call it
+ * replaces (paths relative to the reference checkout) is given with it.
+ *
+ * Conventions (all entry points):
+ *   - Every array argument is a DEVICE pointer owned by the caller.  Nothing
+ *     is allocated by the library on these paths; scratch space comes from a
+ *     caller buffer whose size the matching *_workspace_size() query returns.
+ *   - Point and vector arrays are structure-of-arrays: a (3, N) block holds
+ *     all x, then all y, then all z (index a*N + i).  Dense matrices are
+ *     row-major with a leading dimension, entry (r, c) at r*ld + c, in the
+ *     reference's [point, component] order: row 3i+a, column 3j+b.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = the default
+ *     stream) and the call returns without waiting, except where a function
+ *     says it synchronises.
+ *   - Return value: FL_OK (0) or a negative fl_status.  The message of the
+ *     last failure on the calling thread is returned by fl_get_last_error().
+ *   - Calls on different streams may run concurrently; calls that share a
+ *     workspace buffer must be ordered by the caller.
+ */
+#ifndef FLUIDLETS_HIP_H
+#define FLUIDLETS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FL_ABI_VERSION 1
+
+typedef enum fl_status {
+    FL_OK = 0,
+    FL_EINVAL = -1,     /* invalid argument (ValueError at the Python boundary) */
+    FL_EHIP = -2,       /* HIP runtime error */
+    FL_ENOCONV = -3,    /* iterative solver did not converge (RuntimeError) */
+    FL_ESINGULAR = -4,  /* zero pivot in a factorization */
+    FL_EWORKSPACE = -5  /* workspace too small */
+} fl_status;
+
+/* Kernel families: src/fluidlets/kernels/{stokeslet3d,images,brinkmanlet3d}.py */
+typedef enum fl_kind {
+    FL_STOKESLET3D = 0,      /* kernels/stokeslet3d.py:53-64 */
+    FL_WALL_STOKESLET3D = 1, /* kernels/images.py:92-104 (free Stokeslet + no-slip images) */
+    FL_BRINKMANLET3D = 2     /* kernels/brinkmanlet3d.py:107-141 */
+} fl_kind;
+
+typedef enum fl_precision {
+    FL_FP64 = 0,             /* all pair arithmetic and accumulation in FP64 */
+    FL_FP32C = 1             /* FP32 pair arithmetic with FP64 displacements, FP64 Brinkmanlet
+                                series branch, FP32 partial sums carried in FP64 per source tile */
+} fl_precision;
+
+typedef struct fl_kernel {
+    int32_t kind;            /* fl_kind */
+    int32_t reserved;
+    double alpha;            /* Brinkmanlet inverse screening length, >= 0; ignored otherwise */
+} fl_kernel;
+
+/* ABI version (FL_ABI_VERSION). */
+int fl_version(void);
+
+/* Copies the last error message of the calling thread into buf (NUL-terminated,
+ * truncated to n bytes); returns the status of that failure (FL_OK if none). */
+int fl_get_last_error(char* buf, size_t n);
+
+/* Number of compute units of the current device (for launch sizing by callers). */
+int fl_device_cus(void);
+
+/* ------------------------------------------------------------------ kernel sums
+ * u_i = (1/mu) sum_j G(x_i, y_j; eps_j) f_j  for i < M.
+ * Replaces the Numba loop _apply called at src/fluidlets/core/matvec.py:48
+ * (loop body core/matvec.py:10-21, wrapper core/matvec.py:24-49).
+ *
+ *   tgt   (3, M)   target points          src (3, N) source points
+ *   eps   eps[j * eps_stride]; eps_stride = 0 broadcasts one blob width
+ *   f_re  (3, N)   real part of the strengths
+ *   f_im  (3, N)   imaginary part, or NULL for real strengths
+ *   mu_re, mu_im   viscosity (complex viscosity when mu_im != 0)
+ *   u_re  (3, M)   real part of the velocities (overwritten)
+ *   u_im  (3, M)   imaginary part; required when f_im or mu_im != 0, else may be NULL
+ *   precision      fl_precision
+ *   workspace      device buffer of at least fl_eval_workspace_size() bytes
+ *                  (may be NULL when that size is 0)
+ * Sources are summed in index order inside fixed source chunks whose partial
+ * sums are combined in chunk order, so results are deterministic and do not
+ * depend on other work in flight.
+ */
+size_t fl_eval_workspace_size(int64_t M, int64_t N, int complex_strengths, int precision);
+
+int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N,
+            const double* tgt, const double* src, const double* eps, int64_t eps_stride,
+            const double* f_re, const double* f_im, double mu_re, double mu_im,
+            double* u_re, double* u_im, int precision,
+            void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------ assembly
+ * K[3i+a, 3j+b] = G_ab(x_i, y_j; eps_j) / mu,  i < M, j < N.
+ * Replaces _fill called at src/fluidlets/core/assemble.py:45 (loop
+ * core/assemble.py:15-24, wrapper 27-46).  Any of K64 (real FP64), K32 (real
+ * FP32 copy, same layout, leading dimension ld32) and Kc (complex128,
+ * interleaved re/im, leading dimension ld in complex elements) may be NULL;
+ * all requested outputs are written in the same pass.  K64 and K32 hold
+ * G / mu_re and require mu_im == 0; Kc holds G / (mu_re + i mu_im).
+ */
+int fl_assemble(const fl_kernel* kernel, int64_t M, int64_t N,
+                const double* tgt, const double* src, const double* eps, int64_t eps_stride,
+                double mu_re, double mu_im,
+                double* K64, int64_t ld, float* K32, int64_t ld32, double* Kc, void* stream);
+
+/* Nearest-neighbour (Gallagher-Smith) assembly:
+ * K[3i+a, 3*nearest[k]+b] = sum_k G_ab(x_i, X_k; eps_k) / mu.
+ * Replaces _fill_nearest called at src/fluidlets/core/assemble.py:100 (loop
+ * core/assemble.py:49-58, wrapper 68-101).  The quadrature points must be
+ * grouped by force point: order[] lists quadrature indices so that group j is
+ * order[start[j] .. start[j+1]-1] (start has n_force+1 entries).  Each output
+ * entry is written once, as the in-order sum over its group (no atomics).
+ * quad is (3, Q).  K64 real FP64 (ld), optional K32 copy (ld32).
+ */
+int fl_assemble_nearest(const fl_kernel* kernel, int64_t M, int64_t Q, int64_t n_force,
+                        const double* tgt, const double* quad, const double* eps, int64_t eps_stride,
+                        const int64_t* order, const int64_t* start, double mu_re,
+                        double* K64, int64_t ld, float* K32, int64_t ld32, void* stream);
+
+/* Nearest force point of each quadrature point (squared Euclidean distance,
+ * lowest index on ties): nearest[k] = argmin_j |quad_k - force_j|, quad (3, Q),
+ * force (3, N), nearest (Q) int64.  Replaces the scipy cKDTree query of
+ * src/fluidlets/core/assemble.py:61-65 (nearest_neighbors). */
+int fl_nearest(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* nearest, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDLETS_HIP_H */

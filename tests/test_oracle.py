@@ -1,0 +1,279 @@
+"""Pins of the CPU oracle (no GPU).  The oracle is checked against the
+reference's golden data, against closed forms, and against the NumPy per-pair
+restatement, so that a dropped term, a wrong sign or a swapped index fails at
+least one of them.  Mirrors the assertions of the reference tests named in
+each docstring."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fluidlets_amd.geometry import cube_sphere, fibonacci_sphere
+from helpers import blake, eps_rates, singular_brinkmanlet, singular_stokeslet
+
+rng = np.random.default_rng(7)
+
+
+def above_wall(n):
+    return np.c_[rng.uniform(-1, 1, (n, 2)), rng.uniform(0.2, 1.5, n)]
+
+
+# ---------------------------------------------------------------- golden data
+
+def test_stokeslet_matches_legacy_matlab(kernels_mat):
+    """tests/legacy/test_phase1_port.py:33-36 (per-source deltas)."""
+    d = kernels_mat
+    u = O.evaluate("stokeslet", d["targets"], d["sources"], d["strengths"], d["deltas"])
+    assert np.allclose(u, d["V_stokeslet"], rtol=1e-13, atol=1e-15)
+
+
+def test_wall_matches_legacy_matlab(kernels_mat):
+    """tests/legacy/test_phase1_port.py:39-44."""
+    d = kernels_mat
+    u = O.evaluate("wall", d["targets"], d["sources"], d["strengths"], d["deltas"])
+    assert np.allclose(u, d["V_stokeslet"] + d["V_image"], rtol=1e-13, atol=1e-15)
+    u = O.evaluate("wall", d["wall"], d["sources"], d["strengths"], d["deltas"])
+    assert np.allclose(u, d["V_wall_total"], atol=1e-15)
+    assert np.abs(d["V_wall_free"]).max() > 0.1
+
+
+def test_numpy_image_form_matches_legacy_matlab(kernels_mat):
+    """The matrix-form image restatement reproduces MATLAB ImageEval.m."""
+    d = kernels_mat
+    v = O.pairwise_sum(O.image_velocity, d["targets"], d["sources"], d["strengths"], d["deltas"])
+    assert np.allclose(v, d["V_image"], rtol=1e-13, atol=1e-15)
+
+
+def test_tower_operator_equals_StokesSelf(towers_mat):
+    """tests/legacy/test_phase1_port.py:54-59: the wall matvec applied to the
+    legacy GMRES forces gives MATLAB's residual to 1e-12."""
+    d = towers_mat
+    r = np.linalg.norm(O.evaluate("wall", d["X"], d["X"], d["F"], 0.5) - d["U"]) / np.linalg.norm(d["U"])
+    assert abs(r - d["resid"]) < 1e-12
+
+
+def test_reproduces_2010_cuda_velocity_grid(towers_mat):
+    """tests/legacy/test_phase1_port.py:77-87 (float32 CUDA output, 1e-6)."""
+    from conftest import golden
+    d = towers_mat
+    xs, ys, zs = np.linspace(-1, 8, 32), np.linspace(-5, 25, 32), np.linspace(0, 20, 32)
+    Z, Y, Xg = np.meshgrid(zs, ys, xs, indexing="ij")
+    grid = np.c_[Xg.ravel(), Y.ravel(), Z.ravel()]
+    V = O.evaluate("wall", grid, d["X"], d["F"], 0.5)
+    V[:, 0] += 0.1 * grid[:, 2]
+    stored = golden("legacy_grid.npz")["velocity"].astype(np.float64).reshape(-1, 3)
+    assert np.linalg.norm(V - stored) / np.linalg.norm(stored) < 1e-6
+
+
+def test_sphere_drag_matches_legacy_gmres():
+    """tests/legacy/matlab/check_sphere_drag.m: MATLAB GMRES (1e-10) drag on a
+    Fibonacci sphere with eps = h/2, free space (N = 100..800) and near the wall
+    (N = 400).  Dense LU on the oracle must agree to the GMRES tolerance."""
+    from conftest import golden
+    s = golden("legacy_sphere_drag.npz")
+    for n, F_ref in zip(s["N_list"].astype(int), s["F_free"]):
+        X = fibonacci_sphere(n)
+        h = np.sqrt(4 * np.pi / n)
+        f, info = O.resistance("stokeslet", X, np.tile([1.0, 0, 0], (n, 1)), 0.5 * h)
+        assert info.residual < 1e-12
+        assert abs(f[:, 0].sum() / F_ref - 1) < 1e-7
+    n, epsw = 400, float(s["epsw"])
+    for H, Fpar, Fperp in zip(s["H_list"], s["F_par"], s["F_perp"]):
+        X = fibonacci_sphere(n, 1.0, (0, 0, H))
+        f, _ = O.resistance("wall", X, np.tile([1.0, 0, 0], (n, 1)), epsw)
+        assert abs(f[:, 0].sum() / Fpar - 1) < 1e-7
+        f, _ = O.resistance("wall", X, np.tile([0, 0, 1.0], (n, 1)), epsw)
+        assert abs(f[:, 2].sum() / Fperp - 1) < 1e-7
+
+
+# ---------------------------------------------------------------- C loops vs NumPy forms
+
+@pytest.mark.parametrize("name,vel", [("stokeslet", O.stokeslet_velocity), ("wall", O.wall_velocity)])
+def test_c_loops_match_numpy_forms(name, vel):
+    """tests/test_kernels.py:16-21."""
+    X, X0, F = above_wall(20), above_wall(15), rng.normal(size=(15, 3))
+    eps = rng.uniform(0.05, 0.3, 15)
+    ref = O.pairwise_sum(vel, X, X0, F, eps)
+    assert np.allclose(O.evaluate(name, X, X0, F, eps), ref, rtol=1e-12, atol=1e-14)
+
+
+def test_image_block_matches_matrix_form():
+    """tests/test_kernels.py:24-29."""
+    for _ in range(10):
+        x, x0, f = above_wall(1)[0], above_wall(1)[0], rng.normal(size=3)
+        G = O.image_block(x, x0, 0.1)
+        assert np.allclose(G @ f, O.image_velocity(x, x0, f, 0.1), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("alpha", [0.5, 3.0, 20.0])
+def test_brinkman_c_loop_matches_numpy_form(alpha):
+    """tests/test_brinkmanlet.py:21-27."""
+    X, X0, F = rng.normal(size=(20, 3)), rng.normal(size=(12, 3)), rng.normal(size=(12, 3))
+    eps = rng.uniform(0.05, 0.2, 12)
+    ref = O.pairwise_sum(lambda x, x0, f, e: O.brinkman_velocity(alpha, x, x0, f, e), X, X0, F, eps)
+    assert np.allclose(O.evaluate(f"brinkman:{alpha}", X, X0, F, eps), ref, rtol=1e-12, atol=1e-15)
+
+
+def test_assemble_matches_evaluate():
+    """tests/test_kernels.py:32-36."""
+    for name in ["stokeslet", "wall", "brinkman:2.5"]:
+        X, F = above_wall(12), rng.normal(size=(12, 3))
+        K = O.assemble(name, X, X, 0.2)
+        assert np.allclose(K @ F.ravel(), O.evaluate(name, X, X, F, 0.2).ravel(), rtol=1e-13)
+
+
+def test_nearest_assembly():
+    """tests/test_swimming.py:12-26."""
+    X = cube_sphere(4)
+    nn = O.nearest_neighbors(X, X)
+    assert np.array_equal(nn, np.arange(len(X)))
+    assert np.allclose(O.assemble_nearest("stokeslet", X, X, nn, len(X), 0.1),
+                       O.assemble("stokeslet", X, X, 0.1), rtol=1e-14)
+    x, Xq = cube_sphere(3), cube_sphere(9)
+    nn = O.nearest_neighbors(Xq, x)
+    A = O.assemble_nearest("stokeslet", x, Xq, nn, len(x), 0.05)
+    f = np.random.default_rng(0).normal(size=(len(x), 3))
+    direct = O.evaluate("stokeslet", x, Xq, f[nn], 0.05)
+    assert np.allclose(A @ f.ravel(), direct.ravel(), rtol=1e-12)
+
+
+def test_complex_viscosity():
+    """tests/test_kernels.py:45-52."""
+    X, F = above_wall(5), rng.normal(size=(5, 3))
+    u1 = O.evaluate("stokeslet", X, X, F, 0.1)
+    mu = 2.0 - 0.5j
+    u = O.evaluate("stokeslet", X, X, F, 0.1, mu=mu)
+    assert u.dtype == np.complex128 and np.allclose(u, u1 / mu)
+    Fc = F + 1j * rng.normal(size=F.shape)
+    uc = O.evaluate("wall", X, X, Fc, 0.1, mu=mu)
+    assert np.allclose(uc, (O.evaluate("wall", X, X, Fc.real, 0.1) + 1j * O.evaluate("wall", X, X, Fc.imag, 0.1)) / mu)
+    assert O.assemble("stokeslet", X, X, 0.1, mu=mu).dtype == np.complex128
+
+
+# ---------------------------------------------------------------- closed forms
+
+def test_brinkman_series_and_closed_forms_match_high_precision():
+    """tests/test_brinkmanlet.py:30-38 (values computed at 50 digits by mpmath)."""
+    from conftest import golden
+    g = golden("brinkman_mp.npz")
+    for z, e1, e2 in zip(g["z"], g["f1"], g["f2"]):
+        assert abs(O.brink_f1(z) / e1 - 1) < 1e-14, z
+        assert abs(O.brink_f2(z) / e2 - 1) < 1e-14, z
+
+
+def test_brinkman_alpha_zero_is_stokeslet():
+    """tests/test_brinkmanlet.py:14-18."""
+    X, X0, F = rng.normal(size=(25, 3)), rng.normal(size=(15, 3)), rng.normal(size=(15, 3))
+    assert np.allclose(O.evaluate("brinkman:0", X, X0, F, 0.1), O.evaluate("stokeslet", X, X0, F, 0.1),
+                       rtol=1e-13, atol=1e-15)
+
+
+def test_brinkman_far_field_and_convergence():
+    """tests/test_brinkmanlet.py:70-84."""
+    x, f, a = np.array([0.6, -0.3, 0.4]), np.array([1.0, 0.5, -0.2]), 3.0
+    exact = singular_brinkmanlet(x, f, a)
+    err = [np.abs(O.evaluate(f"brinkman:{a}", [x], [[0, 0, 0]], [f], e)[0] - exact).max()
+           for e in [0.08, 0.04, 0.02, 0.01]]
+    assert np.all(eps_rates(err) > 1.9)
+    a, f = 2.0, np.array([0.3, -1.0, 0.5])
+    x = np.array([12.0, 9.0, -8.0])
+    r = np.linalg.norm(x)
+    far = (-f / r**3 + 3 * (f @ x) * x / r**5) / (4 * np.pi * a**2)
+    assert np.allclose(O.evaluate(f"brinkman:{a}", [x], [[0, 0, 0]], [f], 0.01)[0], far, rtol=1e-6)
+
+
+def test_stokeslet_far_field_and_convergence():
+    """tests/test_stokeslet3d.py:5-10 and tests/test_kernels.py:76-83."""
+    x, f = np.array([3.0, 1.0, -2.0]), np.array([1.0, -0.5, 2.0])
+    assert np.allclose(O.evaluate("stokeslet", [x], [[0, 0, 0]], [f], 1e-6)[0], singular_stokeslet(x, f), rtol=1e-8)
+    x, f = np.array([0.6, -0.3, 0.4]), np.array([1.0, 0.5, -0.2])
+    err = [np.abs(O.evaluate("stokeslet", [x], [[0, 0, 0]], [f], e)[0] - singular_stokeslet(x, f)).max()
+           for e in [0.08, 0.04, 0.02, 0.01]]
+    assert np.all(eps_rates(err) > 1.9)
+
+
+@pytest.mark.parametrize("name", ["stokeslet", "wall", "brinkman:4.0"])
+def test_divergence_free(name):
+    """tests/test_stokeslet3d.py:13-22, tests/test_brinkmanlet.py:53-59."""
+    src = rng.normal(size=(5, 3))
+    src[:, 2] = np.abs(src[:, 2]) + 0.3
+    F = rng.normal(size=(5, 3))
+    x0, h = np.array([0.3, -0.2, 0.6]), 1e-5
+    div = sum((O.evaluate(name, [x0 + h * e], src, F, 0.2)[0, i] - O.evaluate(name, [x0 - h * e], src, F, 0.2)[0, i])
+              / (2 * h) for i, e in enumerate(np.eye(3)))
+    assert abs(div) < 1e-6
+
+
+def test_wall_converges_to_blake_and_vanishes_on_wall():
+    """tests/legacy/test_phase1_port.py:90-96 and tests/test_kernels.py:39-42."""
+    x0, x, f = np.array([0.3, -0.2, 1.0]), np.array([1.2, 0.4, 0.6]), np.array([0.3, -1.0, 0.7])
+    exact = blake(x, x0, f)
+    err = [np.abs(O.evaluate("wall", [x], [x0], [f], e)[0] - exact).max() for e in [0.1, 0.05, 0.025, 0.0125]]
+    assert np.all(eps_rates(err) > 1.9)
+    for fe in np.eye(3):
+        assert np.abs(blake([1.1, 0.7, 0.0], x0, fe)).max() < 1e-12
+    wall = np.c_[rng.uniform(-2, 2, (50, 2)), np.zeros(50)]
+    u = O.evaluate("wall", wall, above_wall(10), rng.normal(size=(10, 3)), 0.1)
+    assert np.abs(u).max() < 1e-14
+
+
+# ---------------------------------------------------------------- solves and physics
+
+def test_sphere_drag_and_brinkman_sphere():
+    """tests/test_kernels.py:61-73 and tests/test_brinkmanlet.py:87-104."""
+    n = 400
+    X = fibonacci_sphere(n)
+    eps = 0.5 * np.sqrt(4 * np.pi / n)
+    f, info = O.resistance("stokeslet", X, np.tile([1.0, 0, 0], (n, 1)), eps)
+    assert info.residual < 1e-12 and 1.0 < f[:, 0].sum() / (6 * np.pi) < 1.02
+    fg, ig = O.resistance("stokeslet", X, np.tile([1.0, 0, 0], (n, 1)), eps, method="gmres")
+    assert ig.residual < 1e-9 and np.allclose(f, fg, rtol=1e-6, atol=1e-8)
+    alpha, N = 1.0, 1500
+    X = fibonacci_sphere(N)
+    eps = 0.4 * np.sqrt(4 * np.pi / N)
+    ez = np.array([0.0, 0.0, 1.0])
+    F, _ = O.resistance(f"brinkman:{alpha}", X, np.tile(ez, (N, 1)), eps)
+    assert np.isclose(F[:, 2].sum(), 6 * np.pi * (1 + alpha + alpha**2 / 3), rtol=0.01)
+    W, _ = O.resistance(f"brinkman:{alpha}", X, np.cross(ez, X), eps)
+    r = np.array([1.5, 2.0, 3.0])
+    u = O.evaluate(f"brinkman:{alpha}", np.c_[r, np.zeros(3), np.zeros(3)], X, W, eps)
+    exact = np.exp(alpha * (1 - r)) * (1 + alpha * r) / (r**3 * (1 + alpha))
+    assert np.allclose(u[:, 1] / r, exact, rtol=0.03)
+
+
+def test_swimmer_exact_rigid_solutions():
+    """tests/test_swimming.py:34-55."""
+    x, Xq = cube_sphere(4), cube_sphere(10)
+    V = np.array([0.2, -0.1, 0.7])
+    U, W, f, _ = O.swimming_problem("stokeslet", x, -np.tile(V, (len(x), 1)), Xq, np.zeros(3), np.eye(3), 0.05)
+    assert np.allclose(U, V, atol=1e-12) and np.allclose(W, 0, atol=1e-12) and np.abs(f).max() < 1e-10
+    Wv = np.array([0.0, 0.3, 1.0])
+    B = np.linalg.qr(np.random.default_rng(1).normal(size=(3, 3)))[0]
+    B *= np.sign(np.linalg.det(B))
+    x0 = np.array([1.0, 2.0, -1.0])
+    U, W, f, _ = O.swimming_problem("stokeslet", x, -np.cross(B.T @ Wv, x), Xq, x0, B, 0.05)
+    assert np.allclose(W, Wv, atol=1e-12) and np.allclose(U, 0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- planted mistakes
+
+def test_planted_mistakes_fail_the_pins(kernels_mat):
+    """A sign error in the rotlet or a dropped dipole term of the NumPy image
+    form fails the MATLAB fixture that the oracle passes."""
+    d = kernels_mat
+
+    def image_bad(x, x0, f, eps, mu=1.0, which="rotlet"):
+        good = O.image_velocity(x, x0, f, eps, mu)
+        xs = np.asarray(x, float) - np.array([x0[0], x0[1], -x0[2]])
+        Q = xs @ xs + eps**2
+        h0 = x0[2]
+        if which == "rotlet":
+            Hr1 = 3 * h0 * eps**2 / Q**2.5
+            rot = Hr1 * (xs[2] * np.diag([1.0, 1.0, 0.0]) - np.outer([0, 0, 1.0], [xs[0], xs[1], 0.0]))
+            return good - 2 * rot @ f / (4 * np.pi)
+        dip = (-h0**2 * (1 / Q**1.5 - 3 * eps**2 / Q**2.5)) * np.diag([-1.0, -1.0, 1.0])
+        return good - dip @ f / (4 * np.pi)
+
+    for which in ["rotlet", "dipole"]:
+        v = O.pairwise_sum(lambda x, x0, f, e: image_bad(x, x0, f, e, which=which), d["targets"], d["sources"],
+                           d["strengths"], d["deltas"])
+        assert not np.allclose(v, d["V_image"], rtol=1e-13, atol=1e-15)
