@@ -9,7 +9,7 @@ by ``python -m fluidlets_amd.build``) called through ctypes; PyTorch-ROCm
 tensors hold device memory.  There is no CPU fallback.
 """
 from .kernels import Brinkmanlet3D, Kernel, Stokeslet3D, VectorKernel, WallStokeslet3D
-from .core import assemble, assemble_nearest, evaluate, nearest_neighbors
+from .core import assemble, assemble_nearest, evaluate, grand_resistance, nearest_neighbors, resistance
 from .stokeslet3d import stokeslet3d_velocity
 
 __version__ = "0.1.0"
@@ -22,4 +22,5 @@ __all__ = [
     "assemble_nearest",
     "nearest_neighbors",
     "evaluate",
+    "resistance",
 ]

@@ -60,6 +60,27 @@ int assemble_nearest_launch(int kind, double alpha, int64_t M, int64_t Q, int64_
 int nearest_launch(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* out,
                    hipStream_t stream);
 
+size_t getrf_aux_bytes(int64_t n, int precision);
+size_t getrf_workspace_bytes(int64_t n, int precision);
+size_t getrs_workspace_bytes(int64_t n, int64_t nrhs, int precision);
+size_t refine_workspace_bytes(int64_t n, int64_t nrhs);
+int sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
+           hipStream_t st);
+int dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
+           hipStream_t st);
+int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb, void* ws,
+           size_t wsb, hipStream_t st);
+int dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb, void* ws,
+           size_t wsb, hipStream_t st);
+int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux, int64_t n,
+                const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
+                double* resid_out, void* ws, size_t ws_bytes, hipStream_t st);
+
+size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision);
+int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double* eps, int64_t eps_stride,
+               double mu_re, const double* U, double* f, double rtol, int restart, int maxiter, int inner_precision,
+               int* iters_out, double* resid_out, void* ws, size_t ws_bytes, hipStream_t st);
+
 }  // namespace fl
 
 using namespace fl;
@@ -138,6 +159,69 @@ int fl_assemble_nearest(const fl_kernel* kernel, int64_t M, int64_t Q, int64_t n
     if (int s = check_kernel(kernel)) return s;
     return assemble_nearest_launch(kernel->kind, kernel->alpha, M, Q, n_force, tgt, quad, eps, eps_stride, order,
                                    start, mu_re, K64, ld, K32, ld32, as_stream(stream));
+}
+
+size_t fl_getrf_aux_size(int64_t n, int precision) { return getrf_aux_bytes(n, precision); }
+
+size_t fl_getrf_workspace_size(int64_t n, int precision) { return getrf_workspace_bytes(n, precision); }
+
+int fl_sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fallback_panels,
+              void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return sgetrf(A, n, lda, ipiv, aux, info, fallback_panels, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fallback_panels,
+              void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return dgetrf(A, n, lda, ipiv, aux, info, fallback_panels, workspace, workspace_bytes, as_stream(stream));
+}
+
+size_t fl_getrs_workspace_size(int64_t n, int64_t nrhs, int precision)
+{
+    return getrs_workspace_bytes(n, nrhs, precision);
+}
+
+int fl_sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb,
+              void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return sgetrs(LU, n, lda, aux, B, nrhs, ldb, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb,
+              void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return dgetrs(LU, n, lda, aux, B, nrhs, ldb, workspace, workspace_bytes, as_stream(stream));
+}
+
+size_t fl_refine_workspace_size(int64_t n, int64_t nrhs) { return refine_workspace_bytes(n, nrhs); }
+
+int fl_refine(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32, int64_t n,
+              const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
+              double* resid_out, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return refine_impl(A64, lda64, LU32, lda32, aux32, n, b, x, nrhs, ldb, tol, max_iter, iters_out, resid_out,
+                       workspace, workspace_bytes, as_stream(stream));
+}
+
+size_t fl_gmres_workspace_size(int64_t N, int restart, int inner_precision)
+{
+    return gmres_workspace_bytes(N, restart, inner_precision);
+}
+
+int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* eps, int64_t eps_stride, double mu_re,
+             const double* U, double* f, double rtol, int restart, int maxiter, int inner_precision, int* iters_out,
+             double* resid_out, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    return gmres_impl(kernel->kind, kernel->alpha, N, X, eps, eps_stride, mu_re, U, f, rtol, restart, maxiter,
+                      inner_precision, iters_out, resid_out, workspace, workspace_bytes, as_stream(stream));
 }
 
 int fl_nearest(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* nearest, void* stream)

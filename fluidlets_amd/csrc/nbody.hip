@@ -64,6 +64,17 @@ __device__ __forceinline__ void brinkman_series(double z, double& f1, double& f2
     f2 = s2;
 }
 
+// 1/sqrt(x) for finite x > 0 (here x = r^2 + eps^2 >= eps^2 > 0): the
+// hardware estimate y0 (v_rsq_f64) refined by one second-order step of the
+// expansion (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + ..., e = 1 - x y0^2, which
+// takes the estimate's relative error d to O(d^3).  No special-value handling.
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    double y0 = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y0, y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
 // ---------------------------------------------------------------- FP64 pairs
 // acc[3r + a] += (G f_r)_a in units of the per-kernel scale.
 
@@ -73,7 +84,7 @@ __device__ __forceinline__ void pair_stokeslet(double d0, double d1, double d2, 
 {
     double r2 = d0 * d0 + d1 * d1 + d2 * d2;
     double Q = r2 + e2;
-    double rs = rsqrt(Q);
+    double rs = rsqrt_pos(Q);
     double s3 = rs * rs * rs;
     double h1 = (Q + e2) * s3;  // (r^2 + 2 eps^2) / Q^{3/2}
 #pragma unroll
@@ -100,12 +111,12 @@ __device__ __forceinline__ void pair_wall(double d0, double d1, double dzf, doub
 {
     double rho2 = d0 * d0 + d1 * d1;
     double Qf = rho2 + dzf * dzf + e2;
-    double rf = rsqrt(Qf);
+    double rf = rsqrt_pos(Qf);
     double s3f = rf * rf * rf;
     double h1f = (Qf + e2) * s3f;
 
     double Qi = rho2 + dzi * dzi + e2;
-    double ri = rsqrt(Qi);
+    double ri = rsqrt_pos(Qi);
     double ri2 = ri * ri;
     double s3 = ri2 * ri;
     double s5 = s3 * ri2;
@@ -141,7 +152,7 @@ __device__ __forceinline__ void pair_brinkman(double d0, double d1, double d2, d
 {
     double r2 = d0 * d0 + d1 * d1 + d2 * d2;
     double Q = r2 + e2;
-    double rs = rsqrt(Q);
+    double rs = rsqrt_pos(Q);
     double z = alpha * (Q * rs);
     double f1, f2;
     if (z < 0.5) {
@@ -240,7 +251,7 @@ __device__ __forceinline__ void pair_brinkman32(double dd0, double dd1, double d
     if (z < 0.5f) {
         // FP64 island: the series and the radial factors in double precision.
         double Qd = dd0 * dd0 + dd1 * dd1 + dd2 * dd2 + e2d;
-        double rsd = rsqrt(Qd);
+        double rsd = rsqrt_pos(Qd);
         double f1, f2;
         brinkman_series(alpha_d * (Qd * rsd), f1, f2);
         double h2d = f2 * (rsd * rsd * rsd);

@@ -1,0 +1,115 @@
+"""Dense LU factorizations and solves on the device (fl_?getrf, fl_?getrs,
+fl_refine).  Device memory is torch tensors; the arithmetic is the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _device as D
+from ._lib import FL_FP32C, FL_FP64, check, lib
+
+
+@dataclass
+class LUFactor:
+    """Factors of P A = L U kept on the device, with what the solves need.
+
+    ``lu`` (n x n, float32 or float64) holds L (unit lower) and U; ``ipiv`` the
+    LAPACK interchanges; ``aux`` the row permutation and the inverted diagonal
+    blocks; ``info`` > 0 flags an exactly zero pivot (1-based column);
+    ``general_panels`` counts panels that needed the all-rows pivot search."""
+    lu: object
+    ipiv: object
+    aux: object
+    n: int
+    info: int = 0
+    general_panels: int = 0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def precision(self) -> int:
+        return FL_FP64 if str(self.lu.dtype) == "torch.float64" else FL_FP32C
+
+    def nbytes(self) -> int:
+        return int(self.lu.numel() * self.lu.element_size() + self.aux.numel())
+
+
+def getrf(A, overwrite: bool = True, stream=None) -> LUFactor:
+    """Factor an (n, n) float32/float64 device tensor in place (or a copy)."""
+    t = D.torch()
+    if A.dim() != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("getrf needs a square matrix")
+    if A.dtype not in (t.float32, t.float64):
+        raise ValueError("getrf needs float32 or float64")
+    if not overwrite:
+        A = A.clone()
+    if not A.is_contiguous():
+        A = A.contiguous()
+    n = int(A.shape[0])
+    prec = FL_FP64 if A.dtype == t.float64 else FL_FP32C
+    L = lib()
+    ipiv = t.empty((max(n, 1),), dtype=t.int32, device=A.device)
+    aux = t.empty((max(int(L.fl_getrf_aux_size(n, prec)), 1),), dtype=t.uint8, device=A.device)
+    nbytes = int(L.fl_getrf_workspace_size(n, prec))
+    ws = D.workspace(nbytes, stream)
+    info = ctypes.c_int(0)
+    fb = ctypes.c_int(0)
+    fn = L.fl_dgetrf if prec == FL_FP64 else L.fl_sgetrf
+    check(fn(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), ctypes.byref(info), ctypes.byref(fb), D.ptr(ws), nbytes,
+             D.stream_handle(stream)), "getrf")
+    return LUFactor(A, ipiv, aux, n, int(info.value), int(fb.value))
+
+
+def getrs(F: LUFactor, B, stream=None):
+    """Solve A X = B in place; B is an (n, nrhs) or (n,) device tensor of the
+    factor's dtype.  Right-hand sides are processed 8 at a time."""
+    L = lib()
+    vec = B.dim() == 1
+    B2 = B.view(-1, 1) if vec else B
+    if not B2.is_contiguous():
+        raise ValueError("B must be contiguous")
+    n, nrhs = int(B2.shape[0]), int(B2.shape[1])
+    if n != F.n:
+        raise ValueError("B has the wrong number of rows")
+    if B2.dtype != F.lu.dtype:
+        raise ValueError("B must have the factor's dtype")
+    prec = F.precision
+    fn = L.fl_dgetrs if prec == FL_FP64 else L.fl_sgetrs
+    for c0 in range(0, nrhs, 8):
+        q = min(8, nrhs - c0)
+        nbytes = int(L.fl_getrs_workspace_size(n, q, prec))
+        ws = D.workspace(nbytes, stream)
+        blk = B2[:, c0:c0 + q]
+        check(fn(D.ptr(F.lu), n, n, D.ptr(F.aux), blk.data_ptr(), q, nrhs, D.ptr(ws), nbytes,
+                 D.stream_handle(stream)), "getrs")
+    return B
+
+
+def refine(A64, F32: LUFactor, b, tol: float = 1e-10, max_iter: int = 30, stream=None):
+    """x solving A64 x = b (FP64) from the FP32 factors by iterative
+    refinement with FP64 residuals.  b: (n, nrhs) float64 device tensor.
+    Returns (x, iterations, per-column final relative residuals)."""
+    t = D.torch()
+    L = lib()
+    b2 = b.view(-1, 1) if b.dim() == 1 else b
+    b2 = b2.contiguous()
+    n, nrhs = int(b2.shape[0]), int(b2.shape[1])
+    x = t.empty_like(b2)
+    iters, resid = [], []
+    for c0 in range(0, nrhs, 8):
+        q = min(8, nrhs - c0)
+        bb = b2[:, c0:c0 + q].contiguous()
+        xx = t.empty_like(bb)
+        nbytes = int(L.fl_refine_workspace_size(n, q))
+        ws = D.workspace(nbytes, stream)
+        it = ctypes.c_int(0)
+        res = (ctypes.c_double * 8)()
+        st = L.fl_refine(D.ptr(A64), n, D.ptr(F32.lu), n, D.ptr(F32.aux), n, D.ptr(bb), D.ptr(xx), q, q, float(tol),
+                         int(max_iter), ctypes.byref(it), res, D.ptr(ws), nbytes, D.stream_handle(stream))
+        check(st, "refine")
+        x[:, c0:c0 + q] = xx
+        iters.append(int(it.value))
+        resid.extend(res[:q])
+    return (x.view(-1) if b.dim() == 1 else x), max(iters), np.array(resid)

@@ -134,6 +134,78 @@ int fl_assemble_nearest(const fl_kernel* kernel, int64_t M, int64_t Q, int64_t n
  * src/fluidlets/core/assemble.py:61-65 (nearest_neighbors). */
 int fl_nearest(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* nearest, void* stream);
 
+/* ------------------------------------------------------------------ dense LU
+ * Blocked right-looking LU with partial pivoting, P A = L U, in place
+ * (row-major, n x n, leading dimension lda >= n).  Replaces the LAPACK getrf
+ * inside scipy.linalg.solve at src/fluidlets/core/solve.py:66 and inside
+ * numpy.linalg.solve at src/fluidlets/swimming.py:101.
+ *   ipiv   n int32 (device): row r was interchanged with row ipiv[r] (0-based,
+ *          LAPACK order)
+ *   aux    fl_getrf_aux_size() bytes (device), written here and read by the
+ *          solves: the folded row permutation and the inverted diagonal blocks
+ *   info   host int: 0, or k+1 when U[k][k] is exactly zero (LAPACK convention;
+ *          the factorization still completes)
+ *   general_panels  host int (nullable): panels whose pivots had to be searched
+ *          over all rows below the diagonal (0 when the block-restricted search
+ *          already found the partial-pivoting pivots)
+ * Diagonal blocks of 128 are factored in LDS; inv(L11) A12, A21 inv(U11)
+ * and the trailing updates run on MFMA (v_mfma_f32_32x32x2_f32 for
+ * fl_sgetrf, v_mfma_f64_16x16x4_f64 for fl_dgetrf).  Synchronises on stream.
+ */
+size_t fl_getrf_aux_size(int64_t n, int precision);
+size_t fl_getrf_workspace_size(int64_t n, int precision);
+int fl_sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* general_panels,
+              void* workspace, size_t workspace_bytes, void* stream);
+int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* general_panels,
+              void* workspace, size_t workspace_bytes, void* stream);
+
+/* Solves A X = B with the factors of fl_?getrf; B is n x nrhs row-major
+ * (leading dimension ldb, nrhs <= 8), overwritten by X.  Replaces the getrs
+ * inside scipy.linalg.solve (src/fluidlets/core/solve.py:66). */
+size_t fl_getrs_workspace_size(int64_t n, int64_t nrhs, int precision);
+int fl_sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb,
+              void* workspace, size_t workspace_bytes, void* stream);
+int fl_dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb,
+              void* workspace, size_t workspace_bytes, void* stream);
+
+/* Mixed-precision solve of A64 x = b: FP32 factors (fl_sgetrf of the FP32
+ * copy of A64) with FP64 iterative refinement,
+ *   x <- x + s LU32^{-1} (r / s),  r = b - A64 x in FP64,  s = max |r| per column,
+ * until every column's ||r||_2 / ||b||_2 <= tol, or max_iter corrections.
+ * b and x are n x nrhs row-major FP64 (ldb, nrhs <= 8).  iters_out (host)
+ * receives the number of corrections, resid_out (host, nrhs doubles) the final
+ * relative residuals.  Returns FL_ENOCONV if tol was not reached.
+ * Replaces scipy.linalg.solve at src/fluidlets/core/solve.py:66 for FP64
+ * results.  Synchronises on stream.
+ */
+size_t fl_refine_workspace_size(int64_t n, int64_t nrhs);
+int fl_refine(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32, int64_t n,
+              const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
+              double* resid_out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------ GMRES
+ * Restarted GMRES(restart) for  sum_j G(x_i, x_j) f_j / mu_re = U_i  with the
+ * matrix-free kernel sum as the operator (real mu).  Replaces
+ * scipy.sparse.linalg.gmres over the evaluate LinearOperator at
+ * src/fluidlets/core/solve.py:68-79, with the same stopping rule
+ * (||U - A f|| <= rtol ||U||, atol = 0), `restart` inner iterations per cycle
+ * and at most `maxiter` cycles.
+ *   X (3, N) points, U (3, N) right-hand side, f (3, N) initial guess on
+ *   entry and solution on exit (device, SoA).
+ *   inner_precision FL_FP64: plain GMRES.  FL_FP32C: flexible GMRES whose
+ *   preconditioner is an inner GMRES cycle with FP32C kernel sums; outer
+ *   matvecs and residuals in FP64.
+ *   iters_out (host): total outer inner-iterations (one per Arnoldi step, the
+ *   count the reference's pr_norm callback gives); resid_out (host): final
+ *   true relative residual.  Returns FL_ENOCONV when not converged (the
+ *   reference raises RuntimeError).  Synchronises on stream.
+ */
+size_t fl_gmres_workspace_size(int64_t N, int restart, int inner_precision);
+int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* eps, int64_t eps_stride,
+             double mu_re, const double* U, double* f, double rtol, int restart, int maxiter,
+             int inner_precision, int* iters_out, double* resid_out,
+             void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,193 @@
+"""Dense LU (fl_dgetrf/fl_sgetrf + solves + FP64 refinement) and GMRES on the
+GPU, against LAPACK (scipy) and the oracle's restatement of core/solve.py.
+
+Bars: FP64 solves match LAPACK to 1e-12 relative on well-conditioned systems
+(with LAPACK's pivots reproduced exactly); mixed-precision solves reach
+<= 1e-10 relative error against the FP64 LU oracle; GMRES matches LU to the
+reference tests' 1e-6 with residual < 1e-9.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle as O
+import fluidlets_amd as fl
+from conftest import golden
+from fluidlets_amd.geometry import fibonacci_sphere
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("n", [1, 7, 128, 129, 300, 777])
+def test_dgetrf_matches_lapack_with_pivoting(n):
+    """Random Gaussian matrices need row interchanges everywhere: the
+    factors and pivots equal LAPACK's getrf."""
+    from fluidlets_amd import linalg
+    print(f"case: dgetrf random n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, 3))
+    F = linalg.getrf(dev(A.copy()))
+    lu_ref, piv_ref = sl.lu_factor(A)
+    assert F.info == 0
+    assert np.array_equal(F.ipiv.cpu().numpy()[:n], piv_ref)
+    assert rel_err(F.lu.cpu().numpy(), lu_ref) < 1e-12
+    x = dev(b.copy())
+    linalg.getrs(F, x)
+    assert rel_err(x.cpu().numpy(), sl.lu_solve((lu_ref, piv_ref), b)) < 1e-11
+
+
+def test_sgetrf_is_a_backward_stable_partial_pivoting_lu():
+    """FP32 factors of a random matrix: P A = L U to FP32 accuracy, the pivots of
+    LAPACK sgetrf (rounding-order differences may swap a near-tie), |L| <= 1."""
+    from fluidlets_amd import linalg
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((300, 300)).astype(np.float32)
+    F = linalg.getrf(dev(A.copy()))
+    lu = F.lu.cpu().numpy().astype(np.float64)
+    piv = F.ipiv.cpu().numpy()
+    _, piv_ref = sl.lu_factor(A)
+    assert (piv == piv_ref).mean() > 0.98
+    Lm = np.tril(lu, -1) + np.eye(300)
+    Um = np.triu(lu)
+    PA = A.astype(np.float64).copy()
+    for r, p in enumerate(piv):
+        PA[[r, p]] = PA[[p, r]]
+    assert np.linalg.norm(PA - Lm @ Um) / np.linalg.norm(A) < 1e-5
+    assert np.abs(np.tril(lu, -1)).max() <= 1.0 + 1e-6
+
+
+@pytest.mark.parametrize("name,kernel", [("wall", fl.WallStokeslet3D()), ("brinkman:5.0", fl.Brinkmanlet3D(5.0))])
+def test_resistance_matrix_takes_the_block_pivot_path(name, kernel):
+    """Resistance matrices: the block-restricted pivot search is verified to be
+    partial pivoting (no general panels, LAPACK's pivots)."""
+    from fluidlets_amd import linalg
+    print(f"case: resistance matrix LU {name}", flush=True)
+    n = 700
+    X = fibonacci_sphere(n, 1.0, (0, 0, 1.5))
+    eps = 0.5 * np.sqrt(4 * np.pi / n)
+    K = O.assemble(name, X, X, eps)
+    F = linalg.getrf(dev(K.copy()))
+    lu_ref, piv_ref = sl.lu_factor(K)
+    assert F.general_panels == 0
+    assert np.array_equal(F.ipiv.cpu().numpy(), piv_ref)
+    assert rel_err(F.lu.cpu().numpy(), lu_ref) < 1e-12
+    Kg = fl.assemble(kernel, X, X, eps)
+    assert rel_err(Kg, K) < 1e-13
+
+
+def test_singular_matrix_reports_info():
+    from fluidlets_amd import linalg
+    A = np.eye(200)
+    A[150, 150] = 0.0
+    F = linalg.getrf(dev(A))
+    assert F.info == 151
+    X = fibonacci_sphere(100, 1.0, (0, 0, 1.5))
+    X[:5, 2] = 0.0                 # points on the wall: zero rows and columns
+    with pytest.raises(np.linalg.LinAlgError):
+        fl.resistance(fl.WallStokeslet3D(), X, np.tile([1.0, 0, 0], (100, 1)), 0.05, method="lu")
+
+
+def test_mixed_precision_refinement_reaches_1e_10():
+    from fluidlets_amd import linalg
+    import torch
+    print("case: sgetrf + refine", flush=True)
+    n = 1500
+    X = fibonacci_sphere(n, 1.0, (0, 0, 1.5))
+    eps = 0.5 * np.sqrt(4 * np.pi / n)
+    K = O.assemble("wall", X, X, eps)
+    b = np.random.default_rng(3).standard_normal((3 * n, 6))
+    xref = sl.solve(K, b)
+    K64 = dev(K)
+    F32 = linalg.getrf(K64.to(torch.float32))
+    x, steps, res = linalg.refine(K64, F32, dev(b), tol=1e-12)
+    assert steps <= 6 and res.max() <= 1e-12
+    assert rel_err(x.cpu().numpy(), xref) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resistance_lu_matches_oracle_and_legacy_drag(precision):
+    """tests/test_kernels.py:61-73 and the MATLAB sphere drags (sphere_drag.mat)."""
+    print(f"case: resistance lu {precision}", flush=True)
+    s = golden("legacy_sphere_drag.npz")
+    for n, F_ref in zip(s["N_list"].astype(int), s["F_free"]):
+        X = fibonacci_sphere(n)
+        eps = 0.5 * np.sqrt(4 * np.pi / n)
+        U = np.tile([1.0, 0, 0], (n, 1))
+        f, info = fl.resistance(fl.Stokeslet3D(), X, U, eps, method="lu", precision=precision)
+        fr, _ = O.resistance("stokeslet", X, U, eps)
+        assert info.residual < 1e-12 and rel_err(f, fr) < 1e-10
+        assert abs(f[:, 0].sum() / F_ref - 1) < 1e-7
+    n, epsw = 400, float(s["epsw"])
+    for H, Fperp in zip(s["H_list"], s["F_perp"]):
+        X = fibonacci_sphere(n, 1.0, (0, 0, H))
+        f, info = fl.resistance(fl.WallStokeslet3D(), X, np.tile([0, 0, 1.0], (n, 1)), epsw, precision=precision)
+        assert info.method == "lu" and abs(f[:, 2].sum() / Fperp - 1) < 1e-7
+
+
+def test_resistance_complex_viscosity():
+    X = fibonacci_sphere(200)
+    U = np.tile([1.0, 0, 0], (200, 1))
+    f1, _ = fl.resistance(fl.Stokeslet3D(), X, U, 0.1, method="lu")
+    mu = 2.0 - 0.5j
+    f, info = fl.resistance(fl.Stokeslet3D(), X, U, 0.1, mu=mu, method="lu")
+    assert f.dtype == np.complex128 and rel_err(f, mu * f1) < 1e-12
+    fr, _ = O.resistance("stokeslet", X, U, 0.1, mu=mu)
+    assert rel_err(f, fr) < 1e-10 and info.residual < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gmres_matches_lu(precision):
+    """tests/test_kernels.py:61-73: GMRES residual < 1e-9 and LU ~ GMRES to 1e-6."""
+    print(f"case: gmres {precision}", flush=True)
+    n = 400
+    X = fibonacci_sphere(n)
+    eps = 0.5 * np.sqrt(4 * np.pi / n)
+    U = np.tile([1.0, 0, 0], (n, 1))
+    F_lu, info_lu = fl.resistance(fl.Stokeslet3D(), X, U, eps, method="lu")
+    F_gm, info_gm = fl.resistance(fl.Stokeslet3D(), X, U, eps, method="gmres", precision=precision)
+    assert info_lu.residual < 1e-12 and info_gm.residual < 1e-9
+    assert np.allclose(F_lu, F_gm, rtol=1e-6, atol=1e-8)
+    assert 1.0 < F_lu[:, 0].sum() / (6 * np.pi) < 1.02
+
+
+def test_gmres_reproduces_legacy_tower_forces(towers_mat):
+    """tests/legacy/test_phase1_port.py:62-74 (wall points stay force-free)."""
+    print("case: gmres towers", flush=True)
+    d = towers_mat
+    X, U, F_legacy = d["X"], d["U"], d["F"]
+    F, info = fl.resistance(fl.WallStokeslet3D(), X, U, 0.5, method="gmres")
+    assert info.residual < 1e-9
+    on_wall = X[:, 2] == 0
+    assert np.abs(F[on_wall]).max() < 1e-12
+    above = ~on_wall
+    assert np.linalg.norm(F[above] - F_legacy[above]) / np.linalg.norm(F_legacy[above]) < 1e-3
+    for k in range(3):
+        tower = slice(147 * k, 147 * (k + 1))
+        assert np.isclose(F[tower, 0].sum(), F_legacy[tower, 0].sum(), rtol=1e-5)
+
+
+def test_gmres_nonconvergence_raises():
+    X = fibonacci_sphere(300)
+    with pytest.raises(RuntimeError):
+        fl.resistance(fl.Stokeslet3D(), X, np.tile([1.0, 0, 0], (300, 1)), 0.05, method="gmres", restart=2,
+                      maxiter=1)
+
+
+def test_grand_resistance_six_modes():
+    """Sphere: T = 6 pi I-ish and R = 8 pi I-ish; mixed precision agrees with FP64 to 1e-10."""
+    n = 1200
+    X = fibonacci_sphere(n)
+    eps = 0.5 * np.sqrt(4 * np.pi / n)
+    R64, f64, _ = fl.grand_resistance(fl.Stokeslet3D(), X, (0, 0, 0), eps)
+    Rm, fm, info = fl.grand_resistance(fl.Stokeslet3D(), X, (0, 0, 0), eps, precision="mixed")
+    assert np.allclose(np.diag(R64)[:3] / (6 * np.pi), 1, atol=0.02)
+    assert np.allclose(np.diag(R64)[3:] / (8 * np.pi), 1, atol=0.03)
+    assert rel_err(Rm, R64) < 1e-10 and rel_err(fm, f64) < 1e-10
+    assert info["refinement_steps"] >= 1
