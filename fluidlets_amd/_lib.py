@@ -50,6 +50,10 @@ _SIGNATURES = {
     "fl_gmres": (c_int, [POINTER(fl_kernel), c_int64, c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p,
                          c_double, c_int, c_int, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t,
                          c_void_p]),
+    "fl_sgemm": (c_int, [c_int64, c_int64, c_int64, c_float, c_void_p, c_int64, c_void_p, c_int64, c_float,
+                         c_void_p, c_int64, c_void_p]),
+    "fl_dgemm": (c_int, [c_int64, c_int64, c_int64, c_double, c_void_p, c_int64, c_void_p, c_int64, c_double,
+                         c_void_p, c_int64, c_void_p]),
 }
 
 _lib = None

@@ -81,6 +81,10 @@ int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double*
                double mu_re, const double* U, double* f, double rtol, int restart, int maxiter, int inner_precision,
                int* iters_out, double* resid_out, void* ws, size_t ws_bytes, hipStream_t st);
 
+template <typename T>
+int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
+                T* C, int64_t ldc, unsigned int* absmax_bits, hipStream_t stream);
+
 }  // namespace fl
 
 using namespace fl;
@@ -222,6 +226,22 @@ int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* 
     if (int s = check_kernel(kernel)) return s;
     return gmres_impl(kernel->kind, kernel->alpha, N, X, eps, eps_stride, mu_re, U, f, rtol, restart, maxiter,
                       inner_precision, iters_out, resid_out, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_sgemm(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+             float beta, float* C, int64_t ldc, void* stream)
+{
+    clear_error();
+    FL_CHECK(lda >= K && ldb >= N && ldc >= N, "leading dimension too small");
+    return gemm_launch<float>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, as_stream(stream));
+}
+
+int fl_dgemm(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
+             int64_t ldb, double beta, double* C, int64_t ldc, void* stream)
+{
+    clear_error();
+    FL_CHECK(lda >= K && ldb >= N && ldc >= N, "leading dimension too small");
+    return gemm_launch<double>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, as_stream(stream));
 }
 
 int fl_nearest(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* nearest, void* stream)

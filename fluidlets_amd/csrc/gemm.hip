@@ -55,10 +55,10 @@ struct Mma<double> {
     __device__ static acc_t zero() { return acc_t{0}; }
     __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     // Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator i
-    // holds D[4 (l / 16) + i][l % 16].
+    // holds D[4 i + l / 16][l % 16].
     __device__ static int a_row(int l) { return l % 16; }
     __device__ static int a_k(int l) { return l / 16; }
-    __device__ static int d_row(int l, int i) { return 4 * (l / 16) + i; }
+    __device__ static int d_row(int l, int i) { return 4 * i + l / 16; }
     __device__ static int d_col(int l) { return l % 16; }
 };
 

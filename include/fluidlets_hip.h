@@ -159,6 +159,16 @@ int fl_sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* i
 int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* general_panels,
               void* workspace, size_t workspace_bytes, void* stream);
 
+/* C = alpha A B + beta C, row-major (M x K, K x N, M x N), on MFMA with FP32
+ * (fl_sgemm) or FP64 (fl_dgemm) accumulation; C is not read when beta == 0.
+ * The LU's update kernel.  It has no counterpart call in the reference (there
+ * it is the gemm inside LAPACK getrf); exported for tests and for callers
+ * composing their own blocked algorithms. */
+int fl_sgemm(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+             float beta, float* C, int64_t ldc, void* stream);
+int fl_dgemm(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
+             int64_t ldb, double beta, double* C, int64_t ldc, void* stream);
+
 /* Solves A X = B with the factors of fl_?getrf; B is n x nrhs row-major
  * (leading dimension ldb, nrhs <= 8), overwritten by X.  Replaces the getrs
  * inside scipy.linalg.solve (src/fluidlets/core/solve.py:66). */
