@@ -77,6 +77,117 @@ def cpu_baseline(X, F, n_targets: int):
                       f"core/matvec.py:_apply with the reference's pow()-based block, -O3, no fast-math"}
 
 
+def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=1):
+    """Mixed-precision wall resistance solve: Fibonacci sphere (radius 1,
+    centre (0,0,1.5)), eps = 0.5 sqrt(4 pi / N), six rigid-body right-hand
+    sides.  Times FP32-MFMA LU + FP64 refinement to 1e-10, the FP64 LU of this
+    library and the FP64 LU of rocSOLVER (torch.linalg.lu_factor), and reports
+    each 6x6 grand-resistance entry's error against the FP64 LU."""
+    from fluidlets_amd import linalg
+    from fluidlets_amd.core.assemble import assemble_device
+    from fluidlets_amd.core.solve import rigid_modes
+    from fluidlets_amd.geometry import fibonacci_sphere
+
+    def grand(X, c, x):
+        f = x.reshape(len(X), 3, 6).transpose(2, 0, 1)
+        r = X - np.asarray(c)
+        R = np.zeros((6, 6))
+        for m in range(6):
+            R[:3, m] = f[m].sum(axis=0)
+            R[3:, m] = np.cross(r, f[m]).sum(axis=0)
+        return R
+
+    out = {}
+    N = n_points
+    n = 3 * N
+    h = np.sqrt(4 * np.pi / N)
+    for name in kernels:
+        if name == "wall":
+            K, center = fl.WallStokeslet3D(), (0.0, 0.0, 1.5)
+        else:
+            K, center = fl.Brinkmanlet3D(float(name.split(":")[1])), (0.0, 0.0, 0.0)
+        X = fibonacci_sphere(N, 1.0, center)
+        dev = D.device()
+        x_soa = D.soa(X, dev)
+        eps_t, stride = D.eps_arg(0.5 * h, N, dev)
+        rhs = torch.from_numpy(rigid_modes(X, center)).to(dev)
+        ev = lambda: torch.cuda.Event(enable_timing=True)
+        # assembly: FP64 + FP32 copy in one pass (HBM-write bound)
+        assemble_device(K, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=True)
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record()
+        K64, K32 = assemble_device(K, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=True)
+        e1.record()
+        torch.cuda.synchronize()
+        asm_ms = e0.elapsed_time(e1)
+        row = {"n": n, "assembly_ms": asm_ms, "assembly_GBps": 108.0 * N * N / (asm_ms * 1e-3) / 1e9}
+        # plain FP64 LU (this library): factor + 6 solves
+        best64 = None
+        for _ in range(repeats):
+            A = K64.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            F = linalg.getrf(A)
+            x64 = rhs.clone()
+            linalg.getrs(F, x64)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best64 = dt if best64 is None else min(best64, dt)
+            del A
+        row["fp64_lu_s"] = best64
+        row["fp64_general_panels"] = F.general_panels
+        x64h = x64.cpu().numpy()
+        del F
+        # mixed: FP32 MFMA LU + FP64 refinement (the FP32 copy is refactored each repeat)
+        bestm = None
+        for _ in range(repeats):
+            A32 = K32.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            F32 = linalg.getrf(A32)
+            xm, steps, res = linalg.refine(K64, F32, rhs, tol=1e-10)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            bestm = dt if bestm is None else min(bestm, dt)
+            t1 = time.perf_counter()
+            xm2, _, _ = linalg.refine(K64, F32, rhs, tol=1e-10)
+            torch.cuda.synchronize()
+            row["mixed_resolve_s"] = time.perf_counter() - t1
+            del F32, A32
+        row["mixed_lu_refine_s"] = bestm
+        row["refinement_steps"] = steps
+        row["refinement_residual_max"] = float(res.max())
+        xmh = xm.cpu().numpy()
+        row["mixed_vs_fp64_force_relerr"] = float(np.linalg.norm(xmh - x64h) / np.linalg.norm(x64h))
+        R64, Rm = grand(X, center, x64h), grand(X, center, xmh)
+        row["grand_resistance_fp64"] = R64.round(6).tolist()
+        row["grand_resistance_max_entry_relerr"] = float(np.abs(Rm - R64).max() / np.abs(R64).max())
+        row["speedup_vs_fp64_lu"] = best64 / bestm
+        # rocSOLVER FP64 (torch.linalg) as the library baseline
+        try:
+            A = K64.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            LUt, piv = torch.linalg.lu_factor(A)
+            xr = torch.linalg.lu_solve(LUt, piv, rhs)
+            torch.cuda.synchronize()
+            row["rocsolver_fp64_lu_s"] = time.perf_counter() - t0
+            row["speedup_vs_rocsolver_fp64"] = row["rocsolver_fp64_lu_s"] / bestm
+            row["rocsolver_vs_fp64_force_relerr"] = float(torch.linalg.norm(xr - x64) / torch.linalg.norm(x64))
+            del A, LUt, xr
+        except Exception as exc:  # report, do not fail the bench
+            row["rocsolver_error"] = repr(exc)[:200]
+        flops = 2.0 * n**3 / 3
+        row["fp64_lu_tflops"] = flops / best64 / 1e12
+        row["mixed_lu_tflops_equiv"] = flops / bestm / 1e12
+        row["solves_per_s_mixed"] = 1.0 / bestm
+        out[name] = row
+        del K64, K32
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +196,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-targets", type=int, default=4096)
     ap.add_argument("--no-table", action="store_true", help="skip the other kernels of the workload")
+    ap.add_argument("--no-solve", action="store_true", help="skip the N=8192 resistance-solve section")
+    ap.add_argument("--solve-kernels", default="wall", help="comma list: wall, brinkman:1, brinkman:5, ...")
+    ap.add_argument("--solve-points", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -174,6 +288,10 @@ def main():
                     row["rel_err_vs_fp64"] = (torch.linalg.norm(uu - ref[name]) / torch.linalg.norm(ref[name])).item()
                 table[f"{name}/{prec}"] = row
         result["c2_table"] = table
+
+    if not args.no_solve:
+        result["wall_sphere_solve"] = c3_solves(torch, fl, D, tuple(args.solve_kernels.split(",")),
+                                                args.solve_points)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(X, F, args.cpu_targets)

@@ -1,11 +1,17 @@
-// MFMA GEMM for the dense LU: C = beta C + alpha A B, all row-major.
+// MFMA GEMM for the dense LU: C = alpha A B + beta C, all row-major.
 //
 // 128 x 128 output tile per 256-lane workgroup (4 wave64 in a 2 x 2 grid, each
 // wave 64 x 64).  FP32 uses v_mfma_f32_32x32x2_f32 (2 x 2 blocks of 32 x 32
 // per wave, K tile 32); FP64 uses v_mfma_f64_16x16x4_f64 (4 x 4 blocks of
 // 16 x 16 per wave, K tile 16).  Tiles are staged through LDS from coalesced
-// loads (consecutive lanes read consecutive columns) with the next tile
-// prefetched into registers while the current one is multiplied.
+// 16-byte loads (scalar, bounds-checked loads on edge tiles or unaligned
+// operands), with the next tile prefetched into registers while the current
+// one is multiplied.
+//
+// The accumulators start from (beta / alpha) C, loaded in the MFMA output
+// layout before the K loop, so the epilogue is stores only:
+// C = alpha * (A B + (beta / alpha) C).  (For the LU's alpha = -1, beta = 1
+// the scaling is exact.)  alpha == 0 is handled as C = beta C.
 //
 // A workgroup reads all of K for its tile before it writes C, so the call is
 // safe in place when the C tile is the A tile (C = A B with N <= 128 and
@@ -33,7 +39,9 @@ struct Mma<float> {
     static constexpr int KS = 2;    // K per instruction
     static constexpr int NB = 2;    // blocks per wave edge (64 / B)
     static constexpr int NACC = 16; // accumulator values per lane per block
+    static constexpr int VW = 4;    // elements per 16-byte load
     using acc_t = floatx16;
+    using vec_t = float4;
     __device__ static acc_t zero() { return acc_t{0}; }
     __device__ static acc_t mma(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0); }
     // Lane l supplies A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator i
@@ -42,6 +50,13 @@ struct Mma<float> {
     __device__ static int a_k(int l) { return l / 32; }
     __device__ static int d_row(int l, int i) { return 8 * (i / 4) + 4 * (l / 32) + (i % 4); }
     __device__ static int d_col(int l) { return l % 32; }
+    __device__ static void unpack(const vec_t& v, float* o)
+    {
+        o[0] = v.x;
+        o[1] = v.y;
+        o[2] = v.z;
+        o[3] = v.w;
+    }
 };
 
 template <>
@@ -51,7 +66,9 @@ struct Mma<double> {
     static constexpr int KS = 4;
     static constexpr int NB = 4;
     static constexpr int NACC = 4;
+    static constexpr int VW = 2;
     using acc_t = doublex4;
+    using vec_t = double2;
     __device__ static acc_t zero() { return acc_t{0}; }
     __device__ static acc_t mma(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     // Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator i
@@ -60,6 +77,11 @@ struct Mma<double> {
     __device__ static int a_k(int l) { return l / 16; }
     __device__ static int d_row(int l, int i) { return 4 * i + l / 16; }
     __device__ static int d_col(int l) { return l % 16; }
+    __device__ static void unpack(const vec_t& v, double* o)
+    {
+        o[0] = v.x;
+        o[1] = v.y;
+    }
 };
 
 template <typename T>
@@ -73,45 +95,85 @@ struct GemmArgs {
     T* C;
     int64_t ldc;
     unsigned int* absmax_bits;
+    int aligned;  // A, B base and leading dimensions allow 16-byte loads
 };
 
 template <typename T>
-__global__ __launch_bounds__(WG) void gemm_kernel(GemmArgs<T> g)
+__global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
 {
     using P = Mma<T>;
     constexpr int KT = P::KT;
-    constexpr int LA = TM + 1;  // padded LDS row (A is stored k-major)
-    constexpr int EA = TM * KT / WG;  // A elements per lane per tile
+    constexpr int VW = P::VW;
+    constexpr int LA = TM + 1;         // padded LDS row (A is stored k-major)
+    constexpr int EA = TM * KT / WG;   // A elements per lane per tile
     constexpr int EB = TN * KT / WG;
+    constexpr int VA = EA / VW;        // 16-byte loads per lane per tile
+    constexpr int VB = EB / VW;
+    constexpr int AV_ROW = KT / VW;    // vectors per A tile row
+    constexpr int BV_ROW = TN / VW;    // vectors per B tile row
     __shared__ T As[KT * LA];
-    __shared__ T Bs[KT * TN];
+    __shared__ __attribute__((aligned(16))) T Bs[KT * TN];
 
     const int tid = threadIdx.x;
     const int lane = tid % 64, wave = tid / 64;
     const int wm = wave / 2, wn = wave % 2;
     const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
+    const bool full_mn = g.aligned && m0 + TM <= g.M && n0 + TN <= g.N;
 
     typename P::acc_t acc[P::NB][P::NB];
+    const bool use_c = g.beta != T(0) && g.alpha != T(0);
+    const T cscale = use_c ? g.beta / g.alpha : T(0);
 #pragma unroll
     for (int i = 0; i < P::NB; ++i)
 #pragma unroll
-        for (int j = 0; j < P::NB; ++j) acc[i][j] = P::zero();
+        for (int j = 0; j < P::NB; ++j) {
+            acc[i][j] = P::zero();
+            if (use_c) {
+#pragma unroll
+                for (int v = 0; v < P::NACC; ++v) {
+                    int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
+                    int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
+                    acc[i][j][v] = (r < g.M && c < g.N) ? cscale * g.C[r * g.ldc + c] : T(0);
+                }
+            }
+        }
 
     T ra[EA], rb[EB];
     auto load = [&](int64_t k0) {
+        if (full_mn && k0 + KT <= g.K) {
 #pragma unroll
-        for (int e = 0; e < EA; ++e) {
-            int idx = e * WG + tid;
-            int r = idx / KT, c = idx % KT;
-            int64_t gr = m0 + r, gc = k0 + c;
-            ra[e] = (gr < g.M && gc < g.K) ? g.A[gr * g.lda + gc] : T(0);
-        }
+            for (int e = 0; e < VA; ++e) {
+                int idx = e * WG + tid;
+                int r = idx / AV_ROW, c = (idx % AV_ROW) * VW;
+                P::unpack(*reinterpret_cast<const typename P::vec_t*>(g.A + (m0 + r) * g.lda + k0 + c), ra + e * VW);
+            }
 #pragma unroll
-        for (int e = 0; e < EB; ++e) {
-            int idx = e * WG + tid;
-            int r = idx / TN, c = idx % TN;
-            int64_t gr = k0 + r, gc = n0 + c;
-            rb[e] = (gr < g.K && gc < g.N) ? g.Bm[gr * g.ldb + gc] : T(0);
+            for (int e = 0; e < VB; ++e) {
+                int idx = e * WG + tid;
+                int r = idx / BV_ROW, c = (idx % BV_ROW) * VW;
+                P::unpack(*reinterpret_cast<const typename P::vec_t*>(g.Bm + (k0 + r) * g.ldb + n0 + c), rb + e * VW);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < VA; ++e) {
+                int idx = e * WG + tid;
+                int r = idx / AV_ROW, c = (idx % AV_ROW) * VW;
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    int64_t gr = m0 + r, gc = k0 + c + q;
+                    ra[e * VW + q] = (gr < g.M && gc < g.K) ? g.A[gr * g.lda + gc] : T(0);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < VB; ++e) {
+                int idx = e * WG + tid;
+                int r = idx / BV_ROW, c = (idx % BV_ROW) * VW;
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    int64_t gr = k0 + r, gc = n0 + c + q;
+                    rb[e * VW + q] = (gr < g.K && gc < g.N) ? g.Bm[gr * g.ldb + gc] : T(0);
+                }
+            }
         }
     };
 
@@ -120,14 +182,20 @@ __global__ __launch_bounds__(WG) void gemm_kernel(GemmArgs<T> g)
     for (int64_t kt = 0; kt < nkt; ++kt) {
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < EA; ++e) {
+        for (int e = 0; e < VA; ++e) {
             int idx = e * WG + tid;
-            As[(idx % KT) * LA + idx / KT] = ra[e];
+            int r = idx / AV_ROW, c = (idx % AV_ROW) * VW;
+#pragma unroll
+            for (int q = 0; q < VW; ++q) As[(c + q) * LA + r] = ra[e * VW + q];
         }
 #pragma unroll
-        for (int e = 0; e < EB; ++e) {
+        for (int e = 0; e < VB; ++e) {
             int idx = e * WG + tid;
-            Bs[(idx / TN) * TN + idx % TN] = rb[e];
+            int r = idx / BV_ROW, c = (idx % BV_ROW) * VW;
+            typename P::vec_t v;
+            if constexpr (VW == 4) v = make_float4(rb[e * 4], rb[e * 4 + 1], rb[e * 4 + 2], rb[e * 4 + 3]);
+            else v = make_double2(rb[e * 2], rb[e * 2 + 1]);
+            *reinterpret_cast<typename P::vec_t*>(&Bs[r * TN + c]) = v;
         }
         __syncthreads();
         if (kt + 1 < nkt) load((kt + 1) * KT);
@@ -146,6 +214,7 @@ __global__ __launch_bounds__(WG) void gemm_kernel(GemmArgs<T> g)
         }
     }
 
+    const T outscale = g.alpha != T(0) ? g.alpha : T(1);
     float local_max = 0.0f;
 #pragma unroll
     for (int i = 0; i < P::NB; ++i)
@@ -156,10 +225,8 @@ __global__ __launch_bounds__(WG) void gemm_kernel(GemmArgs<T> g)
                 int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
                 int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
                 if (r < g.M && c < g.N) {
-                    T* p = g.C + r * g.ldc + c;
-                    T val = g.alpha * acc[i][j][v];
-                    if (g.beta != T(0)) val += g.beta * *p;
-                    *p = val;
+                    T val = outscale * acc[i][j][v];
+                    g.C[r * g.ldc + c] = val;
                     local_max = fmaxf(local_max, (float)fabs((double)val));
                 }
             }
@@ -170,6 +237,16 @@ __global__ __launch_bounds__(WG) void gemm_kernel(GemmArgs<T> g)
     }
 }
 
+// C = beta C (the alpha == 0 case)
+template <typename T>
+__global__ __launch_bounds__(WG) void scale_kernel(T* C, int64_t ldc, int64_t M, int64_t N, T beta)
+{
+    const int64_t idx = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (idx >= M * N) return;
+    T* p = C + (idx / N) * ldc + idx % N;
+    *p = beta == T(0) ? T(0) : beta * *p;
+}
+
 }  // namespace
 
 template <typename T>
@@ -178,7 +255,16 @@ int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t ld
 {
     FL_CHECK(M >= 0 && N >= 0 && K >= 0, "negative GEMM size");
     if (M == 0 || N == 0) return FL_OK;
-    GemmArgs<T> g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits};
+    if (alpha == T(0)) {
+        hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)((M * N + WG - 1) / WG)), dim3(WG), 0, stream, C, ldc, M,
+                           N, beta);
+        FL_LAUNCHED();
+        return FL_OK;
+    }
+    const int vw = 16 / (int)sizeof(T);
+    const int aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
+                        (lda % vw == 0) && (ldb % vw == 0);
+    GemmArgs<T> g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits, aligned};
     dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
     hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(WG), 0, stream, g);
     FL_LAUNCHED();

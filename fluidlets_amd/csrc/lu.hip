@@ -38,104 +38,163 @@ int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t ld
 
 namespace {
 
-constexpr int NB = 128;
+constexpr int NB = 128;   // diagonal block (LDS-resident GEPP, inverted for the solves)
+constexpr int NBO = 256;  // outer panel width: the trailing update runs with K = NBO
 
 // ---------------------------------------------------------------- diagonal block GEPP
+// 1024 lanes (16 waves).  Step c: the pivot (first row of largest |S[r][c]|,
+// r >= c) is read from 16 per-wave candidates written during step c-1, rows c
+// and p are exchanged, then wave w updates rows r = c+1+w, c+17+w, ... with
+// one batched round of LDS loads: l = S[r][c] / S[c][c] (LAPACK getf2 scales
+// by the reciprocal), S[r][j] -= l S[c][j] for j > c, and lane 0 (column c+1)
+// records its rows' candidate for step c+1.  Two barriers per column.
+constexpr int DG_WAVES = 16;
+constexpr int DG_ROWS = (NB + DG_WAVES - 1) / DG_WAVES;  // rows per wave per step
+
 template <typename T>
-__global__ __launch_bounds__(256) void diag_getrf_kernel(T* A, int64_t lda, int nb, int64_t k, int32_t* ipiv,
-                                                         int32_t* info)
+__global__ __launch_bounds__(1024) void diag_getrf_kernel(T* A, int64_t lda, int nb, int64_t k, int32_t* ipiv,
+                                                          int32_t* info)
 {
     __shared__ T S[NB * (NB + 1)];
-    __shared__ T red_val[4];
-    __shared__ int red_idx[4];
-    __shared__ int piv_s;
+    __shared__ T cand_v[2][DG_WAVES];
+    __shared__ int cand_i[2][DG_WAVES];
     constexpr int LS = NB + 1;
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    for (int idx = tid; idx < nb * nb; idx += 256) S[(idx / nb) * LS + idx % nb] = A[(idx / nb) * lda + idx % nb];
+    for (int idx = tid; idx < nb * nb; idx += 1024) S[(idx / nb) * LS + idx % nb] = A[(idx / nb) * lda + idx % nb];
     __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        // first index of max |S[r][c]| over r in [c, nb)
-        T v = (tid >= c && tid < nb) ? (T)fabs((double)S[tid * LS + c]) : T(-1);
-        int vi = tid;
-#pragma unroll
-        for (int off = 32; off > 0; off /= 2) {
-            T ov = __shfl_xor(v, off);
-            int oi = __shfl_xor(vi, off);
-            if (ov > v || (ov == v && oi < vi)) {
-                v = ov;
-                vi = oi;
+    // candidates for column 0
+    if (lane == 0) {
+        T bv = T(-1);
+        int bi = 0;
+        for (int r = wave; r < nb; r += DG_WAVES) {
+            T v = (T)fabs((double)S[r * LS]);
+            if (v > bv) {
+                bv = v;
+                bi = r;
             }
         }
-        if (lane == 0) {
-            red_val[wave] = v;
-            red_idx[wave] = vi;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            T bv = red_val[0];
-            int bi = red_idx[0];
-            for (int w = 1; w < 4; ++w)
-                if (red_val[w] > bv || (red_val[w] == bv && red_idx[w] < bi)) {
-                    bv = red_val[w];
-                    bi = red_idx[w];
+        cand_v[0][wave] = bv;
+        cand_i[0][wave] = bi;
+    }
+    __syncthreads();
+    for (int c = 0; c < nb; ++c) {
+        const int buf = c & 1;
+        if (wave < 2) {
+            // lanes 0..15 hold the 16 wave candidates; first index of the max
+            T bv = lane < DG_WAVES ? cand_v[buf][lane] : T(-1);
+            int p = lane < DG_WAVES ? cand_i[buf][lane] : 1 << 30;
+#pragma unroll
+            for (int off = 8; off > 0; off /= 2) {
+                const T ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(p, off);
+                if (ov > bv || (ov == bv && oi < p)) {
+                    bv = ov;
+                    p = oi;
                 }
-            piv_s = bi;
-            ipiv[k + c] = (int32_t)(k + bi);
-            if (S[bi * LS + c] == T(0)) atomicCAS(info, 0, (int32_t)(k + c + 1));
-        }
-        __syncthreads();
-        const int p = piv_s;
-        if (p != c && tid < nb) {
-            T t = S[c * LS + tid];
-            S[c * LS + tid] = S[p * LS + tid];
-            S[p * LS + tid] = t;
+            }
+            p = __shfl(p, 0);
+            bv = __shfl(bv, 0);
+            if (tid == 0) {
+                ipiv[k + c] = (int32_t)(k + p);
+                if (bv == T(0)) atomicCAS(info, 0, (int32_t)(k + c + 1));
+            }
+            if (p != c && tid < nb) {
+                T t = S[c * LS + tid];
+                S[c * LS + tid] = S[p * LS + tid];
+                S[p * LS + tid] = t;
+            }
         }
         __syncthreads();
         const T pv = S[c * LS + c];
-        if (pv != T(0)) {
-            const T rp = T(1) / pv;
-            if (tid > c && tid < nb) S[tid * LS + c] *= rp;
+        const T rp = pv != T(0) ? T(1) / pv : T(1);
+        const int j0 = c + 1 + lane, j1 = j0 + 64;
+        const T u0 = j0 < nb ? S[c * LS + j0] : T(0);
+        const T u1 = j1 < nb ? S[c * LS + j1] : T(0);
+        T l[DG_ROWS], x0[DG_ROWS], x1[DG_ROWS];
+#pragma unroll
+        for (int i = 0; i < DG_ROWS; ++i) {
+            const int r = c + 1 + wave + DG_WAVES * i;
+            const bool ok = r < nb;
+            l[i] = ok ? S[r * LS + c] * rp : T(0);
+            x0[i] = (ok && j0 < nb) ? S[r * LS + j0] : T(0);
+            x1[i] = (ok && j1 < nb) ? S[r * LS + j1] : T(0);
         }
-        __syncthreads();
-        for (int r = c + 1 + wave; r < nb; r += 4) {
-            const T l = S[r * LS + c];
-            for (int j = c + 1 + lane; j < nb; j += 64) S[r * LS + j] -= l * S[c * LS + j];
+        T nb_v = T(-1);
+        int nb_i = 0;
+#pragma unroll
+        for (int i = 0; i < DG_ROWS; ++i) {
+            const int r = c + 1 + wave + DG_WAVES * i;
+            if (r < nb) {
+                x0[i] -= l[i] * u0;
+                x1[i] -= l[i] * u1;
+                if (lane == 0) S[r * LS + c] = l[i];
+                if (j0 < nb) S[r * LS + j0] = x0[i];
+                if (j1 < nb) S[r * LS + j1] = x1[i];
+                const T a = (T)fabs((double)x0[i]);
+                if (a > nb_v) {
+                    nb_v = a;
+                    nb_i = r;
+                }
+            }
+        }
+        if (lane == 0) {
+            cand_v[buf ^ 1][wave] = nb_v;
+            cand_i[buf ^ 1][wave] = nb_i;
         }
         __syncthreads();
     }
-    for (int idx = tid; idx < nb * nb; idx += 256) A[(idx / nb) * lda + idx % nb] = S[(idx / nb) * LS + idx % nb];
+    for (int idx = tid; idx < nb * nb; idx += 1024) A[(idx / nb) * lda + idx % nb] = S[(idx / nb) * LS + idx % nb];
 }
 
 // ---------------------------------------------------------------- triangular inverses
-// Block 0: X = inv(L11) (unit lower); block 1: X = inv(U11) (upper).  Thread j
-// forms column j; the loops are uniform across threads so the factor entries
-// are wave-uniform loads and X lives in LDS.  Output: nb x nb row-major (ld NB),
+// grid (2, NB / 16): blockIdx.x 0 forms X = inv(L11) (unit lower), 1 forms
+// X = inv(U11) (upper); blockIdx.y selects 16 columns.  The factored block is
+// staged in LDS; column j of X is computed by 8 consecutive lanes of one wave
+// (each summing an eighth of the dot product, combined by shuffles), so rows
+// are produced without workgroup barriers.  Output: nb x nb row-major (ld NB),
 // zero outside the triangle.
+constexpr int TR_COLS = 16;
+
 template <typename T>
-__global__ __launch_bounds__(NB) void trtri_kernel(const T* A, int64_t lda, int nb, T* invL, T* invU)
+__global__ __launch_bounds__(128) void trtri_kernel(const T* A, int64_t lda, int nb, T* invL, T* invU)
 {
-    __shared__ T X[NB * (NB + 1)];
     constexpr int LX = NB + 1;
-    const int j = threadIdx.x;
+    __shared__ T F[NB * LX];
+    __shared__ T X[NB * TR_COLS];
+    const int tid = threadIdx.x;
+    const int jj = tid / 8, part = tid % 8;
+    const int j = blockIdx.y * TR_COLS + jj;
+    for (int idx = tid; idx < nb * nb; idx += 128) F[(idx / nb) * LX + idx % nb] = A[(idx / nb) * lda + idx % nb];
+    __syncthreads();
+    T* out = blockIdx.x == 0 ? invL : invU;
     if (blockIdx.x == 0) {
         for (int i = 0; i < nb; ++i) {
             T s = T(0);
-            for (int kk = 0; kk < i; ++kk) s += A[i * lda + kk] * X[kk * LX + j];
-            X[i * LX + j] = i > j ? -s : (i == j ? T(1) : T(0));
+            // sum over kk in [j, i) of L[i][kk] X[kk][j]
+            for (int kk = j + part; kk < i; kk += 8) s += F[i * LX + kk] * X[kk * TR_COLS + jj];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            s += __shfl_xor(s, 4);
+            const T v = i > j ? -s : (i == j ? T(1) : T(0));
+            if (part == 0) X[i * TR_COLS + jj] = v;
         }
-        __syncthreads();
-        for (int i = 0; i < nb; ++i)
-            if (j < nb) invL[i * NB + j] = X[i * LX + j];
     } else {
         for (int i = nb - 1; i >= 0; --i) {
             T s = T(0);
-            for (int kk = i + 1; kk < nb; ++kk) s += A[i * lda + kk] * X[kk * LX + j];
-            const T d = A[i * lda + i];
-            X[i * LX + j] = i < j ? -s / d : (i == j ? T(1) / d : T(0));
+            // sum over kk in (i, j] of U[i][kk] X[kk][j]
+            for (int kk = i + 1 + part; kk <= j && kk < nb; kk += 8) s += F[i * LX + kk] * X[kk * TR_COLS + jj];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            s += __shfl_xor(s, 4);
+            const T d = F[i * LX + i];
+            const T v = i < j ? -s / d : (i == j ? T(1) / d : T(0));
+            if (part == 0) X[i * TR_COLS + jj] = v;
         }
-        __syncthreads();
-        for (int i = 0; i < nb; ++i)
-            if (j < nb) invU[i * NB + j] = X[i * LX + j];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nb * TR_COLS; idx += 128) {
+        const int i = idx / TR_COLS, c = blockIdx.y * TR_COLS + idx % TR_COLS;
+        if (c < nb) out[i * NB + c] = X[idx];
     }
 }
 
@@ -233,46 +292,86 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 }
 
 // ---------------------------------------------------------------- triangular solves
-// One block step of the forward (LOWER) or backward substitution on nrhs
-// right-hand sides (row-major, ld = nrhs):
-//   y = inv(T_bb) src[b-rows]     (every workgroup forms y; workgroup 0 stores it to dst)
-//   src[r] -= T[r, b-cols] y      for the rows r this workgroup owns outside block b
-// rows below block b for LOWER, above it otherwise.
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_step_kernel(const T* LU, int64_t lda, int64_t n, int64_t b0, int nbk,
-                                                        const T* inv, T* src, T* dst, int nrhs)
+// One block step of the forward (LOWER) or backward substitution on nrhs <= 8
+// right-hand sides stored row-major with ld = nrhs, in two launches:
+//   trsv_diag:   y = inv(T_bb) src[b-rows]  -> dst[b-rows]      (one workgroup)
+//   trsv_update: src[r] -= T[r, b-cols] y   for r below block b (LOWER) or
+//                above it (upper), 8 lanes per row, each lane 16 consecutive
+//                entries of the row (coalesced), reduced across the 8 lanes.
+template <typename T>
+__global__ __launch_bounds__(256) void trsv_diag_kernel(const T* inv, int64_t b0, int nbk, const T* src, T* dst,
+                                                        int nrhs)
 {
     __shared__ T xs[NB * 8];
-    __shared__ T ys[NB * 8];
+    __shared__ T part[2][NB * 8];
     const int tid = threadIdx.x;
     for (int idx = tid; idx < nbk * nrhs; idx += 256) xs[idx] = src[b0 * nrhs + idx];
     __syncthreads();
-    for (int idx = tid; idx < nbk * nrhs; idx += 256) {
-        const int i = idx / nrhs, r = idx % nrhs;
-        T s = T(0);
-        for (int kk = 0; kk < nbk; ++kk) s += inv[i * NB + kk] * xs[kk * nrhs + r];
-        ys[idx] = s;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0)
-        for (int idx = tid; idx < nbk * nrhs; idx += 256) dst[b0 * nrhs + idx] = ys[idx];
-    const int64_t lo = LOWER ? b0 + nbk : 0;
-    const int64_t hi = LOWER ? n : b0;
-    const int64_t r = lo + (int64_t)blockIdx.x * 256 + tid;
-    if (r >= hi) return;
+    const int i = tid % NB, h = tid / NB;  // row i of inv, half h of the k range
     T acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = T(0);
-    const T* row = LU + r * lda + b0;
-    for (int kk = 0; kk < nbk; ++kk) {
-        const T a = row[kk];
+    if (i < nbk) {
+        const T* row = inv + (int64_t)i * NB;
+        const int k0 = h * (NB / 2), k1 = min(nbk, k0 + NB / 2);
+#pragma unroll 8
+        for (int kk = k0; kk < k1; ++kk) {
+            const T a = row[kk];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q < nrhs) acc[q] += a * ys[kk * nrhs + q];
+            for (int q = 0; q < 8; ++q)
+                if (q < nrhs) acc[q] += a * xs[kk * nrhs + q];
+        }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-        if (q < nrhs) src[r * nrhs + q] -= acc[q];
+        if (q < nrhs) part[h][i * nrhs + q] = acc[q];
+    __syncthreads();
+    for (int idx = tid; idx < nbk * nrhs; idx += 256) dst[b0 * nrhs + idx] = part[0][idx] + part[1][idx];
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsv_update_kernel(const T* LU, int64_t lda, int64_t n, int64_t b0, int nbk,
+                                                          const T* ysrc, T* src, int nrhs)
+{
+    __shared__ T ys[NB * 8];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < nbk * nrhs; idx += 256) ys[idx] = ysrc[b0 * nrhs + idx];
+    __syncthreads();
+    const int64_t lo = LOWER ? b0 + nbk : 0;
+    const int64_t hi = LOWER ? n : b0;
+    const int sub = tid % 8;                       // 16-entry segment of the row
+    const int64_t r = lo + (int64_t)blockIdx.x * 32 + tid / 8;
+    T acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = T(0);
+    if (r < hi) {
+        const T* row = LU + r * lda + b0;
+        const int k0 = sub * 16;
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            if (k0 + kk < nbk) {
+                const T a = row[k0 + kk];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (q < nrhs) acc[q] += a * ys[(k0 + kk) * nrhs + q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        T v = acc[q];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        acc[q] = v;
+    }
+    if (r < hi && sub < nrhs) {
+        T v = T(0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == sub) v = acc[q];
+        src[r * nrhs + sub] -= v;
+    }
 }
 
 // dst[i][:] = scale_c * src[perm[i]][:] with a type conversion (gather of the
@@ -302,37 +401,49 @@ __global__ __launch_bounds__(256) void copy_rows_kernel(const TS* src, TD* dst, 
 }
 
 // ---------------------------------------------------------------- refinement
-// r = b - A x (FP64), one wave per row, nrhs <= 8; x and r are n x nrhs
-// (ld nrhs for x/r, ldb for b).
+// r = b - A x (FP64), 8 rows per wave (each x entry loaded once for the 8
+// rows), lanes striding along the rows; x and r are n x nrhs (ld nrhs), b has ld ldb.
+constexpr int RES_ROWS = 8;
 __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t lda, int64_t n, const double* b,
                                                        int64_t ldb, const double* x, double* r, int nrhs)
 {
     const int lane = threadIdx.x % 64;
-    const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
-    if (row >= n) return;
-    double acc[8];
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + threadIdx.x / 64) * RES_ROWS;
+    if (row0 >= n) return;
+    double acc[RES_ROWS][8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-    const double* a = A + row * lda;
+    for (int i = 0; i < RES_ROWS; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[i][q] = 0.0;
     for (int64_t c = lane; c < n; c += 64) {
-        const double v = a[c];
+        double xv[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q < nrhs) acc[q] += v * x[c * nrhs + q];
+        for (int q = 0; q < 8; ++q) xv[q] = q < nrhs ? x[c * nrhs + q] : 0.0;
+#pragma unroll
+        for (int i = 0; i < RES_ROWS; ++i) {
+            const double a = row0 + i < n ? A[(row0 + i) * lda + c] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[i][q] += a * xv[q];
+        }
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        double s = acc[q];
+    for (int i = 0; i < RES_ROWS; ++i)
 #pragma unroll
-        for (int off = 32; off > 0; off /= 2) s += __shfl_xor(s, off);
-        acc[q] = s;
-    }
-    if (lane < nrhs) {
+        for (int q = 0; q < 8; ++q) {
+            double s = acc[i][q];
+#pragma unroll
+            for (int off = 32; off > 0; off /= 2) s += __shfl_xor(s, off);
+            acc[i][q] = s;
+        }
+    if (lane < RES_ROWS * nrhs) {
+        const int i = lane / nrhs, q = lane % nrhs;
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == lane) s = acc[q];
-        r[row * nrhs + lane] = b[row * ldb + lane] - s;
+        for (int ii = 0; ii < RES_ROWS; ++ii)
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq)
+                if (ii == i && qq == q) s = acc[ii][qq];
+        if (row0 + i < n) r[(row0 + i) * nrhs + q] = b[(row0 + i) * ldb + q] - s;
     }
 }
 
@@ -422,70 +533,101 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     int32_t* info_d = reinterpret_cast<int32_t*>(tail + 64);
     FL_HIP(hipMemsetAsync(info_d, 0, sizeof(int32_t), st));
     int fallbacks = 0;
-    for (int64_t k = 0; k < n; k += NB) {
-        const int nb = (int)std::min<int64_t>(NB, n - k);
-        const int64_t m = n - k;
-        const int64_t p = k / NB;
-        T* Akk = A + k * lda + k;
-        FL_HIP(hipMemcpy2DAsync(backup, nb * sizeof(T), Akk, lda * sizeof(T), nb * sizeof(T), m,
-                                hipMemcpyDeviceToDevice, st));
-        FL_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st));
-        hipLaunchKernelGGL(diag_getrf_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, k, ipiv, info_d);
+    auto swap_cols = [&](int64_t k0, int nb, int64_t c0, int64_t c1) -> int {
+        if (c1 <= c0) return FL_OK;
+        hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)div_up(c1 - c0, 256)), dim3(256), 0, st, A, lda, k0, nb,
+                           (const int32_t*)ipiv, c0, c1);
         FL_LAUNCHED();
-        if (m > nb) {
-            // L21 = A21 inv(U11) needs inv(U11) first
-            hipLaunchKernelGGL(trtri_kernel<T>, dim3(2), dim3(NB), 0, st, Akk, lda, nb, FactorAux<T>::invL(aux, n, p),
-                               FactorAux<T>::invU(aux, n, p));
-            FL_LAUNCHED();
-            if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, FactorAux<T>::invU(aux, n, p), NB,
-                                       T(0), Akk + nb * lda, lda, flag, st))
-                return s;
-            unsigned int bits = 0;
-            FL_HIP(hipMemcpyAsync(&bits, flag, sizeof(bits), hipMemcpyDeviceToHost, st));
-            FL_HIP(hipStreamSynchronize(st));
-            float lmax;
-            std::memcpy(&lmax, &bits, sizeof(float));
-            if (!(lmax <= 1.0f + 1.0f / (1 << 20))) {
-                // restore the panel and pivot over every row below the diagonal
-                ++fallbacks;
-                FL_HIP(hipMemcpy2DAsync(Akk, lda * sizeof(T), backup, nb * sizeof(T), nb * sizeof(T), m,
+        return FL_OK;
+    };
+    // Outer panels of NBO columns (the trailing update has K = NBO); each is
+    // factored as sub-panels of NB columns.
+    for (int64_t k = 0; k < n; k += NBO) {
+        const int64_t pe = std::min<int64_t>(n, k + NBO);
+        for (int64_t ks = k; ks < pe; ks += NB) {
+            const int nb = (int)std::min<int64_t>(NB, pe - ks);
+            const int64_t m = n - ks;
+            const int64_t p = ks / NB;
+            T* Akk = A + ks * lda + ks;
+            T* invL = FactorAux<T>::invL(aux, n, p);
+            T* invU = FactorAux<T>::invU(aux, n, p);
+            if (m > nb) {
+                FL_HIP(hipMemcpy2DAsync(backup, nb * sizeof(T), Akk, lda * sizeof(T), nb * sizeof(T), m,
                                         hipMemcpyDeviceToDevice, st));
-                for (int64_t c = k; c < k + nb; ++c) {
-                    hipLaunchKernelGGL(pivot_col_kernel<T>, dim3(1), dim3(1024), 0, st, A, lda, n, k, nb, c, ipiv,
-                                       info_d);
-                    FL_LAUNCHED();
-                    if (c + 1 < k + nb && c + 1 < n) {
-                        hipLaunchKernelGGL(panel_update_kernel<T>, dim3((unsigned)div_up(n - c - 1, 64)), dim3(256), 0,
-                                           st, A, lda, n, k, nb, c);
-                        FL_LAUNCHED();
-                    }
-                }
-                hipLaunchKernelGGL(trtri_kernel<T>, dim3(2), dim3(NB), 0, st, Akk, lda, nb,
-                                   FactorAux<T>::invL(aux, n, p), FactorAux<T>::invU(aux, n, p));
-                FL_LAUNCHED();
+                FL_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st));
             }
-        } else {
-            hipLaunchKernelGGL(trtri_kernel<T>, dim3(2), dim3(NB), 0, st, Akk, lda, nb, FactorAux<T>::invL(aux, n, p),
-                               FactorAux<T>::invU(aux, n, p));
+            hipLaunchKernelGGL(diag_getrf_kernel<T>, dim3(1), dim3(1024), 0, st, Akk, lda, nb, ks, ipiv, info_d);
             FL_LAUNCHED();
-        }
-        // row interchanges to the left and right of the panel
-        if (k > 0) {
-            hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)div_up(k, 256)), dim3(256), 0, st, A, lda, k, nb, ipiv,
-                               (int64_t)0, k);
+            hipLaunchKernelGGL(trtri_kernel<T>, dim3(2, NB / TR_COLS), dim3(128), 0, st, (const T*)Akk, lda, nb, invL,
+                               invU);
             FL_LAUNCHED();
+            if (m > nb) {
+                // L21 = A21 inv(U11), in place, with max |L21| recorded
+                if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0), Akk + nb * lda,
+                                           lda, flag, st))
+                    return s;
+                unsigned int bits = 0;
+                FL_HIP(hipMemcpyAsync(&bits, flag, sizeof(bits), hipMemcpyDeviceToHost, st));
+                FL_HIP(hipStreamSynchronize(st));
+                float lmax;
+                std::memcpy(&lmax, &bits, sizeof(float));
+                if (!(lmax <= 1.0f + 1.0f / (1 << 20))) {
+                    // restore the sub-panel and pivot over every row below the diagonal
+                    ++fallbacks;
+                    FL_HIP(hipMemcpy2DAsync(Akk, lda * sizeof(T), backup, nb * sizeof(T), nb * sizeof(T), m,
+                                            hipMemcpyDeviceToDevice, st));
+                    for (int64_t c = ks; c < ks + nb; ++c) {
+                        hipLaunchKernelGGL(pivot_col_kernel<T>, dim3(1), dim3(1024), 0, st, A, lda, n, ks, nb, c, ipiv,
+                                           info_d);
+                        FL_LAUNCHED();
+                        if (c + 1 < ks + nb && c + 1 < n) {
+                            hipLaunchKernelGGL(panel_update_kernel<T>, dim3((unsigned)div_up(n - c - 1, 64)), dim3(256),
+                                               0, st, A, lda, n, ks, nb, c);
+                            FL_LAUNCHED();
+                        }
+                    }
+                    hipLaunchKernelGGL(trtri_kernel<T>, dim3(2, NB / TR_COLS), dim3(128), 0, st, (const T*)Akk, lda,
+                                       nb, invL, invU);
+                    FL_LAUNCHED();
+                }
+            }
+            // this sub-panel's interchanges on the rest of the outer panel
+            if (int s = swap_cols(ks, nb, k, ks)) return s;
+            if (int s = swap_cols(ks, nb, ks + nb, pe)) return s;
+            // update the outer panel's remaining columns
+            const int64_t rest = pe - ks - nb;
+            if (rest > 0) {
+                T* A12 = Akk + nb;
+                if (int s = gemm_launch<T>(nb, rest, nb, T(1), invL, NB, A12, lda, T(0), A12, lda, nullptr, st))
+                    return s;
+                if (m > nb)
+                    if (int s = gemm_launch<T>(m - nb, rest, nb, T(-1), Akk + nb * lda, lda, A12, lda, T(1),
+                                               Akk + nb * lda + nb, lda, nullptr, st))
+                        return s;
+            }
         }
-        const int64_t nr = n - k - nb;
+        const int nbo = (int)(pe - k);
+        // the outer panel's interchanges to its left and right
+        if (int s = swap_cols(k, nbo, 0, k)) return s;
+        if (int s = swap_cols(k, nbo, pe, n)) return s;
+        const int64_t nr = n - pe;
         if (nr > 0) {
-            hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)div_up(nr, 256)), dim3(256), 0, st, A, lda, k, nb, ipiv,
-                               k + nb, n);
-            FL_LAUNCHED();
-            T* A12 = Akk + nb;
-            if (int s = gemm_launch<T>(nb, nr, nb, T(1), FactorAux<T>::invL(aux, n, p), NB, A12, lda, T(0), A12, lda,
-                                       nullptr, st))
-                return s;
-            if (int s = gemm_launch<T>(m - nb, nr, nb, T(-1), Akk + nb * lda, lda, A12, lda, T(1),
-                                       Akk + nb * lda + nb, lda, nullptr, st))
+            // U12 = inv(L11) A12 block row by block row
+            for (int64_t ks = k; ks < pe; ks += NB) {
+                const int nb = (int)std::min<int64_t>(NB, pe - ks);
+                T* Ar = A + ks * lda + pe;
+                if (int s = gemm_launch<T>(nb, nr, nb, T(1), FactorAux<T>::invL(aux, n, ks / NB), NB, Ar, lda, T(0),
+                                           Ar, lda, nullptr, st))
+                    return s;
+                const int64_t below = pe - ks - nb;
+                if (below > 0)
+                    if (int s = gemm_launch<T>(below, nr, nb, T(-1), A + (ks + nb) * lda + ks, lda, Ar, lda, T(1),
+                                               A + (ks + nb) * lda + pe, lda, nullptr, st))
+                        return s;
+            }
+            // trailing update with K = NBO
+            if (int s = gemm_launch<T>(nr, nr, nbo, T(-1), A + pe * lda + k, lda, A + k * lda + pe, lda, T(1),
+                                       A + pe * lda + pe, lda, nullptr, st))
                 return s;
         }
     }
@@ -516,24 +658,32 @@ template <typename T>
 int solve_permuted(const T* LU, int64_t n, int64_t lda, void* aux, T* y, T* tmp, int nrhs, hipStream_t st)
 {
     const int64_t nblk = div_up(n, NB);
-    // forward: L z = y  (z written to tmp block by block; y rows below updated)
+    // forward: L z = y  (z_b written to tmp; rows of y below block b updated)
     for (int64_t p = 0; p < nblk; ++p) {
         const int64_t b0 = p * NB;
         const int nbk = (int)std::min<int64_t>(NB, n - b0);
-        const int64_t rows = n - b0 - nbk;
-        const unsigned grid = (unsigned)std::max<int64_t>(1, div_up(rows, 256));
-        hipLaunchKernelGGL((trsv_step_kernel<T, true>), dim3(grid), dim3(256), 0, st, LU, lda, n, b0, nbk,
-                           (const T*)FactorAux<T>::invL(aux, n, p), y, tmp, nrhs);
+        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(256), 0, st, (const T*)FactorAux<T>::invL(aux, n, p), b0,
+                           nbk, (const T*)y, tmp, nrhs);
         FL_LAUNCHED();
+        const int64_t rows = n - b0 - nbk;
+        if (rows > 0) {
+            hipLaunchKernelGGL((trsv_update_kernel<T, true>), dim3((unsigned)div_up(rows, 32)), dim3(256), 0, st, LU,
+                               lda, n, b0, nbk, (const T*)tmp, y, nrhs);
+            FL_LAUNCHED();
+        }
     }
-    // backward: U x = z  (x written to y block by block; tmp rows above updated)
+    // backward: U x = z  (x_b written to y; rows of tmp above block b updated)
     for (int64_t p = nblk - 1; p >= 0; --p) {
         const int64_t b0 = p * NB;
         const int nbk = (int)std::min<int64_t>(NB, n - b0);
-        const unsigned grid = (unsigned)std::max<int64_t>(1, div_up(b0, 256));
-        hipLaunchKernelGGL((trsv_step_kernel<T, false>), dim3(grid), dim3(256), 0, st, LU, lda, n, b0, nbk,
-                           (const T*)FactorAux<T>::invU(aux, n, p), tmp, y, nrhs);
+        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(256), 0, st, (const T*)FactorAux<T>::invU(aux, n, p), b0,
+                           nbk, (const T*)tmp, y, nrhs);
         FL_LAUNCHED();
+        if (b0 > 0) {
+            hipLaunchKernelGGL((trsv_update_kernel<T, false>), dim3((unsigned)div_up(b0, 32)), dim3(256), 0, st, LU,
+                               lda, n, b0, nbk, (const T*)y, tmp, nrhs);
+            FL_LAUNCHED();
+        }
     }
     return FL_OK;
 }
@@ -630,7 +780,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
                            (const double*)scale);
         FL_LAUNCHED();
         ++it;
-        hipLaunchKernelGGL(residual_kernel, dim3((unsigned)div_up(n, 4)), dim3(256), 0, st, A64, lda64, n, b, ldb,
+        hipLaunchKernelGGL(residual_kernel, dim3((unsigned)div_up(n, 4 * RES_ROWS)), dim3(256), 0, st, A64, lda64, n, b, ldb,
                            (const double*)xc, r, q);
         FL_LAUNCHED();
     }
