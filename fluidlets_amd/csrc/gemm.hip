@@ -21,6 +21,8 @@
 // (IEEE float bits, atomic max), which the LU uses to verify its pivots.
 #include "fl_internal.h"
 
+#include <cstdlib>
+
 namespace fl {
 namespace {
 
@@ -50,6 +52,8 @@ struct Mma<float> {
     __device__ static int a_k(int l) { return l / 32; }
     __device__ static int d_row(int l, int i) { return 8 * (i / 4) + 4 * (l / 32) + (i % 4); }
     __device__ static int d_col(int l) { return l % 32; }
+    __device__ static int row_lane(int l) { return 4 * (l / 32); }       // d_row = row_lane + row_acc
+    __device__ static int row_acc(int i) { return 8 * (i / 4) + (i % 4); }
     __device__ static void unpack(const vec_t& v, float* o)
     {
         o[0] = v.x;
@@ -77,6 +81,8 @@ struct Mma<double> {
     __device__ static int a_k(int l) { return l / 16; }
     __device__ static int d_row(int l, int i) { return 4 * i + l / 16; }
     __device__ static int d_col(int l) { return l % 16; }
+    __device__ static int row_lane(int l) { return l / 16; }
+    __device__ static int row_acc(int i) { return 4 * i; }
     __device__ static void unpack(const vec_t& v, double* o)
     {
         o[0] = v.x;
@@ -96,7 +102,35 @@ struct GemmArgs {
     int64_t ldc;
     unsigned int* absmax_bits;
     int aligned;  // A, B base and leading dimensions allow 16-byte loads
+    int tiles_m, tiles_n;
+    int swizzle;  // 1: XCD-aware grouped tile order
 };
+
+// Tile of this workgroup.  The grid is 1-D; workgroups are dealt to the 8
+// XCDs round-robin (id % 8), so with `swizzle` each XCD is first given a
+// contiguous range of logical tiles, and logical tiles are ordered in groups
+// of GROUP_M tile rows (column by column inside a group), so the workgroups
+// resident on one XCD share A row blocks and B column blocks in its L2.
+constexpr int NXCD = 8, GROUP_M = 8;
+template <typename T>
+__device__ __forceinline__ void tile_of(const GemmArgs<T>& g, int& tm, int& tn)
+{
+    const int nt = g.tiles_m * g.tiles_n;
+    int id = blockIdx.x;
+    if (!g.swizzle) {
+        tm = id / g.tiles_n;
+        tn = id % g.tiles_n;
+        return;
+    }
+    if (nt % NXCD == 0) id = (id % NXCD) * (nt / NXCD) + id / NXCD;
+    const int per_group = GROUP_M * g.tiles_n;
+    const int group = id / per_group;
+    const int first_m = group * GROUP_M;
+    const int gm = min(GROUP_M, g.tiles_m - first_m);
+    const int in = id % per_group;
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
 
 template <typename T>
 __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
@@ -117,26 +151,43 @@ __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
     const int tid = threadIdx.x;
     const int lane = tid % 64, wave = tid / 64;
     const int wm = wave / 2, wn = wave % 2;
-    const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
+    int tm, tn;
+    tile_of(g, tm, tn);
+    const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
     const bool full_mn = g.aligned && m0 + TM <= g.M && n0 + TN <= g.N;
 
     typename P::acc_t acc[P::NB][P::NB];
     const bool use_c = g.beta != T(0) && g.alpha != T(0);
     const T cscale = use_c ? g.beta / g.alpha : T(0);
+    // interior tiles take unpredicated accesses from one lane base pointer
+    const bool full_c = m0 + TM <= g.M && n0 + TN <= g.N;
+    T* cbase = g.C + (m0 + wm * 64 + P::row_lane(lane)) * g.ldc + n0 + wn * 64 + P::d_col(lane);
+    auto coff = [&](int i, int j, int v) -> int64_t { return (int64_t)(i * P::B + P::row_acc(v)) * g.ldc + j * P::B; };
 #pragma unroll
     for (int i = 0; i < P::NB; ++i)
 #pragma unroll
-        for (int j = 0; j < P::NB; ++j) {
-            acc[i][j] = P::zero();
-            if (use_c) {
+        for (int j = 0; j < P::NB; ++j) acc[i][j] = P::zero();
+    if (use_c) {
+        if (full_c) {
 #pragma unroll
-                for (int v = 0; v < P::NACC; ++v) {
-                    int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
-                    int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
-                    acc[i][j][v] = (r < g.M && c < g.N) ? cscale * g.C[r * g.ldc + c] : T(0);
-                }
-            }
+            for (int i = 0; i < P::NB; ++i)
+#pragma unroll
+                for (int j = 0; j < P::NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < P::NACC; ++v) acc[i][j][v] = cscale * cbase[coff(i, j, v)];
+        } else {
+#pragma unroll
+            for (int i = 0; i < P::NB; ++i)
+#pragma unroll
+                for (int j = 0; j < P::NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < P::NACC; ++v) {
+                        int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
+                        int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
+                        acc[i][j][v] = (r < g.M && c < g.N) ? cscale * g.C[r * g.ldc + c] : T(0);
+                    }
         }
+    }
 
     T ra[EA], rb[EB];
     auto load = [&](int64_t k0) {
@@ -216,20 +267,38 @@ __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
 
     const T outscale = g.alpha != T(0) ? g.alpha : T(1);
     float local_max = 0.0f;
+    if (full_c) {
 #pragma unroll
-    for (int i = 0; i < P::NB; ++i)
+        for (int i = 0; i < P::NB; ++i)
 #pragma unroll
-        for (int j = 0; j < P::NB; ++j)
+            for (int j = 0; j < P::NB; ++j)
 #pragma unroll
-            for (int v = 0; v < P::NACC; ++v) {
-                int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
-                int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
-                if (r < g.M && c < g.N) {
-                    T val = outscale * acc[i][j][v];
-                    g.C[r * g.ldc + c] = val;
-                    local_max = fmaxf(local_max, (float)fabs((double)val));
+                for (int v = 0; v < P::NACC; ++v) cbase[coff(i, j, v)] = outscale * acc[i][j][v];
+        if (g.absmax_bits) {
+#pragma unroll
+            for (int i = 0; i < P::NB; ++i)
+#pragma unroll
+                for (int j = 0; j < P::NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < P::NACC; ++v)
+                        local_max = fmaxf(local_max, (float)fabs((double)(outscale * acc[i][j][v])));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < P::NB; ++i)
+#pragma unroll
+            for (int j = 0; j < P::NB; ++j)
+#pragma unroll
+                for (int v = 0; v < P::NACC; ++v) {
+                    int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
+                    int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
+                    if (r < g.M && c < g.N) {
+                        T val = outscale * acc[i][j][v];
+                        g.C[r * g.ldc + c] = val;
+                        local_max = fmaxf(local_max, (float)fabs((double)val));
+                    }
                 }
-            }
+    }
     if (g.absmax_bits) {
 #pragma unroll
         for (int off = 32; off > 0; off /= 2) local_max = fmaxf(local_max, __shfl_xor(local_max, off));
@@ -264,9 +333,11 @@ int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t ld
     const int vw = 16 / (int)sizeof(T);
     const int aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
                         (lda % vw == 0) && (ldb % vw == 0);
-    GemmArgs<T> g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits, aligned};
-    dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
-    hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(WG), 0, stream, g);
+    static const char* sw_env = std::getenv("FL_GEMM_SWIZZLE");
+    const int swizzle = sw_env && *sw_env ? std::atoi(sw_env) : 1;
+    const int tiles_m = (int)((M + TM - 1) / TM), tiles_n = (int)((N + TN - 1) / TN);
+    GemmArgs<T> g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits, aligned, tiles_m, tiles_n, swizzle};
+    hipLaunchKernelGGL(gemm_kernel<T>, dim3((unsigned)(tiles_m * tiles_n)), dim3(WG), 0, stream, g);
     FL_LAUNCHED();
     return FL_OK;
 }
