@@ -54,6 +54,8 @@ _SIGNATURES = {
                          c_void_p, c_int64, c_void_p]),
     "fl_dgemm": (c_int, [c_int64, c_int64, c_int64, c_double, c_void_p, c_int64, c_void_p, c_int64, c_double,
                          c_void_p, c_int64, c_void_p]),
+    "fl_swim_rhs": (c_int, [c_int64, c_void_p, POINTER(c_double), c_void_p, c_void_p, c_void_p]),
+    "fl_swim_schur": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -85,6 +85,10 @@ template <typename T>
 int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                 T* C, int64_t ldc, unsigned int* absmax_bits, hipStream_t stream);
 
+int swim_rhs_launch(int64_t N, const double* x, const double* x0, const double* v, double* R, hipStream_t st);
+int swim_schur_launch(int64_t N, const double* Y, const double* count, const double* Xsum, double* f, double* UW,
+                      double* S, hipStream_t st);
+
 }  // namespace fl
 
 using namespace fl;
@@ -242,6 +246,19 @@ int fl_dgemm(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int
     clear_error();
     FL_CHECK(lda >= K && ldb >= N && ldc >= N, "leading dimension too small");
     return gemm_launch<double>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, as_stream(stream));
+}
+
+int fl_swim_rhs(int64_t N, const double* x, const double* x0_host, const double* v, double* R, void* stream)
+{
+    clear_error();
+    return swim_rhs_launch(N, x, x0_host, v, R, as_stream(stream));
+}
+
+int fl_swim_schur(int64_t N, const double* Y, const double* count, const double* Xsum, double* f, double* UW,
+                  double* S, void* stream)
+{
+    clear_error();
+    return swim_schur_launch(N, Y, count, Xsum, f, UW, S, as_stream(stream));
 }
 
 int fl_nearest(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* nearest, void* stream)

@@ -1,0 +1,228 @@
+"""Free swimming in Stokes flow on the device (drop-in for
+src/fluidlets/swimming.py: SwimmerShape, SwimState, swimming_problem,
+Trajectory, swim), plus a fixed-step midpoint RK2 integrator.
+
+At every instant the unknowns are the forces f on the fluid at N force
+points and the frame velocity w = [U; Omega]; with the nearest-neighbour
+kernel matrix K (``assemble_nearest``) they solve (swimming.py:90-101)
+
+    K f - U + (x - x0) x Omega = v,   sum count_n f_n = 0,   sum Xsum_n x f_n = 0.
+
+K is assembled and factored on the GPU (FP64 LU, or FP32 LU with FP64
+refinement); the seven right-hand sides [v | C] are solved with the factors
+and the 6 x 6 Schur complement S = B K^{-1} C is formed and solved in FP64 by
+the fl_swim_schur kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Callable
+
+import numpy as np
+
+from . import _device as D
+from . import linalg
+from ._lib import check, lib
+from .core.assemble import assemble_nearest_device, nearest_groups
+from .kernels import Stokeslet3D
+
+
+def _cross_matrix(v: np.ndarray) -> np.ndarray:
+    """(N, 3, 3) matrices C with C @ w = v x w for each row v (swimming.py:33-40)."""
+    v = np.atleast_2d(v)
+    C = np.zeros((len(v), 3, 3))
+    C[:, 0, 1], C[:, 0, 2] = -v[:, 2], v[:, 1]
+    C[:, 1, 0], C[:, 1, 2] = v[:, 2], -v[:, 0]
+    C[:, 2, 0], C[:, 2, 1] = -v[:, 1], v[:, 0]
+    return C
+
+
+@dataclass
+class SwimmerShape:
+    """Swimmer geometry at one instant, in the body frame.
+
+    x : (N, 3) force (collocation) points
+    v : (N, 3) their body-frame velocities (the beat)
+    X : (Q, 3) quadrature points
+    """
+    x: np.ndarray
+    v: np.ndarray
+    X: np.ndarray
+
+
+@dataclass
+class SwimState:
+    U: np.ndarray            # frame velocity (lab)
+    Omega: np.ndarray        # angular velocity (lab)
+    forces: np.ndarray       # (N, 3) force on the fluid at each force point (lab)
+    x: np.ndarray            # (N, 3) force points (lab)
+    X: np.ndarray            # (Q, 3) quadrature points (lab)
+    nearest: np.ndarray      # (Q,) nearest force point of each quadrature point
+
+
+def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, nearest=None,
+                     precision: str = "fp64", stream=None) -> SwimState:
+    """Solve the instantaneous free-swimming problem (reference signature).
+
+    ``precision`` "fp64" (default) factors K in FP64; "mixed" uses FP32 MFMA
+    factors with FP64 iterative refinement of all seven right-hand sides."""
+    t = D.torch()
+    kernel = Stokeslet3D() if kernel is None else kernel
+    if np.iscomplexobj(mu):
+        raise ValueError("swimming_problem needs a real viscosity")
+    mu = float(mu)
+    x0 = np.asarray(x0, float)
+    B = np.asarray(B, float)
+    x = x0 + np.asarray(shape.x, float) @ B.T
+    X = x0 + np.asarray(shape.X, float) @ B.T
+    v = np.asarray(shape.v, float) @ B.T
+    dev = D.device()
+    x_soa, X_soa = D.soa(x, dev), D.soa(X, dev)
+    N, Q = len(x), len(X)
+    if nearest is None:
+        from ._nearest import nearest_device
+        nearest = nearest_device(X_soa, x_soa, stream=stream).cpu().numpy()
+    nearest = np.ascontiguousarray(nearest, dtype=np.int64)
+    order, start = nearest_groups(nearest, N)
+    eps_arr = np.broadcast_to(np.asarray(eps, float), (Q,))
+    e = eps_arr if not (Q > 1 and np.all(eps_arr == eps_arr[0])) else eps_arr[:1]
+    eps_t, stride = D.eps_arg(e, Q, dev)
+    K64, K32 = assemble_nearest_device(kernel, x_soa, X_soa, eps_t, stride, t.from_numpy(order).to(dev),
+                                       t.from_numpy(start).to(dev), N, mu=mu, fp32=(precision == "mixed"),
+                                       stream=stream)
+    L = lib()
+    R = t.empty((3 * N, 7), dtype=t.float64, device=dev)
+    x0c = (ctypes.c_double * 3)(*x0.tolist())
+    check(L.fl_swim_rhs(N, D.ptr(x_soa), x0c, D.ptr(D.soa(v, dev)), D.ptr(R), D.stream_handle(stream)), "fl_swim_rhs")
+    if precision == "fp64":
+        F = linalg.getrf(K64, stream=stream)
+        if F.info > 0:
+            raise np.linalg.LinAlgError("Singular matrix")
+        Y = linalg.getrs(F, R, stream=stream)
+    elif precision == "mixed":
+        F = linalg.getrf(K32, stream=stream)
+        if F.info > 0:
+            raise np.linalg.LinAlgError("Singular matrix")
+        Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream)
+    else:
+        raise ValueError(f"unknown precision {precision!r}")
+    count = np.bincount(nearest, minlength=N).astype(float)
+    Xsum = np.zeros((N, 3))
+    np.add.at(Xsum, nearest, X)
+    f = t.empty((3 * N,), dtype=t.float64, device=dev)
+    UW = t.empty((6,), dtype=t.float64, device=dev)
+    S = t.empty((36,), dtype=t.float64, device=dev)
+    check(L.fl_swim_schur(N, D.ptr(Y), D.ptr(t.from_numpy(count).to(dev)), D.ptr(D.soa(Xsum, dev)), D.ptr(f),
+                          D.ptr(UW), D.ptr(S), D.stream_handle(stream)), "fl_swim_schur")
+    w = UW.cpu().numpy()
+    return SwimState(w[:3].copy(), w[3:].copy(), f.cpu().numpy().reshape(N, 3), x, X, nearest)
+
+
+@dataclass
+class Trajectory:
+    t: np.ndarray            # (T,)
+    x0: np.ndarray           # (T, 3) frame origin
+    B: np.ndarray            # (T, 3, 3) frame axes as columns
+
+
+def _frame_rhs(shape_at, eps, mu, kernel, nearest, precision):
+    def rhs(t, z):
+        b1, b2 = z[3:6], z[6:9]
+        B = np.c_[b1, b2, np.cross(b1, b2)]
+        st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision)
+        return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
+    return rhs
+
+
+def swim(shape_at: Callable[[float], SwimmerShape], t_span, eps, mu=1.0, x0=(0, 0, 0), B0=None, kernel=None,
+         t_eval=None, rtol=1e-6, atol=1e-9, fixed_nearest=False, precision: str = "fp64") -> Trajectory:
+    """Integrate a free swimmer's position and orientation (reference
+    signature, swimming.py:112-148): scipy's RK45 on the host, every
+    right-hand side a device swimming_problem."""
+    from scipy.integrate import solve_ivp
+
+    B0 = np.eye(3) if B0 is None else np.asarray(B0, float)
+    nearest = None
+    if fixed_nearest:
+        from ._nearest import nearest_neighbors
+        s0 = shape_at(t_span[0])
+        nearest = nearest_neighbors(s0.X, s0.x)
+    rhs = _frame_rhs(shape_at, eps, mu, kernel, nearest, precision)
+    z0 = np.concatenate([np.asarray(x0, float), B0[:, 0], B0[:, 1]])
+    sol = solve_ivp(rhs, t_span, z0, t_eval=t_eval, rtol=rtol, atol=atol)
+    if not sol.success:
+        raise RuntimeError(sol.message)
+    b1, b2 = sol.y[3:6].T, sol.y[6:9].T
+    Bt = np.stack([b1, b2, np.cross(b1, b2)], axis=2)
+    return Trajectory(sol.t, sol.y[:3].T, Bt)
+
+
+def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, steps: int, eps, mu=1.0,
+             x0=(0, 0, 0), B0=None, kernel=None, fixed_nearest=True, precision: str = "fp64",
+             stream=None) -> Trajectory:
+    """Fixed-step explicit midpoint (RK2) integration of the frame:
+    z_{n+1} = z_n + dt F(t_n + dt/2, z_n + dt/2 F(t_n, z_n)),  two device
+    swimming_problem solves per step."""
+    B0 = np.eye(3) if B0 is None else np.asarray(B0, float)
+    nearest = None
+    if fixed_nearest:
+        from ._nearest import nearest_neighbors
+        s0 = shape_at(t0)
+        nearest = nearest_neighbors(s0.X, s0.x)
+
+    def F(t, z):
+        b1, b2 = z[3:6], z[6:9]
+        B = np.c_[b1, b2, np.cross(b1, b2)]
+        st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision, stream=stream)
+        return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
+
+    z = np.concatenate([np.asarray(x0, float), B0[:, 0], B0[:, 1]])
+    ts, zs = [t0], [z.copy()]
+    t = t0
+    for _ in range(steps):
+        k1 = F(t, z)
+        z = z + dt * F(t + 0.5 * dt, z + 0.5 * dt * k1)
+        t += dt
+        ts.append(t)
+        zs.append(z.copy())
+    zs = np.array(zs)
+    b1, b2 = zs[:, 3:6], zs[:, 6:9]
+    return Trajectory(np.array(ts), zs[:, :3], np.stack([b1, b2, np.cross(b1, b2)], axis=2))
+
+
+def biflagellate(n_body: int = 1000, n_flag: int = 400, amplitude: float = 0.2, wavelength: float = 1.0,
+                 length: float = 2.0, period: float = 1.0):
+    """shape_at(t) of a model biflagellate: a 2:1 prolate spheroid body
+    (semi-axes 0.5, 0.5, 1 along z, Fibonacci points) and two flagella of
+    n_flag points leaving the anterior pole (z = 1) at +-30 degrees from the
+    axis, each carrying a helical travelling wave of the given amplitude and
+    wavelength (period T).  Force and quadrature points coincide."""
+    from .geometry import spheroid
+
+    body = spheroid(n_body, 0.5, 1.0)
+    s = (np.arange(n_flag) + 1) * length / n_flag
+    k = 2 * np.pi / wavelength
+    om = 2 * np.pi / period
+
+    def flagellum(t, side):
+        ang = side * np.pi / 6
+        d = np.array([np.sin(ang), 0.0, np.cos(ang)])
+        e1 = np.array([np.cos(ang), 0.0, -np.sin(ang)])
+        e2 = np.array([0.0, 1.0, 0.0])
+        ph = k * s - om * t
+        base = np.array([0.0, 0.0, 1.02])
+        # amplitude ramps up from the base so the flagellum leaves the body smoothly
+        amp = amplitude * (1 - np.exp(-(s / 0.2) ** 2))
+        pts = base + s[:, None] * d + amp[:, None] * (np.cos(ph)[:, None] * e1 + side * np.sin(ph)[:, None] * e2)
+        vel = amp[:, None] * om * (np.sin(ph)[:, None] * e1 - side * np.cos(ph)[:, None] * e2)
+        return pts, vel
+
+    def shape_at(t):
+        p1, v1 = flagellum(t, +1)
+        p2, v2 = flagellum(t, -1)
+        x = np.vstack([body, p1, p2])
+        v = np.vstack([np.zeros_like(body), v1, v2])
+        return SwimmerShape(x=x, v=v, X=x)
+
+    return shape_at

@@ -216,6 +216,25 @@ int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* 
              int inner_precision, int* iters_out, double* resid_out,
              void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ free swimmer
+ * The bordered system of src/fluidlets/swimming.py:90-101 (no slip, zero net
+ * force, zero net torque), which the reference solves with numpy.linalg.solve
+ * (swimming.py:101), solved here by a Schur complement on the factored K:
+ *   [ K  C ] [f]   [v]      C = [-I | [x_n - x0]x]          (3N x 6)
+ *   [ B  0 ] [w] = [0],     B = [count_n I ; [Xsum_n]x]      (6 x 3N),  w = [U; Omega]
+ * fl_swim_rhs writes R = [v | C] (3N x 7 row-major, rows in [point, component]
+ * order) from x (3, N) lab force points, x0_host (3 doubles, host) and
+ * v (3, N) lab velocities.  The caller solves Y = K^{-1} R with fl_?getrs or
+ * fl_refine (7 right-hand sides).  fl_swim_schur then forms S = B Y_C (6 x 6,
+ * written to S, device) and g = B Y_v in FP64 with a fixed-order reduction,
+ * solves S w = g (FP64, partial pivoting), writes w to UW (6 doubles,
+ * device) and f = Y_v - Y_C w to f (3N, device).  count (N) and Xsum (3, N)
+ * are the nearest-neighbour group sizes and quadrature-point sums.
+ */
+int fl_swim_rhs(int64_t N, const double* x, const double* x0_host, const double* v, double* R, void* stream);
+int fl_swim_schur(int64_t N, const double* Y, const double* count, const double* Xsum, double* f, double* UW,
+                  double* S, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,126 @@
+"""Free swimmer on the GPU (assemble_nearest + LU + Schur complement) against
+the oracle's dense bordered solve (swimming.py:66-102 restated) and the exact
+rigid-slip solutions of tests/test_swimming.py.  Bar: U, Omega and forces
+within 1e-10 relative of the FP64 bordered solve."""
+import numpy as np
+import pytest
+
+import oracle as O
+import fluidlets_amd as fl
+from fluidlets_amd.geometry import cube_sphere
+from fluidlets_amd.swimming import SwimmerShape, biflagellate, swim, swim_rk2, swimming_problem
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def sphere_shape(slip):
+    x = cube_sphere(4)
+    return SwimmerShape(x=x, v=slip(x), X=cube_sphere(10))
+
+
+def test_nearest_neighbors_match_kdtree():
+    rng = np.random.default_rng(0)
+    F, Q = rng.normal(size=(300, 3)), rng.normal(size=(2000, 3))
+    assert np.array_equal(fl.nearest_neighbors(Q, F), O.nearest_neighbors(Q, F))
+    X = cube_sphere(4)
+    assert np.array_equal(fl.nearest_neighbors(X, X), np.arange(len(X)))
+
+
+def test_assemble_nearest_matches_oracle():
+    """tests/test_swimming.py:12-26 plus entrywise parity with the oracle."""
+    K = fl.Stokeslet3D()
+    X = cube_sphere(4)
+    nn = fl.nearest_neighbors(X, X)
+    assert np.allclose(fl.assemble_nearest(K, X, X, nn, len(X), 0.1), fl.assemble(K, X, X, 0.1), rtol=1e-14)
+    x, Xq = cube_sphere(3), cube_sphere(9)
+    nn = fl.nearest_neighbors(Xq, x)
+    A = fl.assemble_nearest(K, x, Xq, nn, len(x), 0.05)
+    assert rel_err(A, O.assemble_nearest("stokeslet", x, Xq, nn, len(x), 0.05)) < 1e-13
+    f = np.random.default_rng(0).normal(size=(len(x), 3))
+    assert np.allclose(A @ f.ravel(), fl.evaluate(K, x, Xq, f[nn], 0.05).ravel(), rtol=1e-12)
+    W = fl.WallStokeslet3D()
+    xs = x + [0, 0, 2.0]
+    Xs = Xq + [0, 0, 2.0]
+    assert rel_err(fl.assemble_nearest(W, xs, Xs, nn, len(x), 0.05),
+                   O.assemble_nearest("wall", xs, Xs, nn, len(x), 0.05)) < 1e-13
+
+
+def test_uniform_slip_swims_rigidly():
+    """tests/test_swimming.py:34-40."""
+    V = np.array([0.2, -0.1, 0.7])
+    st = swimming_problem(sphere_shape(lambda x: -np.tile(V, (len(x), 1))), np.zeros(3), np.eye(3), 0.05)
+    assert np.allclose(st.U, V, atol=1e-12) and np.allclose(st.Omega, 0, atol=1e-12)
+    assert np.abs(st.forces).max() < 1e-10
+
+
+def test_rotational_slip_spins_rigidly():
+    """tests/test_swimming.py:43-50."""
+    W = np.array([0.0, 0.3, 1.0])
+    B = np.linalg.qr(np.random.default_rng(1).normal(size=(3, 3)))[0]
+    B *= np.sign(np.linalg.det(B))
+    x0 = np.array([1.0, 2.0, -1.0])
+    st = swimming_problem(sphere_shape(lambda x: -np.cross(B.T @ W, x)), x0, B, 0.05)
+    assert np.allclose(st.Omega, W, atol=1e-12) and np.allclose(st.U, 0, atol=1e-12)
+
+
+def test_rigid_body_without_slip_does_not_move():
+    """tests/test_swimming.py:53-55."""
+    st = swimming_problem(sphere_shape(np.zeros_like), np.zeros(3), np.eye(3), 0.05)
+    assert np.allclose(st.U, 0) and np.allclose(st.Omega, 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_biflagellate_matches_bordered_oracle(precision):
+    print(f"case: swimmer parity {precision}", flush=True)
+    shape_at = biflagellate(n_body=300, n_flag=120)
+    sh = shape_at(0.3)
+    B = np.linalg.qr(np.random.default_rng(4).normal(size=(3, 3)))[0]
+    B *= np.sign(np.linalg.det(B))
+    x0 = np.array([0.1, -0.2, 0.3])
+    st = swimming_problem(sh, x0, B, 0.01, precision=precision)
+    U, W, f, nn = O.swimming_problem("stokeslet", sh.x, sh.v, sh.X, x0, B, 0.01)
+    assert np.array_equal(st.nearest, nn)
+    assert rel_err(st.U, U) < 1e-10 and rel_err(st.Omega, W) < 1e-10
+    assert rel_err(st.forces, f) < 1e-10
+    # quadrature points finer than force points (nearest map not the identity)
+    sh2 = SwimmerShape(x=cube_sphere(4), v=np.cross([0.1, 0.2, 0.3], cube_sphere(4)) + [0.0, 0.0, 0.2],
+                       X=cube_sphere(9))
+    st2 = swimming_problem(sh2, x0, B, 0.05, precision=precision)
+    U2, W2, f2, _ = O.swimming_problem("stokeslet", sh2.x, sh2.v, sh2.X, x0, B, 0.05)
+    assert rel_err(st2.U, U2) < 1e-10 and rel_err(st2.Omega, W2) < 1e-10 and rel_err(st2.forces, f2) < 1e-10
+
+
+def test_trajectory_of_steady_slip_swimmer():
+    """tests/test_swimming.py:58-69 (scipy RK45 over device solves)."""
+    V = np.array([0.0, 0.0, 0.5])
+    W = np.array([0.0, 0.0, 0.25])
+
+    def shape_at(t):
+        x = cube_sphere(3)
+        return SwimmerShape(x=x, v=-(V + np.cross(W, x)), X=cube_sphere(6))
+
+    tr = swim(shape_at, (0.0, 2.0), 0.05, t_eval=[2.0], rtol=1e-9, atol=1e-12)
+    assert np.allclose(tr.x0[-1], [0, 0, 1.0], atol=1e-7)
+    angle = 0.25 * 2.0
+    assert np.allclose(tr.B[-1][:, 0], [np.cos(angle), np.sin(angle), 0], atol=1e-7)
+
+
+def test_rk2_is_second_order():
+    """Midpoint RK2 on the steady slip swimmer: the exact motion is a screw
+    (rotation about z at 0.25, drift along z at 0.5); halving dt divides the
+    orientation error by ~4."""
+    V = np.array([0.0, 0.0, 0.5])
+    W = np.array([0.0, 0.0, 0.25])
+
+    def shape_at(t):
+        x = cube_sphere(3)
+        return SwimmerShape(x=x, v=-(V + np.cross(W, x)), X=x)
+
+    errs = []
+    for steps in (5, 10):
+        tr = swim_rk2(shape_at, 0.0, 2.0 / steps, steps, 0.05)
+        assert np.allclose(tr.x0[-1], [0, 0, 1.0], atol=1e-10)
+        angle = 0.5
+        errs.append(np.abs(tr.B[-1][:, 0] - [np.cos(angle), np.sin(angle), 0]).max())
+    assert 3.0 < errs[0] / errs[1] < 5.0
