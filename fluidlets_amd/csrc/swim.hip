@@ -18,43 +18,25 @@ namespace {
 
 constexpr int WG = 256;
 
-// [a]x b = a x b; row r of [a]x
-__device__ __forceinline__ void cross_row(const double a[3], int r, double out[3])
-{
-    // [[0, -a2, a1], [a2, 0, -a0], [-a1, a0, 0]]
-    if (r == 0) {
-        out[0] = 0.0;
-        out[1] = -a[2];
-        out[2] = a[1];
-    } else if (r == 1) {
-        out[0] = a[2];
-        out[1] = 0.0;
-        out[2] = -a[0];
-    } else {
-        out[0] = -a[1];
-        out[1] = a[0];
-        out[2] = 0.0;
-    }
-}
-
+// One lane per point n writes rows 3n, 3n+1, 3n+2 of R = [v | -I | [x_n - x0]x]
+// ([a]x b = a x b; its rows are fixed at compile time, so no lane branches).
 __global__ __launch_bounds__(WG) void swim_rhs_kernel(int64_t N, const double* x, double x00, double x01, double x02,
                                                       const double* v, double* R)
 {
-    const int64_t idx = (int64_t)blockIdx.x * WG + threadIdx.x;
-    if (idx >= 3 * N) return;
-    const int64_t n = idx / 3;
-    const int a = (int)(idx % 3);
-    const double r[3] = {x[n] - x00, x[N + n] - x01, x[2 * N + n] - x02};
-    double c[3];
-    cross_row(r, a, c);
-    double* row = R + idx * 7;
-    row[0] = v[a * N + n];
-    row[1] = a == 0 ? -1.0 : 0.0;
-    row[2] = a == 1 ? -1.0 : 0.0;
-    row[3] = a == 2 ? -1.0 : 0.0;
-    row[4] = c[0];
-    row[5] = c[1];
-    row[6] = c[2];
+    const int64_t n = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (n >= N) return;
+    const double r0 = x[n] - x00, r1 = x[N + n] - x01, r2 = x[2 * N + n] - x02;
+    const double cx[3][3] = {{0.0, -r2, r1}, {r2, 0.0, -r0}, {-r1, r0, 0.0}};
+    double* row = R + 3 * n * 7;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        row[a * 7 + 0] = v[a * N + n];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            row[a * 7 + 1 + b] = a == b ? -1.0 : 0.0;
+            row[a * 7 + 4 + b] = cx[a][b];
+        }
+    }
 }
 
 // One workgroup.  BY[i][c], i < 6 rows of B, c < 7 columns of Y.
@@ -146,7 +128,7 @@ int swim_rhs_launch(int64_t N, const double* x, const double* x0, const double* 
     FL_CHECK(x0 != nullptr, "x0 is NULL");
     if (N == 0) return FL_OK;
     FL_CHECK(x && v && R, "null pointer argument");
-    hipLaunchKernelGGL(swim_rhs_kernel, dim3((unsigned)((3 * N + WG - 1) / WG)), dim3(WG), 0, st, N, x, x0[0], x0[1],
+    hipLaunchKernelGGL(swim_rhs_kernel, dim3((unsigned)((N + WG - 1) / WG)), dim3(WG), 0, st, N, x, x0[0], x0[1],
                        x0[2], v, R);
     FL_LAUNCHED();
     return FL_OK;

@@ -1,0 +1,311 @@
+"""Continuous-batching solve server on one GPU.
+
+Requests (rigid-body resistance solves of any size, free-swimmer time
+stepping) are admitted as they arrive whenever a slot is free and their HBM
+footprint fits the budget; each slot is a worker thread driving its own HIP
+stream (default priority), so admitted requests overlap on the device.  The
+HIP library calls release the GIL, and every kernel is deterministic (fixed
+summation orders, no floating-point atomics), so a request's result does not
+depend on what else is in flight: tests check it bit for bit, alone and in a
+mix.
+
+Resident factors: resistance requests carry a geometry key; the assembled
+FP64 matrix and the factors stay on the device in an LRU cache (counted in
+the memory budget), so another right-hand side on the same geometry costs
+only the triangular solves (and refinement).
+
+Latency is measured from a request's arrival to its completion (queueing
+included); the server reports p50/p99 and throughput.
+"""
+from __future__ import annotations
+
+import hashlib
+import heapq
+import threading
+import time
+from collections import OrderedDict
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+
+from . import _device as D
+
+
+@dataclass
+class Request:
+    kind: str                              # "resistance" | "swimmer"
+    payload: dict
+    arrival: float = 0.0                   # seconds, relative to the trace start
+    rid: int = 0
+
+
+@dataclass
+class Result:
+    rid: int
+    kind: str
+    value: Any
+    arrival: float
+    start: float
+    end: float
+    error: str | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.end - self.arrival
+
+
+def footprint_bytes(req: Request, precision: str) -> int:
+    """HBM bytes a request holds while it runs: the FP64 matrix (8 n^2), the
+    FP32 copy for the mixed path (4 n^2), plus O(n) vectors and workspaces."""
+    if req.kind == "resistance":
+        n = 3 * len(req.payload["points"])
+    else:
+        n = 3 * req.payload["n_points"]
+    per = 8 + (4 if precision == "mixed" else 8)
+    return int(per * n * n + 512 * n + (64 << 20))
+
+
+def geometry_key(kernel, points, eps) -> str:
+    h = hashlib.sha1()
+    h.update(repr(kernel).encode())
+    h.update(np.ascontiguousarray(points, dtype=np.float64).tobytes())
+    h.update(np.ascontiguousarray(np.asarray(eps, dtype=np.float64)).tobytes())
+    return h.hexdigest()
+
+
+class FactorCache:
+    """LRU of resident (K64, factors) per geometry key, bounded in bytes."""
+
+    def __init__(self, max_bytes: int):
+        self.max_bytes = max_bytes
+        self.bytes = 0
+        self.items: OrderedDict[str, tuple] = OrderedDict()
+        self.lock = threading.Lock()
+        self.hits = 0
+        self.misses = 0
+
+    def get(self, key):
+        with self.lock:
+            item = self.items.get(key)
+            if item is not None:
+                self.items.move_to_end(key)
+                self.hits += 1
+            else:
+                self.misses += 1
+            return item
+
+    def put(self, key, item, nbytes):
+        with self.lock:
+            if nbytes > self.max_bytes:
+                return
+            while self.bytes + nbytes > self.max_bytes and self.items:
+                _, old = self.items.popitem(last=False)
+                self.bytes -= old[-1]
+            self.items[key] = (*item, nbytes)
+            self.bytes += nbytes
+
+
+def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, stream=None):
+    """Six-mode grand resistance of one rigid body: factor (or reuse the
+    resident factors) and solve the six rigid-body right-hand sides."""
+    from . import linalg
+    from .core.assemble import assemble_device
+    from .core.solve import rigid_modes
+
+    t = D.torch()
+    kernel, X, center, eps = payload["kernel"], payload["points"], payload["center"], payload["eps"]
+    dev = D.device()
+    key = geometry_key(kernel, X, eps) + precision if cache is not None else None
+    item = cache.get(key) if cache is not None else None
+    rhs = t.from_numpy(rigid_modes(X, center)).to(dev)
+    if item is None:
+        x_soa = D.soa(X, dev)
+        eps_t, stride = D.eps_arg(eps, len(X), dev)
+        K64, K32 = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=(precision == "mixed"),
+                                   stream=stream)
+        if precision == "mixed":
+            F = linalg.getrf(K32, stream=stream)
+        else:
+            F = linalg.getrf(K64.clone(), stream=stream)
+        if F.info > 0:
+            raise np.linalg.LinAlgError("Matrix is singular.")
+        item = (K64, F)
+        if cache is not None:
+            cache.put(key, item, K64.numel() * 8 + F.nbytes())
+    K64, F = item[0], item[1]
+    if precision == "mixed":
+        x, steps, _ = linalg.refine(K64, F, rhs, tol=1e-10, stream=stream)
+    else:
+        x = rhs.clone()
+        linalg.getrs(F, x, stream=stream)
+    f = x.cpu().numpy().reshape(len(X), 3, 6).transpose(2, 0, 1) * float(np.real(payload.get("mu", 1.0)))
+    r = X - np.asarray(center, float)
+    R = np.zeros((6, 6))
+    for m in range(6):
+        R[:3, m] = f[m].sum(axis=0)
+        R[3:, m] = np.cross(r, f[m]).sum(axis=0)
+    return R
+
+
+def run_swimmer(payload: dict, precision: str, stream=None):
+    from .swimming import swim_rk2
+    tr = swim_rk2(payload["shape_at"], 0.0, payload["dt"], payload["steps"], payload["eps"],
+                  precision=precision, stream=stream)
+    return np.concatenate([tr.x0[-1], tr.B[-1].ravel()])
+
+
+class SolveServer:
+    """Continuous-batching server: `slots` concurrent requests, each on its
+    own stream, admitted by HBM footprint against `budget_bytes`."""
+
+    def __init__(self, slots: int = 4, precision: str = "mixed", budget_bytes: int | None = None,
+                 cache_bytes: int = 0):
+        t = D.torch()
+        dev = D.device()
+        if budget_bytes is None:
+            free, _ = t.cuda.mem_get_info(dev)
+            budget_bytes = int(0.7 * free)
+        self.slots = slots
+        self.precision = precision
+        self.budget = budget_bytes
+        self.cache = FactorCache(cache_bytes) if cache_bytes > 0 else None
+        self.streams = [t.cuda.Stream(device=dev) for _ in range(slots)]
+
+    def _execute(self, req: Request, stream):
+        t = D.torch()
+        with t.cuda.stream(stream):
+            if req.kind == "resistance":
+                out = solve_resistance(req.payload, self.precision, self.cache, stream=stream)
+            elif req.kind == "swimmer":
+                out = run_swimmer(req.payload, self.precision, stream=stream)
+            else:
+                raise ValueError(f"unknown request kind {req.kind!r}")
+            stream.synchronize()
+            return out
+
+    def serve(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+        """Serve a trace.  With realtime=True requests become visible at their
+        arrival times (Poisson traces); otherwise all are queued at t = 0."""
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        results: list[Result] = []
+        lock = threading.Condition()
+        state = {"in_use": 0, "running": 0, "t0": None, "next": 0}
+        free_streams = list(range(self.slots))
+
+        def now():
+            return time.perf_counter() - state["t0"]
+
+        def worker_body(req: Request, sid: int, fp: int):
+            start = now()
+            err, value = None, None
+            try:
+                value = self._execute(req, self.streams[sid])
+            except Exception as exc:  # a failing request is reported, the server keeps running
+                err = f"{type(exc).__name__}: {exc}"
+            end = now()
+            with lock:
+                results.append(Result(req.rid, req.kind, value, req.arrival if realtime else 0.0, start, end, err))
+                state["in_use"] -= fp
+                state["running"] -= 1
+                free_streams.append(sid)
+                lock.notify_all()
+
+        state["t0"] = time.perf_counter()
+        waiting: list = []
+        threads = []
+        i = 0
+        with lock:
+            while i < len(reqs) or waiting or state["running"]:
+                # move arrived requests into the waiting list (arrival order)
+                while i < len(reqs) and (not realtime or reqs[i].arrival <= now()):
+                    heapq.heappush(waiting, (reqs[i].arrival, reqs[i].rid, reqs[i]))
+                    i += 1
+                admitted = False
+                while waiting and free_streams:
+                    req = waiting[0][2]
+                    fp = footprint_bytes(req, self.precision)
+                    if fp > self.budget:
+                        heapq.heappop(waiting)
+                        results.append(Result(req.rid, req.kind, None, req.arrival if realtime else 0.0, now(), now(),
+                                              f"rejected: footprint {fp} B exceeds the budget {self.budget} B"))
+                        continue
+                    if state["in_use"] + fp > self.budget and state["running"] > 0:
+                        break  # wait for memory to free up
+                    heapq.heappop(waiting)
+                    sid = free_streams.pop()
+                    state["in_use"] += fp
+                    state["running"] += 1
+                    th = threading.Thread(target=worker_body, args=(req, sid, fp), daemon=True)
+                    threads.append(th)
+                    th.start()
+                    admitted = True
+                if admitted:
+                    continue
+                timeout = None
+                if realtime and i < len(reqs):
+                    timeout = max(0.0, reqs[i].arrival - now())
+                lock.wait(timeout=timeout if timeout is not None else 1.0)
+        for th in threads:
+            th.join()
+        return sorted(results, key=lambda r: r.rid)
+
+
+def summarize(results: list[Result]) -> dict:
+    ok = [r for r in results if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+    span = max((r.end for r in results), default=0.0)
+    return {
+        "requests": len(results),
+        "completed": len(ok),
+        "failed": len(results) - len(ok),
+        "makespan_s": span,
+        "solves_per_s": len(ok) / span if span > 0 else 0.0,
+        "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+        "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+        "latency_mean_s": float(lat.mean()) if len(lat) else None,
+    }
+
+
+def serving_mix(n_requests: int, seed: int = 5, n_min: int = 200, n_max: int = 6000, swimmer_frac: float = 0.3,
+                swimmer_steps: int = 200, swimmer_body: int = 1000, swimmer_flag: int = 400,
+                rate: float = 1.0) -> list[Request]:
+    """The serving workload: Poisson arrivals at `rate` per second; 70 %
+    rigid-body resistance solves on seeded ellipsoids (semi-axes U[0.5, 2],
+    N uniform in [n_min, n_max], Fibonacci points mapped onto the ellipsoid,
+    centre height U[1.2, 4], draws whose vertical semi-axis reaches within
+    eps of the wall rejected; WallStokeslet3D or free-space Brinkmanlet3D with
+    alpha ~ U[0, 10], 50/50; eps = 0.5 sqrt(area / N); six rigid-body
+    right-hand sides) and 30 % biflagellate swimmers stepped by midpoint RK2
+    (dt = one fiftieth of the beat period)."""
+    from .geometry import ellipsoid, ellipsoid_area
+    from .kernels import Brinkmanlet3D, WallStokeslet3D
+    from .swimming import biflagellate
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    shape_at = None
+    for rid in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < swimmer_frac:
+            if shape_at is None:
+                shape_at = biflagellate(n_body=swimmer_body, n_flag=swimmer_flag)
+            out.append(Request("swimmer", {"shape_at": shape_at, "dt": 1.0 / 50, "steps": swimmer_steps,
+                                           "eps": 0.01, "n_points": swimmer_body + 2 * swimmer_flag}, t, rid))
+            continue
+        while True:
+            axes = rng.uniform(0.5, 2.0, 3)
+            N = int(rng.integers(n_min, n_max + 1))
+            h = rng.uniform(1.2, 4.0)
+            eps = 0.5 * np.sqrt(ellipsoid_area(axes) / N)
+            if axes[2] < h - eps:
+                break
+        wall = rng.random() < 0.5
+        if wall:
+            kernel, center = WallStokeslet3D(), (0.0, 0.0, h)
+        else:
+            kernel, center = Brinkmanlet3D(rng.uniform(0.0, 10.0)), (0.0, 0.0, 0.0)
+        X = ellipsoid(N, axes, center)
+        out.append(Request("resistance", {"kernel": kernel, "points": X, "center": center, "eps": eps}, t, rid))
+    return out

@@ -92,9 +92,13 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
                                        t.from_numpy(start).to(dev), N, mu=mu, fp32=(precision == "mixed"),
                                        stream=stream)
     L = lib()
+    # every device buffer stays referenced until its kernels are enqueued
+    # (a temporary freed inside the argument list could be handed to the next
+    # allocation of the same call)
     R = t.empty((3 * N, 7), dtype=t.float64, device=dev)
+    v_soa = D.soa(v, dev)
     x0c = (ctypes.c_double * 3)(*x0.tolist())
-    check(L.fl_swim_rhs(N, D.ptr(x_soa), x0c, D.ptr(D.soa(v, dev)), D.ptr(R), D.stream_handle(stream)), "fl_swim_rhs")
+    check(L.fl_swim_rhs(N, D.ptr(x_soa), x0c, D.ptr(v_soa), D.ptr(R), D.stream_handle(stream)), "fl_swim_rhs")
     if precision == "fp64":
         F = linalg.getrf(K64, stream=stream)
         if F.info > 0:
@@ -113,8 +117,10 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
     f = t.empty((3 * N,), dtype=t.float64, device=dev)
     UW = t.empty((6,), dtype=t.float64, device=dev)
     S = t.empty((36,), dtype=t.float64, device=dev)
-    check(L.fl_swim_schur(N, D.ptr(Y), D.ptr(t.from_numpy(count).to(dev)), D.ptr(D.soa(Xsum, dev)), D.ptr(f),
-                          D.ptr(UW), D.ptr(S), D.stream_handle(stream)), "fl_swim_schur")
+    count_d = t.from_numpy(count).to(dev)
+    Xsum_d = D.soa(Xsum, dev)
+    check(L.fl_swim_schur(N, D.ptr(Y), D.ptr(count_d), D.ptr(Xsum_d), D.ptr(f), D.ptr(UW), D.ptr(S),
+                          D.stream_handle(stream)), "fl_swim_schur")
     w = UW.cpu().numpy()
     return SwimState(w[:3].copy(), w[3:].copy(), f.cpu().numpy().reshape(N, 3), x, X, nearest)
 
