@@ -111,7 +111,9 @@ def test_trajectory_of_steady_slip_swimmer():
 def test_rk2_is_second_order():
     """Midpoint RK2 on the steady slip swimmer: the exact motion is a screw
     (rotation about z at 0.25, drift along z at 0.5); halving dt divides the
-    orientation error by ~4."""
+    position and orientation errors by ~4.  (The midpoint frame is not
+    re-orthonormalised, as in the reference's swim, so b1 x b2 is stretched
+    by O(dt^2) and so is the drift.)"""
     V = np.array([0.0, 0.0, 0.5])
     W = np.array([0.0, 0.0, 0.25])
 
@@ -119,10 +121,11 @@ def test_rk2_is_second_order():
         x = cube_sphere(3)
         return SwimmerShape(x=x, v=-(V + np.cross(W, x)), X=x)
 
-    errs = []
+    errs, xerrs = [], []
     for steps in (5, 10):
         tr = swim_rk2(shape_at, 0.0, 2.0 / steps, steps, 0.05)
-        assert np.allclose(tr.x0[-1], [0, 0, 1.0], atol=1e-10)
+        xerrs.append(np.abs(tr.x0[-1] - [0, 0, 1.0]).max())
         angle = 0.5
         errs.append(np.abs(tr.B[-1][:, 0] - [np.cos(angle), np.sin(angle), 0]).max())
+    assert xerrs[0] < 5e-3 and 3.0 < xerrs[0] / xerrs[1] < 5.0
     assert 3.0 < errs[0] / errs[1] < 5.0
