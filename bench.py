@@ -188,6 +188,74 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=1):
     return out
 
 
+def serving_section(torch, n_requests=40, swimmer_steps=10, n_max=3000, slots=4, seed=5):
+    """Serving mix (70 % six-mode resistance solves on ellipsoids, 30 %
+    biflagellate RK2 swimmers), reduced in size so the default run stays
+    short: n_requests requests, N <= n_max, swimmer_steps RK2 steps.  The
+    same trace is served one request at a time in plain FP64 (the baseline)
+    and by the continuous-batching server (mixed precision, `slots` streams):
+    offline (all requests queued at t = 0) and online (Poisson arrivals at
+    0.9 x the baseline's measured capacity)."""
+    from fluidlets_amd.serving import Request, SolveServer, serving_mix, summarize
+
+    reqs = serving_mix(n_requests, seed=seed, n_max=n_max, swimmer_steps=swimmer_steps, rate=1.0)
+    base_srv = SolveServer(slots=1, precision="fp64")
+    fast_srv = SolveServer(slots=slots, precision="mixed")
+    base_srv.serve(reqs[:2], realtime=False)           # warm-up (JIT-free, but first allocations)
+    fast_srv.serve(reqs[:2], realtime=False)
+    out = {"requests": n_requests, "slots": slots, "swimmer_steps": swimmer_steps, "n_max": n_max}
+    off_base = summarize(base_srv.serve(reqs, realtime=False))
+    off_fast = summarize(fast_srv.serve(reqs, realtime=False))
+    out["offline_baseline_fp64_one_at_a_time"] = off_base
+    out["offline_server_mixed"] = off_fast
+    lam = 0.9 * off_base["solves_per_s"]
+    t = 0.0
+    rng = np.random.default_rng(seed + 1)
+    trace = []
+    for r in reqs:
+        t += rng.exponential(1.0 / lam)
+        trace.append(Request(r.kind, r.payload, t, r.rid))
+    out["arrival_rate_per_s"] = lam
+    out["online_baseline_fp64_one_at_a_time"] = summarize(base_srv.serve(trace, realtime=True))
+    out["online_server_mixed"] = summarize(fast_srv.serve(trace, realtime=True))
+    out["offline_throughput_speedup"] = off_fast["solves_per_s"] / off_base["solves_per_s"]
+    return out
+
+
+def gmres_section(torch, fl, D, precision_list=("fp64", "mixed"), n_side=32, per_rod=128, restart=50,
+                  rtol=1e-8, maxiter=40):
+    """Matrix-free GMRES(restart) on the fixed-rod carpet: 32 x 32 rods of
+    128 helical-lattice points (131,072 points), WallStokeslet3D, eps = 0.01,
+    rods held fixed in the background shear u = (z, 0, 0) (right-hand side
+    -(z, 0, 0)); FP64 GMRES and flexible GMRES with FP32C inner cycles."""
+    from fluidlets_amd.core.gmres import gmres_device
+    from fluidlets_amd.geometry import rod_carpet
+
+    X = rod_carpet(n_side=n_side, per_rod=per_rod)
+    N = len(X)
+    dev = D.device()
+    x_soa = D.soa(X, dev)
+    eps_t, stride = D.eps_arg(0.01, N, dev)
+    U = np.zeros((N, 3))
+    U[:, 0] = -X[:, 2]
+    U_soa = D.soa(U, dev)
+    out = {"points": N, "restart": restart, "rtol": rtol}
+    ref = None
+    for prec in precision_list:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f, its, res = gmres_device(fl.WallStokeslet3D(), x_soa, eps_t, stride, U_soa, rtol=rtol, restart=restart,
+                                   maxiter=maxiter, precision=prec)
+        torch.cuda.synchronize()
+        row = {"seconds": time.perf_counter() - t0, "iterations": its, "residual": res}
+        if ref is None:
+            ref = f.clone()
+        else:
+            row["force_relerr_vs_fp64"] = (torch.linalg.norm(f - ref) / torch.linalg.norm(ref)).item()
+        out[prec] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +267,11 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the N=8192 resistance-solve section")
     ap.add_argument("--solve-kernels", default="wall", help="comma list: wall, brinkman:1, brinkman:5, ...")
     ap.add_argument("--solve-points", type=int, default=8192)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix section")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-swimmer-steps", type=int, default=10)
+    ap.add_argument("--serve-nmax", type=int, default=3000)
+    ap.add_argument("--gmres", action="store_true", help="run the 131,072-point GMRES rod-carpet section")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -292,6 +365,12 @@ def main():
     if not args.no_solve:
         result["wall_sphere_solve"] = c3_solves(torch, fl, D, tuple(args.solve_kernels.split(",")),
                                                 args.solve_points)
+
+    if not args.no_serve:
+        result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax)
+
+    if args.gmres:
+        result["rod_carpet_gmres"] = gmres_section(torch, fl, D)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(X, F, args.cpu_targets)
