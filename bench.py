@@ -77,7 +77,7 @@ def cpu_baseline(X, F, n_targets: int):
                       f"core/matvec.py:_apply with the reference's pow()-based block, -O3, no fast-math"}
 
 
-def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=1):
+def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=1, vendor=True):
     """Mixed-precision wall resistance solve: Fibonacci sphere (radius 1,
     centre (0,0,1.5)), eps = 0.5 sqrt(4 pi / N), six rigid-body right-hand
     sides.  Times FP32-MFMA LU + FP64 refinement to 1e-10, the FP64 LU of this
@@ -166,6 +166,8 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=1):
         row["speedup_vs_fp64_lu"] = best64 / bestm
         # rocSOLVER FP64 (torch.linalg) as the library baseline
         try:
+            if not vendor:
+                raise RuntimeError("vendor LU skipped (--no-vendor-lu)")
             A = K64.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -267,11 +269,14 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the N=8192 resistance-solve section")
     ap.add_argument("--solve-kernels", default="wall", help="comma list: wall, brinkman:1, brinkman:5, ...")
     ap.add_argument("--solve-points", type=int, default=8192)
+    ap.add_argument("--no-vendor-lu", action="store_true", help="skip the rocSOLVER FP64 LU comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix section")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-swimmer-steps", type=int, default=10)
     ap.add_argument("--serve-nmax", type=int, default=3000)
     ap.add_argument("--gmres", action="store_true", help="run the 131,072-point GMRES rod-carpet section")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (the reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -292,6 +297,15 @@ def main():
     wall = fl.WallStokeslet3D()
     u = torch.empty((3, N_POINTS), dtype=torch.float64, device=dev)
     pairs = float(N_POINTS) * N_POINTS
+
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         evaluate_device(wall, x, x, eps_t, stride, f, u_re=u)
@@ -364,7 +378,7 @@ def main():
 
     if not args.no_solve:
         result["wall_sphere_solve"] = c3_solves(torch, fl, D, tuple(args.solve_kernels.split(",")),
-                                                args.solve_points)
+                                                args.solve_points, vendor=not args.no_vendor_lu)
 
     if not args.no_serve:
         result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax)

@@ -277,3 +277,51 @@ def test_planted_mistakes_fail_the_pins(kernels_mat):
         v = O.pairwise_sum(lambda x, x0, f, e: image_bad(x, x0, f, e, which=which), d["targets"], d["sources"],
                            d["strengths"], d["deltas"])
         assert not np.allclose(v, d["V_image"], rtol=1e-13, atol=1e-15)
+
+
+def test_planted_stokeslet_mistakes_fail_the_pins(kernels_mat):
+    """A dropped 2 eps^2 term in H1, the target's blob width used in place of
+    the source's (an index mistake) and swapped target/source operands each
+    fail the MATLAB Stokeslet fixture (per-source deltas)."""
+    d = kernels_mat
+    T, S, F, dl = d["targets"], d["sources"], d["strengths"], d["deltas"]
+
+    def st(x, x0, f, eps, drop_eps=False):
+        r = np.asarray(x, float) - np.asarray(x0, float)
+        Q = r @ r + eps**2
+        h1 = (r @ r + (0.0 if drop_eps else 2 * eps**2)) / Q**1.5
+        return (h1 * f + (r @ f) * r / Q**1.5) / (8 * np.pi)
+
+    good = O.pairwise_sum(st, T, S, F, dl)
+    assert np.allclose(good, d["V_stokeslet"], rtol=1e-13, atol=1e-15)
+    bad_eps = O.pairwise_sum(lambda x, x0, f, e: st(x, x0, f, e, drop_eps=True), T, S, F, dl)
+    assert not np.allclose(bad_eps, d["V_stokeslet"], rtol=1e-13, atol=1e-15)
+    # blob width of target i instead of source j (targets outnumber sources; reuse cyclically)
+    bad_idx = np.array([sum(st(T[i], S[j], F[j], dl[i % len(dl)]) for j in range(len(S))) for i in range(len(T))])
+    assert not np.allclose(bad_idx, d["V_stokeslet"], rtol=1e-13, atol=1e-15)
+    # operands swapped in the displacement: d = x0 - x flips nothing in d d^T but the
+    # wall image is not symmetric in (x, x0), so swap there
+    w = O.pairwise_sum(O.wall_velocity, T + [0, 0, 3.0], S + [0, 0, 3.0], F, dl)
+    w_bad = O.pairwise_sum(lambda x, x0, f, e: O.wall_velocity(x0, x, f, e), T + [0, 0, 3.0], S + [0, 0, 3.0], F, dl)
+    assert not np.allclose(w, w_bad, rtol=1e-6)
+
+
+def test_planted_series_mistake_fails_the_high_precision_pin():
+    """The Brinkmanlet series with one coefficient's sign flipped misses the
+    mpmath values below z = 0.5 by far more than the 1e-14 bar."""
+    from math import factorial
+    from conftest import golden
+    g = golden("brinkman_mp.npz")
+
+    def f1_series(z, flip=None):
+        s = 0.0
+        for n in range(23, -1, -1):
+            c = (-1) ** n * (1 / factorial(n + 2) - 1 / factorial(n + 1) + 1 / factorial(n))
+            s = s * z + (-c if n == flip else c)
+        return s
+
+    small = [(z, e1) for z, e1 in zip(g["z"], g["f1"]) if 1e-6 < z < 0.5]
+    assert small
+    for z, e1 in small:
+        assert abs(f1_series(z) / e1 - 1) < 1e-14
+        assert abs(f1_series(z, flip=1) / e1 - 1) > 1e-6
