@@ -5,7 +5,11 @@ Workload (the N-body kernel-sum drop-in): 262,144 points, sources = targets,
 seeded uniform in [-4,4]^2 x [0.05,4] (numpy default_rng(2)), forces i.i.d.
 N(0,1), eps = 0.02, mu = 1.  One step = one fl.evaluate of the fused
 WallStokeslet3D kernel sum in FP64 over all 6.87e10 target-source pairs, with
-inputs resident in HBM.  value = pair evaluations per second.
+inputs resident in HBM.  value = pair evaluations per second (M N / t).
+Because the targets are the sources and the blob width is uniform, the
+library takes its reciprocal path: the block of each pair in different
+512-point tiles is evaluated once and applied to both points (G(y, x) =
+G(x, y)^T); the roofline counts the flops that path performs.
 
 Also reported in the same JSON line: the other kernels of the workload
 (Stokeslet3D, Brinkmanlet3D(4), Stokeslet3D with mu = 1+0.5i) in FP64 and
@@ -34,6 +38,20 @@ SEED = 2
 # FP64 flops per pair (add/sub/mul = 1, FMA = 2, sqrt/rsqrt/div/exp = 1;
 # per-source constants amortised; real strengths).
 FLOPS_PER_PAIR = {"wall": 116, "stokeslet": 32, "brinkman": 50}
+# The transposed apply of the reciprocal (targets = sources) evaluation, by the
+# same rules: w = d0 g0 + d1 g1 (3), v = b w + t g2 (3), three outputs (4 each).
+FLOPS_TRANSPOSED_APPLY = {"wall": 18, "stokeslet": 17, "brinkman": 17}
+SYM_TILE = 512
+
+
+def sym_flops(n: int, fam: str) -> float:
+    """Algorithmic flops of one reciprocal evaluation over n points: pairs in
+    different 512-point tiles are evaluated once and applied both ways; pairs
+    inside a tile (self-pairs included) are evaluated per ordered pair."""
+    sizes = [min(SYM_TILE, n - k) for k in range(0, n, SYM_TILE)]
+    inside = float(sum(m * m for m in sizes))
+    across = (float(n) * n - inside) / 2
+    return across * (FLOPS_PER_PAIR[fam] + FLOPS_TRANSPOSED_APPLY[fam]) + inside * FLOPS_PER_PAIR[fam]
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and FP64 matrix) peak
 HBM_PEAK_GBS = 8000.0
 
@@ -324,8 +342,11 @@ def main():
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     ms_per_step = 1e3 * wall_s / args.steps
     value = pairs / (wall_s / args.steps)
-    achieved = pairs * FLOPS_PER_PAIR["wall"] / (kernel_ms * 1e-3) / 1e12
-    traffic = profile_traffic("nbody_wall_c2_traffic.txt")
+    from fluidlets_amd._lib import lib as _lib
+    symmetric = os.environ.get("FL_EVAL_SYMMETRIC", "1") != "0" and _lib().fl_eval_sym_workspace_size(N_POINTS, 0, 0) > 0
+    flops = sym_flops(N_POINTS, "wall") if symmetric else pairs * FLOPS_PER_PAIR["wall"]
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    traffic = profile_traffic("nbody_wall_c2_sym_traffic.txt" if symmetric else "nbody_wall_c2_traffic.txt")
 
     result = {
         "metric": "pair-evals/s (FP64)",
@@ -346,8 +367,12 @@ def main():
         "roofline": {"bound": "valu", "pipe": "FP64 vector ALU (compute bound; HBM traffic is ~56 B per point)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "fl::nbody64<WallStokeslet3D>", "kernel_ms": kernel_ms,
-                     "flops_per_pair": FLOPS_PER_PAIR["wall"]},
+                     "kernel": ("fl::nbody64_sym<WallStokeslet3D> (reciprocal: each cross-tile pair once, "
+                                "applied both ways)") if symmetric else "fl::nbody64<WallStokeslet3D>",
+                     "kernel_ms": kernel_ms, "flops_per_launch": flops,
+                     "flops_per_pair": FLOPS_PER_PAIR["wall"],
+                     "flops_per_cross_tile_pair": FLOPS_PER_PAIR["wall"] + FLOPS_TRANSPOSED_APPLY["wall"]
+                     if symmetric else None},
     }
 
     if not args.no_table:
@@ -355,20 +380,27 @@ def main():
         ref = {}
         cases = [("wall", fl.WallStokeslet3D(), 1.0), ("stokeslet", fl.Stokeslet3D(), 1.0),
                  ("brinkman4", fl.Brinkmanlet3D(4.0), 1.0), ("stokeslet_mu1+0.5i", fl.Stokeslet3D(), 1 + 0.5j)]
+        x_copy = x.clone()  # the same points as a separate buffer: the general (non-reciprocal) path
         for name, K, mu in cases:
-            for prec in ["fp64", "fp32c"]:
-                evaluate_device(K, x, x, eps_t, stride, f, mu=mu, precision=prec)
+            for prec in ["fp64", "fp64_general", "fp32c"]:
+                if prec == "fp64_general" and name != "wall":
+                    continue
+                src_t = x_copy if prec == "fp64_general" else x
+                p_arg = "fp64" if prec == "fp64_general" else prec
+                evaluate_device(K, x, src_t, eps_t, stride, f, mu=mu, precision=p_arg)
                 torch.cuda.synchronize()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
-                ur, ui = evaluate_device(K, x, x, eps_t, stride, f, mu=mu, precision=prec)
+                ur, ui = evaluate_device(K, x, src_t, eps_t, stride, f, mu=mu, precision=p_arg)
                 e1.record(stream)
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1)
                 uu = ur if ui is None else torch.complex(ur, ui)
                 row = {"pair_evals_per_s": pairs / (ms * 1e-3), "ms": ms}
                 kfam = "wall" if name == "wall" else ("brinkman" if name.startswith("brinkman") else "stokeslet")
-                row["tflops_fp64_convention"] = pairs * FLOPS_PER_PAIR[kfam] / (ms * 1e-3) / 1e12
+                fl_count = sym_flops(N_POINTS, kfam) if (symmetric and prec == "fp64") else pairs * FLOPS_PER_PAIR[kfam]
+                row["tflops_fp64_convention"] = fl_count / (ms * 1e-3) / 1e12
+                row["path"] = "reciprocal" if (symmetric and prec == "fp64") else "general"
                 if prec == "fp64":
                     ref[name] = uu.clone()
                 else:

@@ -24,6 +24,7 @@ _SIGNATURES = {
     "fl_get_last_error": (c_int, [ctypes.c_char_p, c_size_t]),
     "fl_device_cus": (c_int, []),
     "fl_eval_workspace_size": (c_size_t, [c_int64, c_int64, c_int, c_int]),
+    "fl_eval_sym_workspace_size": (c_size_t, [c_int64, c_int, c_int]),
     "fl_eval": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                         c_void_p, c_double, c_double, c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "fl_assemble": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_double,

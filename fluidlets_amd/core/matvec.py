@@ -38,6 +38,9 @@ def evaluate_device(kernel, tgt, src, eps_t, eps_stride, f_re, f_im=None, mu=1.0
     prec = _precision(precision)
     L = lib()
     nbytes = L.fl_eval_workspace_size(M, N, int(f_im is not None), prec)
+    if M == N and int(eps_stride) == 0 and tgt.data_ptr() == src.data_ptr():
+        # targets are the sources: room for the symmetric (reciprocal) evaluation
+        nbytes = max(nbytes, L.fl_eval_sym_workspace_size(N, int(f_im is not None), prec))
     ws = D.workspace(nbytes, stream)
     k = kernel_struct(kernel)
     st = L.fl_eval(k, M, N, D.ptr(tgt), D.ptr(src), D.ptr(eps_t), int(eps_stride), D.ptr(f_re), D.ptr(f_im),
@@ -67,11 +70,15 @@ def evaluate(kernel, targets, sources, strengths, eps, mu=1.0, precision="fp64")
     if M == 0 or N == 0:
         return np.zeros((M, p), dtype=dtype) / mu
     dev = D.device()
-    tgt, src = D.soa(targets, dev), D.soa(sources, dev)
+    tgt = D.soa(targets, dev)
+    same = targets is sources or (targets.shape == sources.shape and np.array_equal(targets, sources))
+    src = tgt if same else D.soa(sources, dev)
     e = eps if not (N > 1 and np.all(eps == eps[0])) else eps[:1]
     eps_t, stride = D.eps_arg(e, N, dev)
-    if np.iscomplexobj(strengths):
+    if np.iscomplexobj(strengths) and np.any(strengths.imag != 0):
         f_re, f_im = D.soa(strengths.real, dev), D.soa(strengths.imag, dev)
+    elif np.iscomplexobj(strengths):
+        f_re, f_im = D.soa(np.ascontiguousarray(strengths.real), dev), None
     else:
         f_re, f_im = D.soa(strengths, dev), None
     u_re, u_im = evaluate_device(kernel, tgt, src, eps_t, stride, f_re, f_im, mu, precision=precision)

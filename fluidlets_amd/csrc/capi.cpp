@@ -127,6 +127,11 @@ size_t fl_eval_workspace_size(int64_t M, int64_t N, int complex_strengths, int p
     return eval_workspace_size(M, N, complex_strengths ? 2 : 1, precision);
 }
 
+size_t fl_eval_sym_workspace_size(int64_t N, int complex_strengths, int precision)
+{
+    return eval_sym_workspace_size(N, complex_strengths ? 2 : 1, precision);
+}
+
 int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N, const double* tgt, const double* src, const double* eps,
             int64_t eps_stride, const double* f_re, const double* f_im, double mu_re, double mu_im, double* u_re,
             double* u_im, int precision, void* workspace, size_t workspace_bytes, void* stream)

@@ -47,6 +47,10 @@ int device_cus();
 // u_re receives the scaled real part, u_im (nullable) the imaginary part.
 // ws must hold eval_workspace_size() bytes.
 size_t eval_workspace_size(int64_t M, int64_t N, int nf, int precision);
+// Workspace of the symmetric path (tgt == src, M == N, one blob width, real
+// strengths, FP64), 0 when that path is not used for this size; eval_launch
+// takes it when ws holds at least this many bytes.
+size_t eval_sym_workspace_size(int64_t N, int nf, int precision);
 int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt, const double* src,
                 const double* eps, int64_t eps_stride, const double* f, const double* fi, double w_re, double w_im,
                 double* u_re, double* u_im, int precision, void* ws, size_t ws_bytes, hipStream_t stream);

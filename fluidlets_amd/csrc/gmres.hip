@@ -207,8 +207,9 @@ size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision)
     const size_t vec = align_up((size_t)n * sizeof(double), 256);
     size_t b = (size_t)(restart + 1) * vec + 2 * vec + 4096;  // V, w, r, dots
     if (inner_precision == FL_FP32C) b += (size_t)restart * vec + (size_t)(kInnerRestart + 1) * vec + vec;
-    b += align_up(eval_workspace_size(N, N, 1, inner_precision), 256) +
-         align_up(eval_workspace_size(N, N, 1, FL_FP64), 256);
+    b += align_up(std::max({eval_workspace_size(N, N, 1, inner_precision), eval_workspace_size(N, N, 1, FL_FP64),
+                            eval_sym_workspace_size(N, 1, FL_FP64)}),
+                  256);
     return b;
 }
 
@@ -248,7 +249,8 @@ int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double*
     }
     Op op{kind, alpha, N, X, eps, eps_stride, kernel_scale(kind) / mu_re, nullptr, 0, st};
     op.ews = p;
-    op.ews_bytes = std::max(eval_workspace_size(N, N, 1, FL_FP64), eval_workspace_size(N, N, 1, inner_precision));
+    op.ews_bytes = std::max({eval_workspace_size(N, N, 1, FL_FP64), eval_workspace_size(N, N, 1, inner_precision),
+                             eval_sym_workspace_size(N, 1, FL_FP64)});
 
     double bnorm = 0.0;
     if (int e = dot_host(U, U, n, s, st, bnorm)) return e;

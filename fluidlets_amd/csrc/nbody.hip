@@ -97,52 +97,65 @@ __device__ __forceinline__ void pair_stokeslet(double d0, double d1, double d2, 
     }
 }
 
-// Free Stokeslet + no-slip image system for a source at height h0 (hh = h0^2).
-// d0, d1: in-plane displacement; dzf = x3 - h0; dzi = x3 + h0 (from the image).
-// Image coefficients are doubled so that both parts share the 1/(8 pi) scale:
-//   A1 = 2 H1, s3 = 2 H2, P1 = 2 Hdp1, P2 = 2 Hdp2, B1..B3 = 2 Hdb1..3, R1 = 2 Hr1.
-// Applied to f with w = d0 f0 + d1 f1, g.d = dzi f2 - w (the dipole's sign
-// flip on the in-plane columns), the 3x3 image block of images.py:48-74 is
-//   rows 0,1:  (-A1 - P1 + dzi (R1 - B2)) f_a + d_a [P2 g.d - s3 f.d + B2 f2 + B3 dzi g.d]
-//   row 2:     (-A1 + P1 + 2 dzi B2) f2 + dzi [P2 g.d - s3 f.d] + (B1 + dzi^2 B3) g.d - R1 w
-template <int NF>
-__device__ __forceinline__ void pair_wall(double d0, double d1, double dzf, double dzi, double e2, double h0,
-                                          double hh, const double* f, double* acc)
-{
-    double rho2 = d0 * d0 + d1 * d1;
-    double Qf = rho2 + dzf * dzf + e2;
-    double rf = rsqrt_pos(Qf);
-    double s3f = rf * rf * rf;
-    double h1f = (Qf + e2) * s3f;
+// Free Stokeslet + no-slip image system (kernels/images.py:26-104) as one
+// 3x3 block, in units of 1/(8 pi).  For a target at height z and a source at
+// height h (dzf = z - h, dzi = z + h, in-plane displacement d0, d1), summing
+// the free block and the image block of images.py:48-74 (image coefficients
+// doubled to share the 1/(8 pi) scale) and collecting terms gives
+//   G = [[C + b d0^2,  b d0 d1,     g d0],
+//        [b d0 d1,     C + b d1^2,  g d1],
+//        [t d0,        t d1,        Z   ]]
+// with Qf = r_f^2 + e2, Qi = r_i^2 + e2, s3f = Qf^-3/2, s3 = Qi^-3/2,
+// s5 = Qi^-5/2, h1f = (Qf + e2) s3f, a = s3f - s3, k = 6 h z s5:
+//   b = a + k,  g = dzf a - k dzi,  t = dzf a + k dzi,
+//   C = h1f - Qi^-1/2 - s3 (e2 + 2 h z) + e2 k,
+//   Z = h1f + dzf^2 s3f - Qi^-1/2 - s3 (e2 + h^2 + z^2) - k (e2 + dzi^2).
+// (Swapping target and source maps G to its transpose: g <-> -t under
+// d -> -d, the reciprocal symmetry of the image system.)
+struct WallBlock {
+    double C, b, g, t, Z;
+};
 
-    double Qi = rho2 + dzi * dzi + e2;
-    double ri = rsqrt_pos(Qi);
-    double ri2 = ri * ri;
-    double s3 = ri2 * ri;
-    double s5 = s3 * ri2;
-    double e2s5 = e2 * s5;
-    double A1 = ri + e2 * s3;
-    double P1 = -2.0 * hh * (s3 - 3.0 * e2s5);
-    double P2 = 6.0 * hh * s5;
-    double B1 = -2.0 * h0 * (3.0 * e2s5 + s3);
-    double B2 = 2.0 * h0 * s3;
-    double B3 = -6.0 * h0 * s5;
-    double R1 = 6.0 * h0 * e2s5;
-    double cxy = h1f - A1 - P1 + dzi * (R1 - B2);
-    double cz = h1f - A1 + P1 + 2.0 * dzi * B2;
-    double bz = B1 + dzi * dzi * B3;
-    double b3z = B3 * dzi;
+// h2 = 2h, h6 = 6h, ehh = e2 + h^2 (per source); zz = z^2 (per target).
+__device__ __forceinline__ WallBlock wall_block(double d0, double d1, double z, double h, double e2, double h2,
+                                                double h6, double ehh, double zz)
+{
+    const double dzf = z - h, dzi = z + h;
+    const double tq = fma(d0, d0, d1 * d1) + e2;
+    const double Qf = fma(dzf, dzf, tq);
+    const double Qi = fma(dzi, dzi, tq);
+    const double rf = rsqrt_pos(Qf);
+    const double ri = rsqrt_pos(Qi);
+    const double s3f = rf * rf * rf;
+    const double ri2 = ri * ri;
+    const double s3 = ri2 * ri;
+    const double s5 = s3 * ri2;
+    const double hr = (Qf + e2) * s3f - ri;
+    const double k = (h6 * z) * s5;
+    const double a = s3f - s3;
+    const double kd = k * dzi;
+    WallBlock B;
+    B.b = a + k;
+    B.g = fma(dzf, a, -kd);
+    B.t = fma(dzf, a, kd);
+    B.C = fma(e2, k, fma(-s3, fma(h2, z, e2), hr));
+    B.Z = fma(-k, fma(dzi, dzi, e2), fma(-s3, ehh + zz, fma(dzf * dzf, s3f, hr)));
+    return B;
+}
+
+template <int NF>
+__device__ __forceinline__ void pair_wall(double d0, double d1, double z, double h, double e2, double h2, double h6,
+                                          double ehh, double zz, const double* f, double* acc)
+{
+    const WallBlock B = wall_block(d0, d1, z, h, e2, h2, h6, ehh, zz);
 #pragma unroll
     for (int r = 0; r < NF; ++r) {
-        double f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
-        double w = d0 * f0 + d1 * f1;
-        double tf = s3f * (w + dzf * f2);
-        double gd = dzi * f2 - w;
-        double bp = P2 * gd - s3 * (w + dzi * f2);
-        double bv = bp + B2 * f2 + b3z * gd + tf;
-        acc[3 * r] += cxy * f0 + d0 * bv;
-        acc[3 * r + 1] += cxy * f1 + d1 * bv;
-        acc[3 * r + 2] += cz * f2 + dzf * tf + dzi * bp + bz * gd - R1 * w;
+        const double f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
+        const double w = fma(d0, f0, d1 * f1);
+        const double v = fma(B.b, w, B.g * f2);
+        acc[3 * r] = fma(B.C, f0, fma(d0, v, acc[3 * r]));
+        acc[3 * r + 1] = fma(B.C, f1, fma(d1, v, acc[3 * r + 1]));
+        acc[3 * r + 2] = fma(B.t, w, fma(B.Z, f2, acc[3 * r + 2]));
     }
 }
 
@@ -315,21 +328,38 @@ __device__ __forceinline__ void store_result(const NbArgs& a, int64_t i, const d
     }
 }
 
-// LDS record of one source for the FP64 kernels: x, y, z, eps^2, h0^2, f (3*NF).
-template <int NF>
+// LDS record of one source for the FP64 kernels: x, y, z, eps^2, then for the
+// wall kernel 2h, 6h, eps^2 + h^2 (h = z, the source height), then f (3*NF).
+template <int KIND, int NF>
 struct Src64 {
-    static constexpr int W = (5 + 3 * NF + 1) / 2 * 2;  // doubles, even for 16-byte reads
+    static constexpr int FOFF = KIND == FL_WALL_STOKESLET3D ? 7 : 4;  // offset of the strengths
+    static constexpr int W = (FOFF + 3 * NF + 1) / 2 * 2;             // doubles, even for 16-byte reads
 };
+
+template <int KIND, int NF>
+__device__ __forceinline__ void fill_src64(double* rec, double x, double y, double z, double e2)
+{
+    rec[0] = x;
+    rec[1] = y;
+    rec[2] = z;
+    rec[3] = e2;
+    if (KIND == FL_WALL_STOKESLET3D) {
+        rec[4] = 2.0 * z;
+        rec[5] = 6.0 * z;
+        rec[6] = fma(z, z, e2);
+    }
+}
 
 template <int KIND, int NF, int TPL>
 __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
 {
-    constexpr int W = Src64<NF>::W;
+    constexpr int W = Src64<KIND, NF>::W;
+    constexpr int FO = Src64<KIND, NF>::FOFF;
     __shared__ __attribute__((aligned(16))) double lds[TILE * W];
     const int tid = threadIdx.x;
     const int64_t M = a.M, N = a.N;
 
-    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
+    double tx[TPL], ty[TPL], tz[TPL], tzz[TPL], acc[TPL][3 * NF];
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
         int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
@@ -337,6 +367,7 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
         tx[t] = a.tgt[ic];
         ty[t] = a.tgt[M + ic];
         tz[t] = a.tgt[2 * M + ic];
+        tzz[t] = tz[t] * tz[t];
 #pragma unroll
         for (int k = 0; k < 3 * NF; ++k) acc[t][k] = 0.0;
     }
@@ -349,14 +380,11 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
 #pragma unroll
         for (int k = 0; k < W; ++k) rec[k] = 0.0;
         if (j < je) {
-            rec[0] = a.src[j];
-            rec[1] = a.src[N + j];
-            rec[2] = a.src[2 * N + j];
-            double e = a.eps[j * a.eps_stride];
-            rec[3] = e * e;
-            rec[4] = rec[2] * rec[2];
+            const double e = a.eps[j * a.eps_stride];
+            fill_src64<KIND, NF>(rec, a.src[j], a.src[N + j], a.src[2 * N + j], e * e);
 #pragma unroll
-            for (int k = 0; k < 3 * NF; ++k) rec[5 + k] = k < 3 ? a.f[(int64_t)k * N + j] : a.fi[(int64_t)(k - 3) * N + j];
+            for (int k = 0; k < 3 * NF; ++k)
+                rec[FO + k] = k < 3 ? a.f[(int64_t)k * N + j] : a.fi[(int64_t)(k - 3) * N + j];
         }
         __syncthreads();
 #pragma unroll
@@ -375,12 +403,12 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
 #pragma unroll
             for (int t = 0; t < TPL; ++t) {
                 if (KIND == FL_STOKESLET3D)
-                    pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + 5, acc[t]);
+                    pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + FO, acc[t]);
                 else if (KIND == FL_WALL_STOKESLET3D)
-                    pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t] + r[2], r[3], r[2], r[4], r + 5,
+                    pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t], r[2], r[3], r[4], r[5], r[6], tzz[t], r + FO,
                                   acc[t]);
                 else
-                    pair_brinkman<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], a.alpha, a.inv_alpha, r + 5,
+                    pair_brinkman<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], a.alpha, a.inv_alpha, r + FO,
                                       acc[t]);
             }
         }
@@ -498,6 +526,251 @@ __global__ __launch_bounds__(WG) void nbody_reduce(NbArgs a)
     }
 }
 
+// ---------------------------------------------------------------- symmetric sums
+// When the targets are the sources and every source has the same blob width,
+// the kernels are reciprocal: G(y, x) = G(x, y)^T (the free-space blocks are
+// symmetric and even in d; the wall block maps to its transpose, see
+// wall_block).  Then u_i = sum_j G(x_i, x_j) f_j is formed by evaluating each
+// unordered pair's block once and applying it both ways:
+//   u_i += G(x_i, x_j) f_j     (row part, accumulated in the lane of target i)
+//   u_j += G(x_i, x_j)^T f_i   (column part, reduced over the targets i)
+// The points are cut into tiles of SB = 512 (256 lanes x 2 targets).  Workgroup
+// (K, q) takes target tile K against source tiles L in [K + q c, K + (q+1) c),
+// L >= K; the diagonal tile L = K is summed as rows only (each ordered pair
+// once, self-pair included).  Column sums are reduced in fixed order: within a
+// wave through an LDS transpose (batches of 4 sources), then over the 4 waves,
+// and written per (K, L) to colp; row sums per (K, q) to rowp.  nbody_sym_reduce
+// adds, for each point, its row partials in q order and then its column
+// partials in K order, so results are deterministic.
+constexpr int SB = 2 * WG;
+constexpr int SBATCH = 4;
+// Row pitch of the column-reduction buffer: 68 doubles shifts consecutive rows
+// by 8 LDS banks, so the stride-4 reads of the 48 reducing lanes spread evenly.
+constexpr int SRS = 68;
+
+struct SymArgs {
+    const double* x;   // (3, N)
+    const double* f;   // (3, N)
+    const double* eps; // one blob width, eps[0]
+    double* u_re;      // (3, N)
+    double* u_im;      // (3, N) or null
+    double* rowp;      // (T, Q, 3, SB)
+    double* colp;      // (T (T - 1) / 2, 3, SB)
+    int64_t N;
+    int T, Q, c;
+    double alpha, inv_alpha, w_re, w_im;
+};
+
+__device__ __forceinline__ int64_t sym_pair_index(int64_t K, int64_t L, int64_t T)
+{
+    return K * T - K * (K + 1) / 2 + (L - K - 1);
+}
+
+// Block of (target, source) applied to the source strength f (row) and, when
+// COL, its transpose applied to the target strength g (added to col).
+template <int KIND, bool COL>
+__device__ __forceinline__ void sym_pair(double d0, double d1, double d2, double z, double zz, const double* r,
+                                         const double* g, double alpha, double inv_alpha, double* acc, double* col)
+{
+    constexpr int FO = Src64<KIND, 1>::FOFF;
+    const double f0 = r[FO], f1 = r[FO + 1], f2 = r[FO + 2];
+    if (KIND == FL_WALL_STOKESLET3D) {
+        const WallBlock B = wall_block(d0, d1, z, r[2], r[3], r[4], r[5], r[6], zz);
+        const double w = fma(d0, f0, d1 * f1);
+        const double v = fma(B.b, w, B.g * f2);
+        acc[0] = fma(B.C, f0, fma(d0, v, acc[0]));
+        acc[1] = fma(B.C, f1, fma(d1, v, acc[1]));
+        acc[2] = fma(B.t, w, fma(B.Z, f2, acc[2]));
+        if (COL) {
+            const double wg = fma(d0, g[0], d1 * g[1]);
+            const double vg = fma(B.b, wg, B.t * g[2]);
+            col[0] = fma(B.C, g[0], fma(d0, vg, col[0]));
+            col[1] = fma(B.C, g[1], fma(d1, vg, col[1]));
+            col[2] = fma(B.g, wg, fma(B.Z, g[2], col[2]));
+        }
+    } else {
+        // H1 I + H2 d d^T (symmetric)
+        double H1, H2;
+        const double e2 = r[3];
+        const double Q = fma(d0, d0, fma(d1, d1, fma(d2, d2, e2)));
+        const double rs = rsqrt_pos(Q);
+        const double s3 = rs * rs * rs;
+        if (KIND == FL_STOKESLET3D) {
+            H2 = s3;
+            H1 = (Q + e2) * s3;
+        } else {
+            const double zb = alpha * (Q * rs);
+            double b1, b2;
+            if (zb < 0.5) {
+                brinkman_series(zb, b1, b2);
+            } else {
+                const double E = exp(-zb);
+                const double iz = rs * inv_alpha;
+                const double iz2 = iz * iz;
+                b1 = E * (1.0 + iz + iz2) - iz2;
+                b2 = 3.0 * iz2 - E * (1.0 + 3.0 * iz + 3.0 * iz2);
+            }
+            H2 = b2 * s3;
+            H1 = fma(b1, rs, e2 * H2);
+        }
+        const double t = H2 * fma(d0, f0, fma(d1, f1, d2 * f2));
+        acc[0] = fma(H1, f0, fma(t, d0, acc[0]));
+        acc[1] = fma(H1, f1, fma(t, d1, acc[1]));
+        acc[2] = fma(H1, f2, fma(t, d2, acc[2]));
+        if (COL) {
+            const double tg = H2 * fma(d0, g[0], fma(d1, g[1], d2 * g[2]));
+            col[0] = fma(H1, g[0], fma(tg, d0, col[0]));
+            col[1] = fma(H1, g[1], fma(tg, d1, col[1]));
+            col[2] = fma(H1, g[2], fma(tg, d2, col[2]));
+        }
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
+{
+    constexpr int W = Src64<KIND, 1>::W;
+    constexpr int FO = Src64<KIND, 1>::FOFF;
+    __shared__ __attribute__((aligned(16))) double lds[WG * W];          // 256 sources at a time
+    __shared__ double red[4][SBATCH * 3 * SRS];  // per wave: [source][comp][lane], rows padded to SRS
+    __shared__ double colw[4][64 * 3];                                   // per wave: [source][comp]
+    const int K = blockIdx.x, q = blockIdx.y;
+    const int L0 = K + q * a.c;
+    if (L0 >= a.T) return;
+    const int L1 = min(a.T, L0 + a.c);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t N = a.N;
+    const double e2 = a.eps[0] * a.eps[0];
+
+    double tx[2], ty[2], tz[2], tzz[2], g[2][3], acc[2][3];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int64_t i = (int64_t)K * SB + t * WG + tid;
+        const int64_t ic = i < N ? i : N - 1;
+        tx[t] = a.x[ic];
+        ty[t] = a.x[N + ic];
+        tz[t] = a.x[2 * N + ic];
+        tzz[t] = tz[t] * tz[t];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            g[t][c] = i < N ? a.f[c * N + i] : 0.0;
+            acc[t][c] = 0.0;
+        }
+    }
+
+    for (int L = L0; L < L1; ++L) {
+        const bool diag = L == K;
+        for (int half = 0; half < 2; ++half) {
+            const int64_t jbase = (int64_t)L * SB + half * WG;
+            {
+                const int64_t j = jbase + tid;
+                const int64_t jc = j < N ? j : N - 1;
+                double rec[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) rec[k] = 0.0;
+                fill_src64<KIND, 1>(rec, a.x[jc], a.x[N + jc], a.x[2 * N + jc], e2);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) rec[FO + k] = j < N ? a.f[(int64_t)k * N + j] : 0.0;
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < W; k += 2)
+                    *reinterpret_cast<double2*>(&lds[tid * W + k]) = make_double2(rec[k], rec[k + 1]);
+                __syncthreads();
+            }
+            if (diag) {
+                for (int s = 0; s < WG; ++s) {
+                    double r[W];
+#pragma unroll
+                    for (int k = 0; k < W; k += 2) {
+                        const double2 v = *reinterpret_cast<const double2*>(&lds[s * W + k]);
+                        r[k] = v.x;
+                        r[k + 1] = v.y;
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        sym_pair<KIND, false>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], tzz[t], r, g[t],
+                                              a.alpha, a.inv_alpha, acc[t], nullptr);
+                }
+                continue;
+            }
+            for (int sb = 0; sb < WG / 64; ++sb) {
+                for (int s4 = 0; s4 < 64 / SBATCH; ++s4) {
+#pragma unroll 1
+                    for (int b = 0; b < SBATCH; ++b) {
+                        const int s = sb * 64 + s4 * SBATCH + b;
+                        double r[W];
+#pragma unroll
+                        for (int k = 0; k < W; k += 2) {
+                            const double2 v = *reinterpret_cast<const double2*>(&lds[s * W + k]);
+                            r[k] = v.x;
+                            r[k + 1] = v.y;
+                        }
+                        double col[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+                            sym_pair<KIND, true>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], tzz[t], r, g[t],
+                                                 a.alpha, a.inv_alpha, acc[t], col);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) red[wave][(b * 3 + c) * SRS + lane] = col[c];
+                    }
+                    // the wave's own LDS writes are visible to its later reads (in-order LDS)
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    // 12 (source, comp) sums of 64 values: 4 lanes x 16 values each
+                    // (lane part q takes entries q, q + 4, ..., q + 60)
+                    const int p = min(lane >> 2, SBATCH * 3 - 1);
+                    const double* src = &red[wave][p * SRS + (lane & 3)];
+                    double v = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) v += src[4 * k];
+                    v += __shfl_xor(v, 1);
+                    v += __shfl_xor(v, 2);
+                    if ((lane & 3) == 0 && lane < SBATCH * 3 * 4)
+                        colw[wave][(s4 * SBATCH + p / 3) * 3 + p % 3] = v;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                __syncthreads();
+                if (tid < 192) {
+                    const int c = tid >> 6, s = tid & 63;
+                    const double v = ((colw[0][s * 3 + c] + colw[1][s * 3 + c]) + colw[2][s * 3 + c]) + colw[3][s * 3 + c];
+                    a.colp[(sym_pair_index(K, L, a.T) * 3 + c) * SB + half * WG + sb * 64 + s] = v;
+                }
+                __syncthreads();
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            a.rowp[(((int64_t)K * a.Q + q) * 3 + c) * SB + t * WG + tid] = acc[t][c];
+}
+
+// u_i = w * (row partials of tile K in q order + column partials (K', K) in K' order).
+__global__ __launch_bounds__(WG) void nbody_sym_reduce(SymArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
+    const int64_t N = a.N;
+    if (i >= N) return;
+    const int64_t K = i / SB, t = i % SB;
+    const int nq = (int)((a.T - K + a.c - 1) / a.c);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < nq; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += a.rowp[((K * a.Q + q) * 3 + c) * SB + t];
+    for (int64_t Kp = 0; Kp < K; ++Kp)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += a.colp[(sym_pair_index(Kp, K, a.T) * 3 + c) * SB + t];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.u_re[c * N + i] = a.w_re * s[c];
+        if (a.u_im) a.u_im[c * N + i] = a.w_im * s[c];
+    }
+}
+
 int env_int(const char* name, int dflt)
 {
     const char* v = std::getenv(name);
@@ -524,6 +797,35 @@ int pick_split(int64_t M, int64_t N, int tpl)
     s = std::min<int64_t>(s, smax);
     s = std::min<int64_t>(s, 256);
     return (int)std::max<int64_t>(1, s);
+}
+
+// Symmetric plan: T tiles of SB points, c tiles per workgroup along a row,
+// Q = ceil(T / c) chunks (grid.y).  Used only when there are enough tile pairs
+// to give every CU several workgroups (T (T + 1) / 2 >= 4 CUs) and not
+// switched off by FL_EVAL_SYMMETRIC=0 (measurement switch).
+struct SymPlan {
+    int T = 0, c = 0, Q = 0;
+    size_t bytes = 0;
+    bool ok = false;
+};
+
+SymPlan sym_plan(int64_t N)
+{
+    SymPlan p;
+    if (env_int("FL_EVAL_SYMMETRIC", 1) == 0 || N <= 0) return p;
+    const int64_t T = (N + SB - 1) / SB;
+    const int64_t pairs = T * (T + 1) / 2;
+    const int64_t cus = device_cus();
+    if (pairs < 4 * cus || T > 65535) return p;
+    const int64_t c = std::min<int64_t>(16, std::max<int64_t>(1, pairs / (16 * cus)));
+    p.T = (int)T;
+    p.c = (int)c;
+    p.Q = (int)((T + c - 1) / c);
+    const size_t rowb = align_up((size_t)T * p.Q * 3 * SB * sizeof(double), 256);
+    const size_t colb = (size_t)(T * (T - 1) / 2) * 3 * SB * sizeof(double);
+    p.bytes = rowb + colb;
+    p.ok = true;
+    return p;
 }
 
 template <int KIND, int NF, int TPL>
@@ -561,6 +863,13 @@ size_t eval_workspace_size(int64_t M, int64_t N, int nf, int precision)
     return (size_t)s * nf * 3 * (size_t)M * sizeof(double);
 }
 
+size_t eval_sym_workspace_size(int64_t N, int nf, int precision)
+{
+    if (nf != 1 || precision != FL_FP64) return 0;
+    const SymPlan p = sym_plan(N);
+    return p.ok ? p.bytes : 0;
+}
+
 int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt, const double* src,
                 const double* eps, int64_t eps_stride, const double* f, const double* fi, double w_re, double w_im,
                 double* u_re, double* u_im, int precision, void* ws, size_t ws_bytes, hipStream_t stream)
@@ -578,6 +887,36 @@ int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt,
         return FL_OK;
     }
     FL_CHECK(tgt && src && eps && f && u_re, "null pointer argument");
+    if (tgt == src && M == N && eps_stride == 0 && nf == 1 && precision == FL_FP64) {
+        const SymPlan p = sym_plan(N);
+        if (p.ok && ws && ws_bytes >= p.bytes) {
+            SymArgs s;
+            s.x = src;
+            s.f = f;
+            s.eps = eps;
+            s.u_re = u_re;
+            s.u_im = u_im;
+            s.rowp = static_cast<double*>(ws);
+            s.colp = s.rowp + align_up((size_t)p.T * p.Q * 3 * SB * sizeof(double), 256) / sizeof(double);
+            s.N = N;
+            s.T = p.T;
+            s.Q = p.Q;
+            s.c = p.c;
+            s.alpha = alpha;
+            s.inv_alpha = alpha > 0.0 ? 1.0 / alpha : 0.0;
+            s.w_re = w_re;
+            s.w_im = w_im;
+            dim3 grid((unsigned)p.T, (unsigned)p.Q);
+            if (kind == FL_STOKESLET3D) hipLaunchKernelGGL(nbody64_sym<FL_STOKESLET3D>, grid, dim3(WG), 0, stream, s);
+            else if (kind == FL_WALL_STOKESLET3D)
+                hipLaunchKernelGGL(nbody64_sym<FL_WALL_STOKESLET3D>, grid, dim3(WG), 0, stream, s);
+            else hipLaunchKernelGGL(nbody64_sym<FL_BRINKMANLET3D>, grid, dim3(WG), 0, stream, s);
+            FL_LAUNCHED();
+            hipLaunchKernelGGL(nbody_sym_reduce, dim3((unsigned)((N + WG - 1) / WG)), dim3(WG), 0, stream, s);
+            FL_LAUNCHED();
+            return FL_OK;
+        }
+    }
     const int tpl = pick_tpl(M);
     const int nsplit = pick_split(M, N, tpl);
     const size_t need = eval_workspace_size(M, N, nf, precision);

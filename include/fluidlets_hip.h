@@ -91,8 +91,18 @@ int fl_device_cus(void);
  * Sources are summed in index order inside fixed source chunks whose partial
  * sums are combined in chunk order, so results are deterministic and do not
  * depend on other work in flight.
+ *
+ * Symmetric evaluation: when tgt == src (the same pointer), M == N,
+ * eps_stride == 0, the strengths are real and precision is FL_FP64, the
+ * kernels are reciprocal (G(y, x) = G(x, y)^T) and each unordered pair's block
+ * is evaluated once and applied both ways.  fl_eval takes that path when the
+ * workspace holds at least fl_eval_sym_workspace_size(N, 0, FL_FP64) bytes
+ * (0 means the path is not used at this size); the partial sums are again
+ * combined in a fixed order.  This is the matvec of resistance problems
+ * (core/solve.py:71-73 passes the points as both targets and sources).
  */
 size_t fl_eval_workspace_size(int64_t M, int64_t N, int complex_strengths, int precision);
+size_t fl_eval_sym_workspace_size(int64_t N, int complex_strengths, int precision);
 
 int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N,
             const double* tgt, const double* src, const double* eps, int64_t eps_stride,

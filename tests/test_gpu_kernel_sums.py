@@ -176,8 +176,9 @@ def test_full_size_c2_wall_subset_and_invariants():
     u, _ = evaluate_device(K, x, x, eps_t, st, f)
     torch.cuda.synchronize()
     u_host = D.to_host_aos(u)
-    ref = O.evaluate("wall", X[:512], X, F, 0.02)
-    close(u_host[:512], ref)
+    # first, a middle and the last (ragged-free) tile: row partials only, both, column partials only
+    for sl in [slice(0, 512), slice(131000, 131200), slice(n - 300, n)]:
+        close(u_host[sl], O.evaluate("wall", X[sl], X, F, 0.02))
     # linearity: u(2f1 - 3f2) = 2u(f1) - 3u(f2)
     g = D.soa(rng.standard_normal((n, 3)), dev)
     ug, _ = evaluate_device(K, x, x, eps_t, st, g)
@@ -187,3 +188,70 @@ def test_full_size_c2_wall_subset_and_invariants():
     W = np.c_[rng.uniform(-4, 4, (1000, 2)), np.zeros(1000)]
     uw, _ = evaluate_device(K, D.soa(W, dev), x, eps_t, st, f)
     assert uw.abs().max().item() < 1e-11 * u.abs().max().item()
+
+
+@pytest.mark.parametrize("name", ["stokeslet", "wall", "brinkman:4.0", "brinkman:20.0"])
+def test_symmetric_path_matches_general_path_and_oracle(name):
+    """Targets = sources with one blob width takes the reciprocal (each pair
+    once, applied both ways) evaluation; it must agree with the general path
+    (same points passed as a separate copy) and with the oracle on target
+    subsets in the first, a middle and the last, ragged tile.  N = 30,011 is
+    not a multiple of the 512-point tile."""
+    import torch
+    from fluidlets_amd import _device as D
+    from fluidlets_amd._lib import lib
+    from fluidlets_amd.core.matvec import evaluate_device
+    print(f"case: symmetric path {name}", flush=True)
+    n = 30011
+    assert lib().fl_eval_sym_workspace_size(n, 0, 0) > 0
+    rng = np.random.default_rng(11)
+    X = rng.uniform([-2, -2, 0.05], [2, 2, 2], (n, 3))
+    X[7] = X[6]                                  # a coincident pair
+    F = rng.standard_normal((n, 3))
+    dev = D.device()
+    x, f = D.soa(X, dev), D.soa(F, dev)
+    eps_t, st = D.eps_arg(0.03, n, dev)
+    K = KERNELS[name]
+    us, _ = evaluate_device(K, x, x, eps_t, st, f)
+    ug, _ = evaluate_device(K, x, x.clone(), eps_t, st, f)
+    us2, _ = evaluate_device(K, x, x, eps_t, st, f)
+    torch.cuda.synchronize()
+    assert torch.equal(us, us2)                  # deterministic
+    assert (torch.linalg.norm(us - ug) / torch.linalg.norm(ug)).item() < 1e-13
+    uh = D.to_host_aos(us)
+    for sl in [slice(0, 300), slice(15000, 15300), slice(n - 300, n)]:
+        close(uh[sl], O.evaluate(name, X[sl], X, F, 0.03))
+    # complex viscosity through the same path
+    uc = fl.evaluate(K, X, X, F, 0.03, mu=1 + 0.5j)
+    assert np.allclose(uc, uh / (1 + 0.5j), rtol=1e-13, atol=1e-15 * np.abs(uh).max())
+
+
+def test_symmetric_path_in_gmres():
+    """Matrix-free GMRES on 23,552 wall points uses the reciprocal matvec: the
+    solution meets the residual bar and agrees with the general-path solve."""
+    import torch
+    from fluidlets_amd import _device as D
+    from fluidlets_amd.core.gmres import gmres_device
+    from fluidlets_amd.geometry import rod_carpet
+    print("case: symmetric matvec in GMRES", flush=True)
+    X = rod_carpet(n_side=16, per_rod=92)
+    n = len(X)
+    assert n == 23552
+    dev = D.device()
+    x = D.soa(X, dev)
+    eps_t, st = D.eps_arg(0.01, n, dev)
+    U = np.zeros((n, 3))
+    U[:, 0] = -X[:, 2]
+    U_d = D.soa(U, dev)
+    K = fl.WallStokeslet3D()
+    f1, it1, r1 = gmres_device(K, x, eps_t, st, U_d, rtol=1e-9, restart=50, maxiter=60)
+    os.environ["FL_EVAL_SYMMETRIC"] = "0"
+    try:
+        f2, it2, r2 = gmres_device(K, x, eps_t, st, U_d, rtol=1e-9, restart=50, maxiter=60)
+    finally:
+        del os.environ["FL_EVAL_SYMMETRIC"]
+    torch.cuda.synchronize()
+    assert r1 <= 1e-9 and r2 <= 1e-9
+    assert (torch.linalg.norm(f1 - f2) / torch.linalg.norm(f2)).item() < 1e-7
+    u = fl.evaluate(K, X, X, D.to_host_aos(f1), 0.01)
+    assert np.linalg.norm(u - U) / np.linalg.norm(U) < 2e-9
