@@ -33,9 +33,15 @@ def hipcc() -> str:
     return exe
 
 
+# Per-source extra flags.  lu.hip: the diagonal-block kernel keeps its factor
+# and its triangular inverses in registers, which needs its constant-trip loops
+# fully unrolled (register arrays with static indices, no scratch memory).
+EXTRA = {"lu.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [hipcc(), "-x", "hip", f"--offload-arch={ARCH}", *COMMON, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -2,20 +2,19 @@
 // iterative refinement, for the dense resistance matrices.
 //
 // Panel step k (width nb <= NB = 128), in row-major storage:
-//   1. the nb x nb diagonal block is factored by GEPP in LDS by one workgroup,
-//      its pivot search restricted to the block's rows;
-//   2. inv(L11) and inv(U11) are formed (one workgroup each) and kept in the
-//      factor's aux buffer for the triangular solves;
-//   3. L21 = A21 inv(U11) on MFMA, in place; the GEMM epilogue records
+//   1. one workgroup factors the nb x nb diagonal block by GEPP in registers,
+//      its pivot search restricted to the block's rows, and forms inv(L11)
+//      and inv(U11), kept in the factor's aux buffer for the triangular solves;
+//   2. L21 = A21 inv(U11) on MFMA, in place; the GEMM epilogue records
 //      max |L21|.  If it is <= 1 every pivot chosen in step 1 is also the
 //      pivot partial pivoting over the whole column would choose (all
 //      multipliers below the block are bounded by the block's pivot), so the
 //      factorization is exactly GEPP.  Otherwise the panel is restored from a
 //      copy and refactored column by column with the pivot search over all
 //      rows below the diagonal (the general path);
-//   4. the panel's row interchanges are applied left and right of it;
-//   5. U12 = inv(L11) A12 on MFMA, in place;
-//   6. A22 -= L21 U12 on MFMA (FP32 or FP64 accumulation).
+//   3. the panel's row interchanges are applied left and right of it;
+//   4. U12 = inv(L11) A12 on MFMA, in place;
+//   5. A22 -= L21 U12 on MFMA (FP32 or FP64 accumulation).
 // The LAPACK pivot vector (ipiv[r] = row swapped with r) is also folded into a
 // row permutation stored with the factor, so solves gather the right-hand
 // side in one pass.
@@ -41,113 +40,9 @@ namespace {
 constexpr int NB = 128;   // diagonal block (LDS-resident GEPP, inverted for the solves)
 constexpr int NBO = 256;  // outer panel width: the trailing update runs with K = NBO
 
-// ---------------------------------------------------------------- diagonal block GEPP
-// 1024 lanes (16 waves).  Step c: the pivot (first row of largest |S[r][c]|,
-// r >= c) is read from 16 per-wave candidates written during step c-1, rows c
-// and p are exchanged, then wave w updates rows r = c+1+w, c+17+w, ... with
-// one batched round of LDS loads: l = S[r][c] / S[c][c] (LAPACK getf2 scales
-// by the reciprocal), S[r][j] -= l S[c][j] for j > c, and lane 0 (column c+1)
-// records its rows' candidate for step c+1.  Two barriers per column.
-constexpr int DG_WAVES = 16;
-constexpr int DG_ROWS = (NB + DG_WAVES - 1) / DG_WAVES;  // rows per wave per step
-
-template <typename T>
-__global__ __launch_bounds__(1024) void diag_getrf_kernel(T* A, int64_t lda, int nb, int64_t k, int32_t* ipiv,
-                                                          int32_t* info)
-{
-    __shared__ T S[NB * (NB + 1)];
-    __shared__ T cand_v[2][DG_WAVES];
-    __shared__ int cand_i[2][DG_WAVES];
-    constexpr int LS = NB + 1;
-    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    for (int idx = tid; idx < nb * nb; idx += 1024) S[(idx / nb) * LS + idx % nb] = A[(idx / nb) * lda + idx % nb];
-    __syncthreads();
-    // candidates for column 0
-    if (lane == 0) {
-        T bv = T(-1);
-        int bi = 0;
-        for (int r = wave; r < nb; r += DG_WAVES) {
-            T v = (T)fabs((double)S[r * LS]);
-            if (v > bv) {
-                bv = v;
-                bi = r;
-            }
-        }
-        cand_v[0][wave] = bv;
-        cand_i[0][wave] = bi;
-    }
-    __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        const int buf = c & 1;
-        if (wave < 2) {
-            // lanes 0..15 hold the 16 wave candidates; first index of the max
-            T bv = lane < DG_WAVES ? cand_v[buf][lane] : T(-1);
-            int p = lane < DG_WAVES ? cand_i[buf][lane] : 1 << 30;
-#pragma unroll
-            for (int off = 8; off > 0; off /= 2) {
-                const T ov = __shfl_xor(bv, off);
-                const int oi = __shfl_xor(p, off);
-                if (ov > bv || (ov == bv && oi < p)) {
-                    bv = ov;
-                    p = oi;
-                }
-            }
-            p = __shfl(p, 0);
-            bv = __shfl(bv, 0);
-            if (tid == 0) {
-                ipiv[k + c] = (int32_t)(k + p);
-                if (bv == T(0)) atomicCAS(info, 0, (int32_t)(k + c + 1));
-            }
-            if (p != c && tid < nb) {
-                T t = S[c * LS + tid];
-                S[c * LS + tid] = S[p * LS + tid];
-                S[p * LS + tid] = t;
-            }
-        }
-        __syncthreads();
-        const T pv = S[c * LS + c];
-        const T rp = pv != T(0) ? T(1) / pv : T(1);
-        const int j0 = c + 1 + lane, j1 = j0 + 64;
-        const T u0 = j0 < nb ? S[c * LS + j0] : T(0);
-        const T u1 = j1 < nb ? S[c * LS + j1] : T(0);
-        T l[DG_ROWS], x0[DG_ROWS], x1[DG_ROWS];
-#pragma unroll
-        for (int i = 0; i < DG_ROWS; ++i) {
-            const int r = c + 1 + wave + DG_WAVES * i;
-            const bool ok = r < nb;
-            l[i] = ok ? S[r * LS + c] * rp : T(0);
-            x0[i] = (ok && j0 < nb) ? S[r * LS + j0] : T(0);
-            x1[i] = (ok && j1 < nb) ? S[r * LS + j1] : T(0);
-        }
-        T nb_v = T(-1);
-        int nb_i = 0;
-#pragma unroll
-        for (int i = 0; i < DG_ROWS; ++i) {
-            const int r = c + 1 + wave + DG_WAVES * i;
-            if (r < nb) {
-                x0[i] -= l[i] * u0;
-                x1[i] -= l[i] * u1;
-                if (lane == 0) S[r * LS + c] = l[i];
-                if (j0 < nb) S[r * LS + j0] = x0[i];
-                if (j1 < nb) S[r * LS + j1] = x1[i];
-                const T a = (T)fabs((double)x0[i]);
-                if (a > nb_v) {
-                    nb_v = a;
-                    nb_i = r;
-                }
-            }
-        }
-        if (lane == 0) {
-            cand_v[buf ^ 1][wave] = nb_v;
-            cand_i[buf ^ 1][wave] = nb_i;
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < nb * nb; idx += 1024) A[(idx / nb) * lda + idx % nb] = S[(idx / nb) * LS + idx % nb];
-}
-
-// ---------------------------------------------------------------- triangular inverses
-// grid (2, NB / 16): blockIdx.x 0 forms X = inv(L11) (unit lower), 1 forms
+// ---------------------------------------------------------------- triangular inverses (general path)
+// Used after a panel has been refactored column by column with full-column
+// pivoting.  grid (2, NB / 16): blockIdx.x 0 forms X = inv(L11) (unit lower), 1 forms
 // X = inv(U11) (upper); blockIdx.y selects 16 columns.  The factored block is
 // staged in LDS; column j of X is computed by 8 consecutive lanes of one wave
 // (each summing an eighth of the dot product, combined by shuffles), so rows
@@ -195,6 +90,308 @@ __global__ __launch_bounds__(128) void trtri_kernel(const T* A, int64_t lda, int
     for (int idx = tid; idx < nb * TR_COLS; idx += 128) {
         const int i = idx / TR_COLS, c = blockIdx.y * TR_COLS + idx % TR_COLS;
         if (c < nb) out[i * NB + c] = X[idx];
+    }
+}
+
+// ---------------------------------------------------------------- register-resident diagonal block
+// One workgroup of 256 lanes factors the nb x nb diagonal block (nb <= NB) by
+// GEPP and forms inv(L11) and inv(U11) for the panel's products and the
+// triangular solves.
+//
+// Factorization: lane (slot s = tid >> 1, half h = tid & 1) holds the slot's
+// entries in columns 2k + h, k < 64, in registers.  Rows are not moved: each
+// slot carries its position pos in LAPACK's row order after the interchanges
+// so far.  Step c takes the active slot with the largest |a(c)| (the smallest
+// pos on ties, as isamax does), records ipiv[c] = its pos, exchanges its
+// position with the slot at position c, and eliminates column c from the
+// other active slots (multiplier a(c) / pivot, as getf2 scales by the
+// reciprocal) with the pivot row read from LDS.  Each wave publishes its best
+// candidate and that candidate's row, double-buffered by column parity, so a
+// column costs one barrier.  Slot s is finally written to row pos(s), which
+// is the row-interchanged LU factor.
+//
+// Inverses: each of the four 64 x 64 diagonal triangles is inverted by
+// substitution (one wave per triangle, one lane per column); the off-diagonal
+// blocks follow from inv(L)21 = -inv(L22) L21 inv(L11) and
+// inv(U)12 = -inv(U11) U12 inv(U22).  Outputs are NB x NB, ld NB, zero outside
+// the triangle (identity beyond nb).
+// Lane exchange through DPP (a VALU modifier, no LDS round trip).  CTRL:
+// quad_perm [1,0,3,2] = 0xB1 (swap neighbours), quad_perm [2,3,0,1] = 0x4E,
+// row_ror:4 = 0x124, row_ror:8 = 0x128 (rotations inside 16-lane rows).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float read_lane(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+__device__ __forceinline__ double read_lane(double x, int l)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                            __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
+// Largest v over the wave (every lane receives it).
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+    v = fmax(v, dpp_mov<0xB1>(v));
+    v = fmax(v, dpp_mov<0x4E>(v));
+    v = fmax(v, dpp_mov<0x124>(v));
+    v = fmax(v, dpp_mov<0x128>(v));
+    return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
+}
+
+// 16-byte vector of T for LDS and global moves.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+
+// Column j of the inverse of the 64 x 64 unit lower triangle of F (pitch lf),
+// by forward substitution; written to X[i * lx + j].
+template <typename T>
+__device__ __forceinline__ void tri_inverse_lower(const T* F, int lf, int j, T* X, int lx)
+{
+    T x[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+        T s0 = T(0), s1 = T(0);
+#pragma unroll
+        for (int m = 0; m + 1 < 64; m += 2) {
+            if (m + 1 < i) {
+                s0 = fma(F[i * lf + m], x[m], s0);
+                s1 = fma(F[i * lf + m + 1], x[m + 1], s1);
+            }
+        }
+        if (i & 1) s0 = fma(F[i * lf + i - 1], x[i - 1], s0);
+        x[i] = i < j ? T(0) : (i == j ? T(1) : -(s0 + s1));
+    }
+#pragma unroll
+    for (int i = 0; i < 64; ++i) X[i * lx + j] = x[i];
+}
+
+// Column j of the inverse of the 64 x 64 upper triangle of F, by backward
+// substitution; x[q] holds row 63 - q.
+template <typename T>
+__device__ __forceinline__ void tri_inverse_upper(const T* F, int lf, int j, T* X, int lx)
+{
+    T x[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) {
+        const int i = 63 - q;
+        T s0 = T(0), s1 = T(0);
+#pragma unroll
+        for (int m = 0; m + 1 < 64; m += 2) {
+            if (m + 1 < q) {
+                s0 = fma(F[i * lf + 63 - m], x[m], s0);
+                s1 = fma(F[i * lf + 62 - m], x[m + 1], s1);
+            }
+        }
+        if (q & 1) s0 = fma(F[i * lf + 64 - q], x[q - 1], s0);
+        const T rd = T(1) / F[i * lf + i];
+        x[q] = i > j ? T(0) : (i == j ? rd : -(s0 + s1) * rd);
+    }
+#pragma unroll
+    for (int q = 0; q < 64; ++q) X[(63 - q) * lx + j] = x[q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_lu_kernel(T* A, int64_t lda, int nb, int64_t k0, int32_t* ipiv,
+                                                      int32_t* info, T* invL, T* invU)
+{
+    constexpr int H = NB / 2;
+    constexpr int LS = NB + 1;
+    constexpr int GC = 8;  // columns per unrolled group; the register window advances by GC / 2
+    constexpr int V = Vec16<T>::n;
+    __shared__ T S[NB * LS];
+    __shared__ __attribute__((aligned(16))) T prow[2][4][NB];  // per-wave candidate rows, by column parity
+    __shared__ T cand_v[2][4];
+    __shared__ int cand_p[2][4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int slot = tid >> 1, h = tid & 1;
+
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int r = idx / NB, c = idx % NB;
+        S[r * LS + c] = (r < nb && c < nb) ? A[r * lda + c] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    // a[k] holds column 2 (k + kb) + h of this slot, kb = GC / 2 * group (a window
+    // that drops the finished columns into S row `slot` as the factorization advances)
+    T a[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) a[k] = S[slot * LS + 2 * k + h];
+    __syncthreads();
+
+    int pos = slot;
+    bool done = slot >= nb;
+    for (int g = 0; g < NB / GC; ++g) {
+#pragma unroll
+        for (int cl = 0; cl < GC; ++cl) {
+            const int c = g * GC + cl;
+            if (c < nb) {  // uniform
+                const int kc = cl >> 1, hc = cl & 1, buf = cl & 1;
+                const bool cand = h == hc && !done;
+                const T v = cand ? (a[kc] < T(0) ? -a[kc] : a[kc]) : T(-1);
+                // wave candidate: largest |a(c)|, smallest position among equals
+                const T wv = wave_max(v);
+                unsigned long long tied = __ballot(cand && v == wv);
+                int wp = 0x7fffffff;
+                while (tied) {  // uniform; one pass unless values tie exactly
+                    const int l = __ffsll((long long)tied) - 1;
+                    wp = min(wp, __builtin_amdgcn_readlane(pos, l));
+                    tied &= tied - 1;
+                }
+                if (!done && pos == wp) {
+                    auto* dst = reinterpret_cast<typename Vec16<T>::type*>(&prow[buf][wave][h * H]);
+#pragma unroll
+                    for (int k = 0; k < H; k += V) {
+                        typename Vec16<T>::type u;
+                        T* up = reinterpret_cast<T*>(&u);
+#pragma unroll
+                        for (int q = 0; q < V; ++q) up[q] = a[k + q];
+                        dst[k / V] = u;
+                    }
+                }
+                if (lane == 0) {
+                    cand_v[buf][wave] = wv;
+                    cand_p[buf][wave] = wp;
+                }
+                __syncthreads();
+                T bv = cand_v[buf][0];
+                int bp = cand_p[buf][0], bw = 0;
+#pragma unroll
+                for (int w = 1; w < 4; ++w) {
+                    const T ov = cand_v[buf][w];
+                    const int op = cand_p[buf][w];
+                    if (ov > bv || (ov == bv && op < bp)) {
+                        bv = ov;
+                        bp = op;
+                        bw = w;
+                    }
+                }
+                const T pv = prow[buf][bw][hc * H + kc];
+                if (tid == 0) {
+                    ipiv[k0 + c] = (int32_t)(k0 + bp);
+                    if (pv == T(0)) atomicCAS(info, 0, (int32_t)(k0 + c + 1));
+                }
+                const T rp = pv != T(0) ? T(1) / pv : T(1);
+                const bool piv = !done && pos == bp;
+                if (pos == c) pos = bp;
+                if (piv) pos = c;
+                done = done || piv;
+                const T lm = a[kc] * rp;
+                const T lo = dpp_mov<0xB1>(lm);
+                const T l = done ? T(0) : (h == hc ? lm : lo);
+                const auto* ph = reinterpret_cast<const typename Vec16<T>::type*>(&prow[buf][bw][h * H]);
+                const T akc = a[kc];
+#pragma unroll
+                for (int k0v = 0; k0v < H; k0v += V) {
+                    if (k0v + V - 1 >= kc) {
+                        const typename Vec16<T>::type u = ph[k0v / V];
+                        const T* up = reinterpret_cast<const T*>(&u);
+#pragma unroll
+                        for (int q = 0; q < V; ++q)
+                            if (k0v + q > kc) a[k0v + q] = fma(-l, up[q], a[k0v + q]);
+                        if (kc >= k0v && kc < k0v + V) {
+                            // column 2 kc + h: the multiplier (h == hc) or an update if right of c
+                            const T upd = fma(-l, up[kc - k0v], akc);
+                            a[kc] = h == hc ? (done ? akc : lm) : (2 * kc + h > cl ? upd : akc);
+                        }
+                    }
+                }
+            }
+        }
+        // columns 2 (kb + k) + h, k < GC / 2, are final: store them and advance the window
+#pragma unroll
+        for (int k = 0; k < GC / 2; ++k) S[slot * LS + 2 * (g * (GC / 2) + k) + h] = a[k];
+#pragma unroll
+        for (int k = 0; k < H - GC / 2; ++k) a[k] = a[k + GC / 2];
+#pragma unroll
+        for (int k = H - GC / 2; k < H; ++k) a[k] = T(0);
+    }
+    // rows by slot -> rows by position
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < H; ++k) a[k] = S[slot * LS + 2 * k + h];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < H; ++k) S[pos * LS + 2 * k + h] = a[k];
+    __syncthreads();
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        A[r * lda + c] = S[r * LS + c];
+    }
+
+    // inverses of the four 64 x 64 diagonal triangles (waves 0, 1: L11, L22; waves 2, 3: U11, U22)
+    if (wave < 2) tri_inverse_lower<T>(S + (wave * 64) * (LS + 1), LS, lane, invL + (wave * 64) * (NB + 1), NB);
+    else tri_inverse_upper<T>(S + ((wave - 2) * 64) * (LS + 1), LS, lane, invU + ((wave - 2) * 64) * (NB + 1), NB);
+    if (wave == 0)
+#pragma unroll
+        for (int i = 0; i < 64; ++i) invL[i * NB + 64 + lane] = T(0);
+    if (wave == 3)
+#pragma unroll
+        for (int i = 0; i < 64; ++i) invU[(64 + i) * NB + lane] = T(0);
+    __syncthreads();  // the diagonal inverse blocks are read back by the other waves
+    {
+        const int i = tid >> 2, jb = (tid & 3) * 16;
+        T tl[16], tu[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
+        for (int m = 0; m < H; ++m) {
+            const T l21 = S[(H + i) * LS + m];
+            const T u12 = S[i * LS + H + m];
+            const T* xl1 = invL + m * NB + jb;
+            const T* xu2 = invU + (H + m) * NB + H + jb;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                tl[q] = fma(l21, xl1[q], tl[q]);
+                tu[q] = fma(u12, xu2[q], tu[q]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            S[(H + i) * LS + jb + q] = tl[q];  // L21 inv(L11) in place of L21
+            S[i * LS + H + jb + q] = tu[q];    // U12 inv(U22) in place of U12
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
+        for (int m = 0; m < H; ++m) {
+            const T xl2 = invL[(H + i) * NB + H + m];
+            const T xu1 = invU[i * NB + m];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                tl[q] = fma(xl2, S[(H + m) * LS + jb + q], tl[q]);
+                tu[q] = fma(xu1, S[m * LS + H + jb + q], tu[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            invL[(H + i) * NB + jb + q] = -tl[q];
+            invU[i * NB + H + jb + q] = -tu[q];
+        }
     }
 }
 
@@ -291,87 +488,184 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
     }
 }
 
-// ---------------------------------------------------------------- triangular solves
-// One block step of the forward (LOWER) or backward substitution on nrhs <= 8
-// right-hand sides stored row-major with ld = nrhs, in two launches:
-//   trsv_diag:   y = inv(T_bb) src[b-rows]  -> dst[b-rows]      (one workgroup)
-//   trsv_update: src[r] -= T[r, b-cols] y   for r below block b (LOWER) or
-//                above it (upper), 8 lanes per row, each lane 16 consecutive
-//                entries of the row (coalesced), reduced across the 8 lanes.
-template <typename T>
-__global__ __launch_bounds__(256) void trsv_diag_kernel(const T* inv, int64_t b0, int nbk, const T* src, T* dst,
-                                                        int nrhs)
+// ---------------------------------------------------------------- chained block substitution
+// Forward (LOWER: L z = y, unit diagonal) or backward (U x = z) substitution
+// by NB-row blocks in one launch, for NR-padded right-hand sides stored
+// row-major with ld nrhs <= NR.  Block row p needs every block solved before
+// it in the sweep:
+//   acc = b_p - sum_j T_pj x_j     (j < p forward, j > p backward),
+//   x_p = inv(T_pp) acc            (the diagonal block's inverse kept in aux).
+// Workgroups take tickets in dispatch order (ticket t solves block t forward,
+// nblk - 1 - t backward) and wait on a block's flag before reading it, so a
+// workgroup waits only on workgroups dispatched before it.  x_p is published
+// by a release store of flag[p] after every thread's stores are fenced.  A
+// wait is bounded (CH_SPIN polls): a workgroup that gives up sets sync[1] and
+// writes NaN for its block, then publishes its flag, so every workgroup ends.
+// sync = [ticket, error, flag[0 .. nblk)], zeroed before the launch.
+constexpr int CH_SPIN = 1 << 22;
+
+__global__ void zero_u32_kernel(unsigned* p, int64_t n)
 {
-    __shared__ T xs[NB * 8];
-    __shared__ T part[2][NB * 8];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < nbk * nrhs; idx += 256) xs[idx] = src[b0 * nrhs + idx];
-    __syncthreads();
-    const int i = tid % NB, h = tid / NB;  // row i of inv, half h of the k range
-    T acc[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = T(0);
-    if (i < nbk) {
-        const T* row = inv + (int64_t)i * NB;
-        const int k0 = h * (NB / 2), k1 = min(nbk, k0 + NB / 2);
-#pragma unroll 8
-        for (int kk = k0; kk < k1; ++kk) {
-            const T a = row[kk];
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (q < nrhs) acc[q] += a * xs[kk * nrhs + q];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-        if (q < nrhs) part[h][i * nrhs + q] = acc[q];
-    __syncthreads();
-    for (int idx = tid; idx < nbk * nrhs; idx += 256) dst[b0 * nrhs + idx] = part[0][idx] + part[1][idx];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_update_kernel(const T* LU, int64_t lda, int64_t n, int64_t b0, int nbk,
-                                                          const T* ysrc, T* src, int nrhs)
+template <typename T, bool LOWER, int NR>
+__global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t lda, int64_t n, const T* invs,
+                                                         const T* b, T* x, int nrhs, unsigned* sync)
 {
-    __shared__ T ys[NB * 8];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < nbk * nrhs; idx += 256) ys[idx] = ysrc[b0 * nrhs + idx];
+    constexpr int H = NB / 2;
+    constexpr int V = Vec16<T>::n;
+    constexpr int TP = NB + (sizeof(T) == 4 ? 4 : 2);  // LDS row pitch: rows 16-byte aligned, 4 banks apart
+    using VT = typename Vec16<T>::type;
+    __shared__ __attribute__((aligned(16))) T Ts[NB * TP];
+    __shared__ __attribute__((aligned(16))) T xs[NB * NR];
+    __shared__ int tk;
+    unsigned* flags = sync + 2;
+    const int tid = threadIdx.x, r = tid >> 1, hf = tid & 1;
+    const int64_t nblk = (n + NB - 1) / NB;
+    if (tid == 0) tk = (int)atomicAdd(&sync[0], 1u);
     __syncthreads();
-    const int64_t lo = LOWER ? b0 + nbk : 0;
-    const int64_t hi = LOWER ? n : b0;
-    const int sub = tid % 8;                       // 16-entry segment of the row
-    const int64_t r = lo + (int64_t)blockIdx.x * 32 + tid / 8;
-    T acc[8];
+    const int64_t p = LOWER ? tk : nblk - 1 - tk;
+    const int64_t r0 = p * NB;
+    const int nbp = (int)min((int64_t)NB, n - r0);
+    bool gave_up = false;
+    T part[NR];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = T(0);
-    if (r < hi) {
-        const T* row = LU + r * lda + b0;
-        const int k0 = sub * 16;
+    for (int q = 0; q < NR; ++q) part[q] = T(0);
+    const int64_t nsteps = LOWER ? p : nblk - 1 - p;
+    // this thread's half row of T_pj: all H entries loaded at once (issued a
+    // step ahead, so the loads overlap the previous block's products)
+    T trow[H];
+    auto load_row = [&](int64_t j) {
+        const int64_t c0 = j * NB;
+        const int nbj = (int)min((int64_t)NB, n - c0);
+        const T* src = LU + (r0 + min(r, nbp - 1)) * lda + c0 + hf * H;
+        const bool full = r < nbp && nbj == NB && ((uintptr_t)src % 16) == 0;
+        if (full) {
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            if (k0 + kk < nbk) {
-                const T a = row[k0 + kk];
+            for (int k = 0; k < H; k += V) {
+                const VT u = *reinterpret_cast<const VT*>(src + k);
+                const T* up = reinterpret_cast<const T*>(&u);
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (q < nrhs) acc[q] += a * ys[(k0 + kk) * nrhs + q];
+                for (int q = 0; q < V; ++q) trow[k + q] = up[q];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < H; ++k) trow[k] = (r < nbp && hf * H + k < nbj) ? src[k] : T(0);
+        }
+    };
+    // the diagonal block's inverse row (always NB x NB, zero or identity
+    // beyond the block; rows of acc beyond it are zero)
+    const T* inv = invs + p * 2 * NB * NB + (LOWER ? 0 : NB * NB) + (int64_t)r * NB + hf * H;
+    auto load_inv = [&]() {
+#pragma unroll
+        for (int k = 0; k < H; k += V) {
+            const VT u = *reinterpret_cast<const VT*>(inv + k);
+            const T* up = reinterpret_cast<const T*>(&u);
+#pragma unroll
+            for (int q = 0; q < V; ++q) trow[k + q] = up[q];
+        }
+    };
+    if (nsteps > 0) load_row(LOWER ? 0 : nblk - 1);
+    else load_inv();
+#pragma unroll 1
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const int64_t j = LOWER ? s : nblk - 1 - s;
+        const int64_t c0 = j * NB;
+        const int nbj = (int)min((int64_t)NB, n - c0);
+        if (tid == 0) {
+            int spins = 0;
+            while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                if (++spins > CH_SPIN) {
+                    atomicOr(&sync[1], 1u);
+                    gave_up = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        {
+            VT* dst = reinterpret_cast<VT*>(&Ts[r * TP + hf * H]);
+#pragma unroll
+            for (int k = 0; k < H; k += V) {
+                VT u;
+                T* up = reinterpret_cast<T*>(&u);
+#pragma unroll
+                for (int q = 0; q < V; ++q) up[q] = trow[k + q];
+                dst[k / V] = u;
+            }
+        }
+        for (int idx = tid; idx < NB * NR; idx += 256) {
+            const int c = idx / NR, q = idx % NR;
+            xs[idx] = (c < nbj && q < nrhs) ? x[(c0 + c) * nrhs + q] : T(0);
+        }
+        __syncthreads();
+        if (s + 1 < nsteps) load_row(LOWER ? s + 1 : nblk - 2 - s);
+        else load_inv();
+        const T* tr = &Ts[r * TP + hf * H];
+#pragma unroll 1
+        for (int k0 = 0; k0 < H; k0 += V) {
+            const VT u = *reinterpret_cast<const VT*>(tr + k0);
+            const T* up = reinterpret_cast<const T*>(&u);
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const T* xk = &xs[(hf * H + k0 + k) * NR];
+#pragma unroll
+                for (int q = 0; q < NR; ++q) part[q] = fma(up[k], xk[q], part[q]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) part[q] += dpp_mov<0xB1>(part[q]);  // the two halves of row r
+    if (hf == 0)
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            xs[r * NR + q] = (r < nbp && q < nrhs) ? b[(r0 + r) * nrhs + q] - part[q] : T(0);
+    {
+        VT* dst = reinterpret_cast<VT*>(&Ts[r * TP + hf * H]);
+#pragma unroll
+        for (int k = 0; k < H; k += V) {
+            VT u;
+            T* up = reinterpret_cast<T*>(&u);
+#pragma unroll
+            for (int q = 0; q < V; ++q) up[q] = trow[k + q];
+            dst[k / V] = u;
+        }
+    }
+    __syncthreads();
+    T y[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) y[q] = T(0);
+    {
+        const T* tr = &Ts[r * TP + hf * H];
+#pragma unroll 1
+        for (int k0 = 0; k0 < H; k0 += V) {
+            const VT u = *reinterpret_cast<const VT*>(tr + k0);
+            const T* up = reinterpret_cast<const T*>(&u);
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const T* xk = &xs[(hf * H + k0 + k) * NR];
+#pragma unroll
+                for (int q = 0; q < NR; ++q) y[q] = fma(up[k], xk[q], y[q]);
             }
         }
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        T v = acc[q];
-        v += __shfl_xor(v, 1);
-        v += __shfl_xor(v, 2);
-        v += __shfl_xor(v, 4);
-        acc[q] = v;
-    }
-    if (r < hi && sub < nrhs) {
-        T v = T(0);
+    for (int q = 0; q < NR; ++q) y[q] += dpp_mov<0xB1>(y[q]);
+    if (tid == 0 && gave_up) atomicOr(&sync[1], 2u);
+    __shared__ int bad;
+    if (tid == 0) bad = gave_up;
+    __syncthreads();
+    if (hf == 0 && r < nbp)
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == sub) v = acc[q];
-        src[r * nrhs + sub] -= v;
-    }
+        for (int q = 0; q < NR; ++q)
+            if (q < nrhs) x[(r0 + r) * nrhs + q] = bad ? T(NAN) : y[q];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&flags[p], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // dst[i][:] = scale_c * src[perm[i]][:] with a type conversion (gather of the
@@ -556,10 +850,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                                         hipMemcpyDeviceToDevice, st));
                 FL_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st));
             }
-            hipLaunchKernelGGL(diag_getrf_kernel<T>, dim3(1), dim3(1024), 0, st, Akk, lda, nb, ks, ipiv, info_d);
-            FL_LAUNCHED();
-            hipLaunchKernelGGL(trtri_kernel<T>, dim3(2, NB / TR_COLS), dim3(128), 0, st, (const T*)Akk, lda, nb, invL,
-                               invU);
+            hipLaunchKernelGGL(diag_lu_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, info_d, invL, invU);
             FL_LAUNCHED();
             if (m > nb) {
                 // L21 = A21 inv(U11), in place, with max |L21| recorded
@@ -652,39 +943,38 @@ size_t getrs_ws_bytes(int64_t n, int64_t nrhs)
     return align_up((size_t)n * nrhs * sizeof(T), 256);
 }
 
-// Solves in place on a contiguous n x nrhs buffer `y` that already holds the
-// permuted right-hand side; tmp is another n x nrhs buffer.  Result in y.
-template <typename T>
-int solve_permuted(const T* LU, int64_t n, int64_t lda, void* aux, T* y, T* tmp, int nrhs, hipStream_t st)
+// Words of the chained solves' sync area (two sweeps).
+size_t chain_sync_bytes(int64_t n) { return align_up((size_t)2 * (div_up(n, NB) + 2) * sizeof(unsigned), 256); }
+
+template <typename T, bool LOWER>
+void launch_chain(const T* LU, int64_t lda, int64_t n, const T* invs, const T* b, T* x, int nrhs, unsigned* sync,
+                  hipStream_t st)
 {
-    const int64_t nblk = div_up(n, NB);
-    // forward: L z = y  (z_b written to tmp; rows of y below block b updated)
-    for (int64_t p = 0; p < nblk; ++p) {
-        const int64_t b0 = p * NB;
-        const int nbk = (int)std::min<int64_t>(NB, n - b0);
-        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(256), 0, st, (const T*)FactorAux<T>::invL(aux, n, p), b0,
-                           nbk, (const T*)y, tmp, nrhs);
-        FL_LAUNCHED();
-        const int64_t rows = n - b0 - nbk;
-        if (rows > 0) {
-            hipLaunchKernelGGL((trsv_update_kernel<T, true>), dim3((unsigned)div_up(rows, 32)), dim3(256), 0, st, LU,
-                               lda, n, b0, nbk, (const T*)tmp, y, nrhs);
-            FL_LAUNCHED();
-        }
-    }
-    // backward: U x = z  (x_b written to y; rows of tmp above block b updated)
-    for (int64_t p = nblk - 1; p >= 0; --p) {
-        const int64_t b0 = p * NB;
-        const int nbk = (int)std::min<int64_t>(NB, n - b0);
-        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(256), 0, st, (const T*)FactorAux<T>::invU(aux, n, p), b0,
-                           nbk, (const T*)tmp, y, nrhs);
-        FL_LAUNCHED();
-        if (b0 > 0) {
-            hipLaunchKernelGGL((trsv_update_kernel<T, false>), dim3((unsigned)div_up(b0, 32)), dim3(256), 0, st, LU,
-                               lda, n, b0, nbk, (const T*)y, tmp, nrhs);
-            FL_LAUNCHED();
-        }
-    }
+    const dim3 grid((unsigned)div_up(n, NB)), blk(256);
+    if (nrhs <= 1) hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 1>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+    else if (nrhs <= 2)
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 2>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+    else if (nrhs <= 4)
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 4>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+    else hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 8>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+}
+
+// Solves in place on a contiguous n x nrhs buffer `y` that already holds the
+// permuted right-hand side; tmp is another n x nrhs buffer, sync holds
+// chain_sync_bytes(n).  Result in y.
+template <typename T>
+int solve_permuted(const T* LU, int64_t n, int64_t lda, void* aux, T* y, T* tmp, int nrhs, unsigned* sync,
+                   hipStream_t st)
+{
+    const int64_t words = (int64_t)(chain_sync_bytes(n) / sizeof(unsigned));
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)div_up(words, 256)), dim3(256), 0, st, sync, words);
+    FL_LAUNCHED();
+    const T* invs = FactorAux<T>::invL(aux, n, 0);
+    const int64_t half = div_up(n, NB) + 2;
+    launch_chain<T, true>(LU, lda, n, invs, (const T*)y, tmp, nrhs, sync, st);
+    FL_LAUNCHED();
+    launch_chain<T, false>(LU, lda, n, invs, (const T*)tmp, y, nrhs, sync + half, st);
+    FL_LAUNCHED();
     return FL_OK;
 }
 
@@ -697,15 +987,17 @@ int getrs_impl(const T* LU, int64_t n, int64_t lda, const void* aux_c, T* B, int
     FL_CHECK(ldb >= nrhs, "ldb < nrhs");
     if (n == 0 || nrhs == 0) return FL_OK;
     const size_t one = getrs_ws_bytes<T>(n, nrhs);
-    if (ws_bytes < 2 * one || !ws) return set_error(FL_EWORKSPACE, "getrs needs %zu workspace bytes", 2 * one);
+    const size_t need = 2 * one + chain_sync_bytes(n);
+    if (ws_bytes < need || !ws) return set_error(FL_EWORKSPACE, "getrs needs %zu workspace bytes", need);
     void* aux = const_cast<void*>(aux_c);
     T* y = static_cast<T*>(ws);
     T* tmp = reinterpret_cast<T*>(static_cast<char*>(ws) + one);
+    unsigned* sync = reinterpret_cast<unsigned*>(static_cast<char*>(ws) + 2 * one);
     const unsigned g = (unsigned)div_up(n * nrhs, 256);
     hipLaunchKernelGGL((gather_rows_kernel<T, T>), dim3(g), dim3(256), 0, st, (const T*)B, ldb, y, n, (int)nrhs,
                        (const int32_t*)FactorAux<T>::perm(aux), (const double*)nullptr);
     FL_LAUNCHED();
-    if (int s = solve_permuted<T>(LU, n, lda, aux, y, tmp, (int)nrhs, st)) return s;
+    if (int s = solve_permuted<T>(LU, n, lda, aux, y, tmp, (int)nrhs, sync, st)) return s;
     hipLaunchKernelGGL((copy_rows_kernel<T, T>), dim3(g), dim3(256), 0, st, (const T*)y, B, ldb, n, (int)nrhs);
     FL_LAUNCHED();
     return FL_OK;
@@ -714,7 +1006,7 @@ int getrs_impl(const T* LU, int64_t n, int64_t lda, const void* aux_c, T* B, int
 size_t refine_ws_bytes(int64_t n, int64_t nrhs)
 {
     return 3 * align_up((size_t)n * nrhs * sizeof(double), 256) + 2 * align_up((size_t)n * nrhs * sizeof(float), 256) +
-           1024;
+           1024 + chain_sync_bytes(n);
 }
 
 }  // namespace
@@ -741,6 +1033,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     double* scale = small + 16;
     double* inv_scale = small + 24;
     double* bnorms = small + 32;
+    unsigned* sync = reinterpret_cast<unsigned*>(w + 3 * d8 + 2 * d4 + 1024);
     const unsigned g = (unsigned)div_up(n * nrhs, 256);
     const int q = (int)nrhs;
 
@@ -775,7 +1068,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
                            (int64_t)nrhs, y, n, q, (const int32_t*)FactorAux<float>::perm(aux),
                            (const double*)inv_scale);
         FL_LAUNCHED();
-        if (int s = solve_permuted<float>(LU32, n, lda32, aux, y, tmp, q, st)) return s;
+        if (int s = solve_permuted<float>(LU32, n, lda32, aux, y, tmp, q, sync, st)) return s;
         hipLaunchKernelGGL(axpy_rows_kernel<float>, dim3(g), dim3(256), 0, st, xc, (int64_t)nrhs, (const float*)y, n, q,
                            (const double*)scale);
         FL_LAUNCHED();
@@ -786,8 +1079,13 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     }
     hipLaunchKernelGGL((copy_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, (const double*)xc, x, ldb, n, q);
     FL_LAUNCHED();
+    unsigned sync_err[2] = {0u, 0u};
+    const int64_t half = div_up(n, NB) + 2;
+    FL_HIP(hipMemcpyAsync(&sync_err[0], sync + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipMemcpyAsync(&sync_err[1], sync + half + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     FL_HIP(hipStreamSynchronize(st));
     if (iters_out) *iters_out = it;
+    if (sync_err[0] | sync_err[1]) return set_error(FL_EHIP, "triangular solve: a block wait timed out");
     if (!(worst <= tol)) return set_error(FL_ENOCONV, "refinement reached %.3e after %d steps (tol %.1e)", worst, it, tol);
     return FL_OK;
 }
@@ -802,7 +1100,8 @@ size_t getrf_workspace_bytes(int64_t n, int precision)
 }
 size_t getrs_workspace_bytes(int64_t n, int64_t nrhs, int precision)
 {
-    return 2 * (precision == FL_FP64 ? getrs_ws_bytes<double>(n, nrhs) : getrs_ws_bytes<float>(n, nrhs));
+    return 2 * (precision == FL_FP64 ? getrs_ws_bytes<double>(n, nrhs) : getrs_ws_bytes<float>(n, nrhs)) +
+           chain_sync_bytes(n);
 }
 size_t refine_workspace_bytes(int64_t n, int64_t nrhs) { return refine_ws_bytes(n, nrhs); }
 
