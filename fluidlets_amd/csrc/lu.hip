@@ -216,6 +216,68 @@ __device__ __forceinline__ void tri_inverse_upper(const T* F, int lf, int j, T* 
     for (int q = 0; q < 64; ++q) X[(63 - q) * lx + j] = x[q];
 }
 
+// inv(L) and inv(U) of the factored NB x NB block in S (pitch NB + 1), as
+// NB x NB row-major arrays (ld NB, zero outside the triangle): the four 64 x 64
+// diagonal triangles by substitution (waves 0, 1: L11, L22; waves 2, 3: U11,
+// U22), then inv(L)21 = -inv(L22) L21 inv(L11) and inv(U)12 = -inv(U11) U12
+// inv(U22).  Overwrites S's L21 and U12 blocks.  All 256 lanes call it.
+template <typename T>
+__device__ __forceinline__ void block_inverses(T* S, T* invL, T* invU, int tid)
+{
+    constexpr int H = NB / 2;
+    constexpr int LS = NB + 1;
+    const int lane = tid & 63, wave = tid >> 6;
+    if (wave < 2) tri_inverse_lower<T>(S + (wave * 64) * (LS + 1), LS, lane, invL + (wave * 64) * (NB + 1), NB);
+    else tri_inverse_upper<T>(S + ((wave - 2) * 64) * (LS + 1), LS, lane, invU + ((wave - 2) * 64) * (NB + 1), NB);
+    if (wave == 0)
+#pragma unroll
+        for (int i = 0; i < 64; ++i) invL[i * NB + 64 + lane] = T(0);
+    if (wave == 3)
+#pragma unroll
+        for (int i = 0; i < 64; ++i) invU[(64 + i) * NB + lane] = T(0);
+    __syncthreads();  // the diagonal inverse blocks are read back by the other waves
+    {
+        const int i = tid >> 2, jb = (tid & 3) * 16;
+        T tl[16], tu[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
+        for (int m = 0; m < H; ++m) {
+            const T l21 = S[(H + i) * LS + m];
+            const T u12 = S[i * LS + H + m];
+            const T* xl1 = invL + m * NB + jb;
+            const T* xu2 = invU + (H + m) * NB + H + jb;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                tl[q] = fma(l21, xl1[q], tl[q]);
+                tu[q] = fma(u12, xu2[q], tu[q]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            S[(H + i) * LS + jb + q] = tl[q];  // L21 inv(L11) in place of L21
+            S[i * LS + H + jb + q] = tu[q];    // U12 inv(U22) in place of U12
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
+        for (int m = 0; m < H; ++m) {
+            const T xl2 = invL[(H + i) * NB + H + m];
+            const T xu1 = invU[i * NB + m];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                tl[q] = fma(xl2, S[(H + m) * LS + jb + q], tl[q]);
+                tu[q] = fma(xu1, S[m * LS + H + jb + q], tu[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            invL[(H + i) * NB + jb + q] = -tl[q];
+            invU[i * NB + H + jb + q] = -tu[q];
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void diag_lu_kernel(T* A, int64_t lda, int nb, int64_t k0, int32_t* ipiv,
                                                       int32_t* info, T* invL, T* invU)
@@ -343,56 +405,148 @@ __global__ __launch_bounds__(256) void diag_lu_kernel(T* A, int64_t lda, int nb,
         A[r * lda + c] = S[r * LS + c];
     }
 
-    // inverses of the four 64 x 64 diagonal triangles (waves 0, 1: L11, L22; waves 2, 3: U11, U22)
-    if (wave < 2) tri_inverse_lower<T>(S + (wave * 64) * (LS + 1), LS, lane, invL + (wave * 64) * (NB + 1), NB);
-    else tri_inverse_upper<T>(S + ((wave - 2) * 64) * (LS + 1), LS, lane, invU + ((wave - 2) * 64) * (NB + 1), NB);
-    if (wave == 0)
+    block_inverses<T>(S, invL, invU, tid);
+}
+
+// ---------------------------------------------------------------- diagonal block without interchanges
+// The resistance matrices never need a row interchange (every pivot is the
+// largest entry left in its column), and then GEPP is LU without
+// interchanges.  This kernel factors the nb x nb diagonal block that way, by
+// 32 x 32 sub-blocks (right-looking):
+//   leaf   wave 0 factors the 32 x 32 diagonal sub-block, one row per lane,
+//          the pivot row broadcast with readlane (no barrier per column);
+//   panels rows below: L = A inv(U_leaf), columns right: U = inv(L_leaf) A,
+//          by substitution, one row or column per lane;
+//   update the trailing sub-blocks, 4 x 4 register tiles per lane;
+// records ipiv[c] = c and folds the largest |multiplier| into *lmax_bits
+// (float bits, atomic max; infinite for a zero pivot).  If that exceeds 1,
+// some pivot was not the largest in its column and the caller refactors the
+// panel with interchanges.
+// Then forms inv(L11), inv(U11) as the pivoting kernel does.
+constexpr int SUBB = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int nb, int64_t k0, int32_t* ipiv,
+                                                         unsigned int* lmax_bits, T* invL, T* invU)
+{
+    constexpr int LS = NB + 1;
+    __shared__ T S[NB * LS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int r = idx / NB, c = idx % NB;
+        S[r * LS + c] = (r < nb && c < nb) ? A[r * lda + c] : (r == c ? T(1) : T(0));
+    }
+    for (int c = tid; c < nb; c += 256) ipiv[k0 + c] = (int32_t)(k0 + c);
+    __syncthreads();
+    T lmax = T(0);
+    for (int b = 0; b < NB / SUBB; ++b) {
+        const int o = b * SUBB;
+        if (o >= nb) break;  // uniform: the rest is identity padding
+        // leaf: rows o .. o + 31 by lanes 0 .. 31 of wave 0
+        if (wave == 0) {
+            T a[SUBB];
+            const int r = o + (lane & (SUBB - 1));
 #pragma unroll
-        for (int i = 0; i < 64; ++i) invL[i * NB + 64 + lane] = T(0);
-    if (wave == 3)
+            for (int j = 0; j < SUBB; ++j) a[j] = S[r * LS + o + j];
 #pragma unroll
-        for (int i = 0; i < 64; ++i) invU[(64 + i) * NB + lane] = T(0);
-    __syncthreads();  // the diagonal inverse blocks are read back by the other waves
-    {
-        const int i = tid >> 2, jb = (tid & 3) * 16;
-        T tl[16], tu[16];
+            for (int c = 0; c < SUBB; ++c) {
+                const T pv = read_lane(a[c], c);
+                if (pv == T(0)) lmax = T(INFINITY);  // a zero pivot: leave it to the pivoting tiers
+                const T rp = pv != T(0) ? T(1) / pv : T(1);
+                const bool below = (lane & (SUBB - 1)) > c && lane < SUBB;
+                const T l = below ? a[c] * rp : T(0);
+                if (below) {
+                    a[c] = l;
+                    lmax = fmax(lmax, l < T(0) ? -l : l);
+                }
 #pragma unroll
-        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
-        for (int m = 0; m < H; ++m) {
-            const T l21 = S[(H + i) * LS + m];
-            const T u12 = S[i * LS + H + m];
-            const T* xl1 = invL + m * NB + jb;
-            const T* xu2 = invU + (H + m) * NB + H + jb;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                tl[q] = fma(l21, xl1[q], tl[q]);
-                tu[q] = fma(u12, xu2[q], tu[q]);
+                for (int j = c + 1; j < SUBB; ++j) a[j] = fma(-l, read_lane(a[j], c), a[j]);
             }
+            if (lane < SUBB)
+#pragma unroll
+                for (int j = 0; j < SUBB; ++j) S[r * LS + o + j] = a[j];
         }
         __syncthreads();
+        const int rest = NB - o - SUBB;
+        if (rest > 0) {
+            if (tid < 128) {
+                // row i below the leaf: x U_leaf = a
+                const int i = o + SUBB + tid;
+                if (tid < rest) {
+                    T x[SUBB];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            S[(H + i) * LS + jb + q] = tl[q];  // L21 inv(L11) in place of L21
-            S[i * LS + H + jb + q] = tu[q];    // U12 inv(U22) in place of U12
-        }
-        __syncthreads();
+                    for (int j = 0; j < SUBB; ++j) x[j] = S[i * LS + o + j];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
-        for (int m = 0; m < H; ++m) {
-            const T xl2 = invL[(H + i) * NB + H + m];
-            const T xu1 = invU[i * NB + m];
+                    for (int c = 0; c < SUBB; ++c) {
+                        const T d = S[(o + c) * LS + o + c];
+                        if (d == T(0)) lmax = T(INFINITY);
+                        x[c] = x[c] * (d != T(0) ? T(1) / d : T(1));
+                        lmax = fmax(lmax, x[c] < T(0) ? -x[c] : x[c]);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                tl[q] = fma(xl2, S[(H + m) * LS + jb + q], tl[q]);
-                tu[q] = fma(xu1, S[m * LS + H + jb + q], tu[q]);
+                        for (int j = c + 1; j < SUBB; ++j) x[j] = fma(-x[c], S[(o + c) * LS + o + j], x[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < SUBB; ++j) S[i * LS + o + j] = x[j];
+                }
+            } else {
+                // column j right of the leaf: L_leaf y = a (unit diagonal)
+                const int jc = o + SUBB + (tid - 128);
+                if (tid - 128 < rest) {
+                    T y[SUBB];
+#pragma unroll
+                    for (int i = 0; i < SUBB; ++i) y[i] = S[(o + i) * LS + jc];
+#pragma unroll
+                    for (int c = 0; c < SUBB; ++c)
+#pragma unroll
+                        for (int i = c + 1; i < SUBB; ++i) y[i] = fma(-S[(o + i) * LS + o + c], y[c], y[i]);
+#pragma unroll
+                    for (int i = 0; i < SUBB; ++i) S[(o + i) * LS + jc] = y[i];
+                }
             }
-        }
+            __syncthreads();
+            // trailing update of rows and columns o + 32 .. NB - 1, 4 x 4 per tile
+            const int q4 = rest / 4;
+            for (int tile = tid; tile < q4 * q4; tile += 256) {
+                const int i0 = o + SUBB + (tile / q4) * 4, j0 = o + SUBB + (tile % q4) * 4;
+                T acc[4][4];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            invL[(H + i) * NB + jb + q] = -tl[q];
-            invU[i * NB + H + jb + q] = -tu[q];
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[u][v] = S[(i0 + u) * LS + j0 + v];
+#pragma unroll 4
+                for (int k = 0; k < SUBB; ++k) {
+                    T lv[4], uv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) lv[u] = S[(i0 + u) * LS + o + k];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) uv[v] = S[(o + k) * LS + j0 + v];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[u][v] = fma(-lv[u], uv[v], acc[u][v]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) S[(i0 + u) * LS + j0 + v] = acc[u][v];
+            }
+            __syncthreads();
         }
     }
+    // largest multiplier of the block -> *lmax_bits
+    const T wl = wave_max(lmax);
+    if (lane == 0) atomicMax(lmax_bits, __float_as_uint((float)wl));
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx / nb, c = idx % nb;
+        A[r * lda + c] = S[r * LS + c];
+    }
+    block_inverses<T>(S, invL, invU, tid);
+}
+
+// info = the first zero pivot: keeps an earlier one.
+__global__ void fold_info_kernel(const int32_t* blk, int32_t* info)
+{
+    if (*blk != 0) atomicCAS(info, 0, *blk);
 }
 
 // ---------------------------------------------------------------- row interchanges
@@ -823,7 +977,8 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         return set_error(FL_EWORKSPACE, "getrf needs %zu workspace bytes", getrf_ws_bytes<T>(n));
     T* backup = static_cast<T*>(ws);
     char* tail = static_cast<char*>(ws) + align_up((size_t)n * NB * sizeof(T), 256);
-    unsigned int* flag = reinterpret_cast<unsigned int*>(tail);
+    unsigned int* flag = reinterpret_cast<unsigned int*>(tail);      // largest |multiplier| of a panel
+    int32_t* info_blk = reinterpret_cast<int32_t*>(tail + 4);        // zero pivot found by tier 1
     int32_t* info_d = reinterpret_cast<int32_t*>(tail + 64);
     FL_HIP(hipMemsetAsync(info_d, 0, sizeof(int32_t), st));
     int fallbacks = 0;
@@ -845,28 +1000,15 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             T* Akk = A + ks * lda + ks;
             T* invL = FactorAux<T>::invL(aux, n, p);
             T* invU = FactorAux<T>::invU(aux, n, p);
-            if (m > nb) {
-                FL_HIP(hipMemcpy2DAsync(backup, nb * sizeof(T), Akk, lda * sizeof(T), nb * sizeof(T), m,
-                                        hipMemcpyDeviceToDevice, st));
-                FL_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), st));
-            }
-            hipLaunchKernelGGL(diag_lu_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, info_d, invL, invU);
-            FL_LAUNCHED();
-            if (m > nb) {
-                // L21 = A21 inv(U11), in place, with max |L21| recorded
-                if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0), Akk + nb * lda,
-                                           lda, flag, st))
-                    return s;
-                unsigned int bits = 0;
-                FL_HIP(hipMemcpyAsync(&bits, flag, sizeof(bits), hipMemcpyDeviceToHost, st));
-                FL_HIP(hipStreamSynchronize(st));
-                float lmax;
-                std::memcpy(&lmax, &bits, sizeof(float));
-                if (!(lmax <= 1.0f + 1.0f / (1 << 20))) {
-                    // restore the sub-panel and pivot over every row below the diagonal
+            // Tier 0: no interchanges; tier 1: interchanges inside the diagonal
+            // block; tier 2: pivot search over every row below the diagonal.
+            // A tier is accepted when no multiplier of the panel exceeds 1
+            // (then its pivots are the ones partial pivoting chooses).
+            FL_HIP(hipMemcpy2DAsync(backup, nb * sizeof(T), Akk, lda * sizeof(T), nb * sizeof(T), m,
+                                    hipMemcpyDeviceToDevice, st));
+            for (int tier = 0;; ++tier) {
+                if (tier == 2) {
                     ++fallbacks;
-                    FL_HIP(hipMemcpy2DAsync(Akk, lda * sizeof(T), backup, nb * sizeof(T), nb * sizeof(T), m,
-                                            hipMemcpyDeviceToDevice, st));
                     for (int64_t c = ks; c < ks + nb; ++c) {
                         hipLaunchKernelGGL(pivot_col_kernel<T>, dim3(1), dim3(1024), 0, st, A, lda, n, ks, nb, c, ipiv,
                                            info_d);
@@ -880,7 +1022,36 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                     hipLaunchKernelGGL(trtri_kernel<T>, dim3(2, NB / TR_COLS), dim3(128), 0, st, (const T*)Akk, lda,
                                        nb, invL, invU);
                     FL_LAUNCHED();
+                    break;
                 }
+                FL_HIP(hipMemsetAsync(flag, 0, 2 * sizeof(unsigned int), st));  // flag and info_blk
+                if (tier == 0)
+                    hipLaunchKernelGGL(diag_nopiv_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag,
+                                       invL, invU);
+                else
+                    hipLaunchKernelGGL(diag_lu_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, info_blk,
+                                       invL, invU);
+                FL_LAUNCHED();
+                // L21 = A21 inv(U11), in place, with max |L21| folded into the flag
+                if (m > nb)
+                    if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0),
+                                               Akk + nb * lda, lda, flag, st))
+                        return s;
+                unsigned int bits = 0;
+                FL_HIP(hipMemcpyAsync(&bits, flag, sizeof(bits), hipMemcpyDeviceToHost, st));
+                FL_HIP(hipStreamSynchronize(st));
+                float lmax;
+                std::memcpy(&lmax, &bits, sizeof(float));
+                if (lmax <= 1.0f + 1.0f / (1 << 20)) {
+                    if (tier == 1) {
+                        hipLaunchKernelGGL(fold_info_kernel, dim3(1), dim3(1), 0, st, (const int32_t*)info_blk, info_d);
+                        FL_LAUNCHED();
+                    }
+                    break;
+                }
+                // restore the sub-panel (and a clean info) and try the next tier
+                FL_HIP(hipMemcpy2DAsync(Akk, lda * sizeof(T), backup, nb * sizeof(T), nb * sizeof(T), m,
+                                        hipMemcpyDeviceToDevice, st));
             }
             // this sub-panel's interchanges on the rest of the outer panel
             if (int s = swap_cols(ks, nb, k, ks)) return s;
