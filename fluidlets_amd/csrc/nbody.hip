@@ -109,16 +109,17 @@ __device__ __forceinline__ void pair_stokeslet(double d0, double d1, double d2, 
 // s5 = Qi^-5/2, h1f = (Qf + e2) s3f, a = s3f - s3, k = 6 h z s5:
 //   b = a + k,  g = dzf a - k dzi,  t = dzf a + k dzi,
 //   C = h1f - Qi^-1/2 - s3 (e2 + 2 h z) + e2 k,
-//   Z = h1f + dzf^2 s3f - Qi^-1/2 - s3 (e2 + h^2 + z^2) - k (e2 + dzi^2).
+//   Z = h1f + dzf^2 s3f - Qi^-1/2 - s3 (e2 + h^2 + z^2) - k (e2 + dzi^2)
+//     = C + dzf^2 a - k (2 e2 + dzi^2)          (dzf^2 + 2 h z = h^2 + z^2).
 // (Swapping target and source maps G to its transpose: g <-> -t under
 // d -> -d, the reciprocal symmetry of the image system.)
 struct WallBlock {
     double C, b, g, t, Z;
 };
 
-// h2 = 2h, h6 = 6h, ehh = e2 + h^2 (per source); zz = z^2 (per target).
+// h2 = 2h, h6 = 6h, e2x2 = 2 e2 (per source).
 __device__ __forceinline__ WallBlock wall_block(double d0, double d1, double z, double h, double e2, double h2,
-                                                double h6, double ehh, double zz)
+                                                double h6, double e2x2)
 {
     const double dzf = z - h, dzi = z + h;
     const double tq = fma(d0, d0, d1 * d1) + e2;
@@ -134,20 +135,21 @@ __device__ __forceinline__ WallBlock wall_block(double d0, double d1, double z, 
     const double k = (h6 * z) * s5;
     const double a = s3f - s3;
     const double kd = k * dzi;
+    const double p = dzf * a;
     WallBlock B;
     B.b = a + k;
-    B.g = fma(dzf, a, -kd);
-    B.t = fma(dzf, a, kd);
+    B.g = p - kd;
+    B.t = p + kd;
     B.C = fma(e2, k, fma(-s3, fma(h2, z, e2), hr));
-    B.Z = fma(-k, fma(dzi, dzi, e2), fma(-s3, ehh + zz, fma(dzf * dzf, s3f, hr)));
+    B.Z = fma(-k, fma(dzi, dzi, e2x2), fma(dzf, p, B.C));
     return B;
 }
 
 template <int NF>
 __device__ __forceinline__ void pair_wall(double d0, double d1, double z, double h, double e2, double h2, double h6,
-                                          double ehh, double zz, const double* f, double* acc)
+                                          double e2x2, const double* f, double* acc)
 {
-    const WallBlock B = wall_block(d0, d1, z, h, e2, h2, h6, ehh, zz);
+    const WallBlock B = wall_block(d0, d1, z, h, e2, h2, h6, e2x2);
 #pragma unroll
     for (int r = 0; r < NF; ++r) {
         const double f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
@@ -329,7 +331,7 @@ __device__ __forceinline__ void store_result(const NbArgs& a, int64_t i, const d
 }
 
 // LDS record of one source for the FP64 kernels: x, y, z, eps^2, then for the
-// wall kernel 2h, 6h, eps^2 + h^2 (h = z, the source height), then f (3*NF).
+// wall kernel 2h, 6h, 2 eps^2 (h = z, the source height), then f (3*NF).
 template <int KIND, int NF>
 struct Src64 {
     static constexpr int FOFF = KIND == FL_WALL_STOKESLET3D ? 7 : 4;  // offset of the strengths
@@ -346,7 +348,7 @@ __device__ __forceinline__ void fill_src64(double* rec, double x, double y, doub
     if (KIND == FL_WALL_STOKESLET3D) {
         rec[4] = 2.0 * z;
         rec[5] = 6.0 * z;
-        rec[6] = fma(z, z, e2);
+        rec[6] = 2.0 * e2;
     }
 }
 
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
     const int tid = threadIdx.x;
     const int64_t M = a.M, N = a.N;
 
-    double tx[TPL], ty[TPL], tz[TPL], tzz[TPL], acc[TPL][3 * NF];
+    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
         int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
@@ -367,7 +369,6 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
         tx[t] = a.tgt[ic];
         ty[t] = a.tgt[M + ic];
         tz[t] = a.tgt[2 * M + ic];
-        tzz[t] = tz[t] * tz[t];
 #pragma unroll
         for (int k = 0; k < 3 * NF; ++k) acc[t][k] = 0.0;
     }
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
                 if (KIND == FL_STOKESLET3D)
                     pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + FO, acc[t]);
                 else if (KIND == FL_WALL_STOKESLET3D)
-                    pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t], r[2], r[3], r[4], r[5], r[6], tzz[t], r + FO,
+                    pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t], r[2], r[3], r[4], r[5], r[6], r + FO,
                                   acc[t]);
                 else
                     pair_brinkman<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], a.alpha, a.inv_alpha, r + FO,
@@ -569,13 +570,13 @@ __device__ __forceinline__ int64_t sym_pair_index(int64_t K, int64_t L, int64_t 
 // Block of (target, source) applied to the source strength f (row) and, when
 // COL, its transpose applied to the target strength g (added to col).
 template <int KIND, bool COL>
-__device__ __forceinline__ void sym_pair(double d0, double d1, double d2, double z, double zz, const double* r,
+__device__ __forceinline__ void sym_pair(double d0, double d1, double d2, double z, const double* r,
                                          const double* g, double alpha, double inv_alpha, double* acc, double* col)
 {
     constexpr int FO = Src64<KIND, 1>::FOFF;
     const double f0 = r[FO], f1 = r[FO + 1], f2 = r[FO + 2];
     if (KIND == FL_WALL_STOKESLET3D) {
-        const WallBlock B = wall_block(d0, d1, z, r[2], r[3], r[4], r[5], r[6], zz);
+        const WallBlock B = wall_block(d0, d1, z, r[2], r[3], r[4], r[5], r[6]);
         const double w = fma(d0, f0, d1 * f1);
         const double v = fma(B.b, w, B.g * f2);
         acc[0] = fma(B.C, f0, fma(d0, v, acc[0]));
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
     const int64_t N = a.N;
     const double e2 = a.eps[0] * a.eps[0];
 
-    double tx[2], ty[2], tz[2], tzz[2], g[2][3], acc[2][3];
+    double tx[2], ty[2], tz[2], g[2][3], acc[2][3];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int64_t i = (int64_t)K * SB + t * WG + tid;
@@ -650,7 +651,6 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
         tx[t] = a.x[ic];
         ty[t] = a.x[N + ic];
         tz[t] = a.x[2 * N + ic];
-        tzz[t] = tz[t] * tz[t];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             g[t][c] = i < N ? a.f[c * N + i] : 0.0;
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
                     }
 #pragma unroll
                     for (int t = 0; t < 2; ++t)
-                        sym_pair<KIND, false>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], tzz[t], r, g[t],
+                        sym_pair<KIND, false>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], r, g[t],
                                               a.alpha, a.inv_alpha, acc[t], nullptr);
                 }
                 continue;
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
                         double col[3] = {0.0, 0.0, 0.0};
 #pragma unroll
                         for (int t = 0; t < 2; ++t)
-                            sym_pair<KIND, true>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], tzz[t], r, g[t],
+                            sym_pair<KIND, true>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], r, g[t],
                                                  a.alpha, a.inv_alpha, acc[t], col);
 #pragma unroll
                         for (int c = 0; c < 3; ++c) red[wave][(b * 3 + c) * SRS + lane] = col[c];
