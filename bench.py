@@ -215,24 +215,32 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
     return out
 
 
-def serving_section(torch, n_requests=40, swimmer_steps=10, n_max=3000, slots=4, seed=5):
+def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4, seed=5, cache_gib=48):
     """Serving mix (70 % six-mode resistance solves on ellipsoids, 30 %
     biflagellate RK2 swimmers), reduced in size so the default run stays
-    short: n_requests requests, N <= n_max, swimmer_steps RK2 steps.  The
-    same trace is served one request at a time in plain FP64 (the baseline)
-    and by the continuous-batching server (mixed precision, `slots` streams):
-    offline (all requests queued at t = 0) and online (Poisson arrivals at
-    0.9 x the baseline's measured capacity)."""
+    short: n_requests requests, N <= n_max, swimmer_steps RK2 steps (50 = one
+    beat).  The same trace is served one request at a time in plain FP64 (the
+    baseline: every step factors its matrix) and by the continuous-batching
+    server (mixed precision, `slots` streams, a resident factor cache of
+    cache_gib GiB that starts empty): offline (all requests queued at t = 0)
+    and online (Poisson arrivals at 0.9 x the baseline's measured capacity)."""
     from fluidlets_amd.serving import Request, SolveServer, serving_mix, summarize
 
     reqs = serving_mix(n_requests, seed=seed, n_max=n_max, swimmer_steps=swimmer_steps, rate=1.0)
+    warm = [r for r in reqs if r.kind == "resistance"][:2]
     base_srv = SolveServer(slots=1, precision="fp64")
-    fast_srv = SolveServer(slots=slots, precision="mixed")
-    base_srv.serve(reqs[:2], realtime=False)           # warm-up (JIT-free, but first allocations)
-    fast_srv.serve(reqs[:2], realtime=False)
-    out = {"requests": n_requests, "slots": slots, "swimmer_steps": swimmer_steps, "n_max": n_max}
+    base_srv.serve(warm, realtime=False)               # warm-up: first allocations and library start-up
+    SolveServer(slots=slots, precision="mixed").serve(warm, realtime=False)
+
+    def fresh():
+        return SolveServer(slots=slots, precision="mixed", cache_bytes=int(cache_gib) << 30)
+
+    out = {"requests": n_requests, "slots": slots, "swimmer_steps": swimmer_steps, "n_max": n_max,
+           "server_cache_gib": cache_gib}
     off_base = summarize(base_srv.serve(reqs, realtime=False))
+    fast_srv = fresh()
     off_fast = summarize(fast_srv.serve(reqs, realtime=False))
+    off_fast["cache_hits"], off_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
     out["offline_baseline_fp64_one_at_a_time"] = off_base
     out["offline_server_mixed"] = off_fast
     lam = 0.9 * off_base["solves_per_s"]
@@ -244,7 +252,10 @@ def serving_section(torch, n_requests=40, swimmer_steps=10, n_max=3000, slots=4,
         trace.append(Request(r.kind, r.payload, t, r.rid))
     out["arrival_rate_per_s"] = lam
     out["online_baseline_fp64_one_at_a_time"] = summarize(base_srv.serve(trace, realtime=True))
-    out["online_server_mixed"] = summarize(fast_srv.serve(trace, realtime=True))
+    fast_srv = fresh()
+    on_fast = summarize(fast_srv.serve(trace, realtime=True))
+    on_fast["cache_hits"], on_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
+    out["online_server_mixed"] = on_fast
     out["offline_throughput_speedup"] = off_fast["solves_per_s"] / off_base["solves_per_s"]
     return out
 
@@ -297,7 +308,8 @@ def main():
     ap.add_argument("--no-vendor-lu", action="store_true", help="skip the rocSOLVER FP64 LU comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix section")
     ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-swimmer-steps", type=int, default=10)
+    ap.add_argument("--serve-swimmer-steps", type=int, default=50)
+    ap.add_argument("--serve-cache-gib", type=int, default=48, help="resident factor cache of the mixed server")
     ap.add_argument("--serve-nmax", type=int, default=3000)
     ap.add_argument("--gmres", action="store_true", help="run the 131,072-point GMRES rod-carpet section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -420,7 +432,8 @@ def main():
                                                 args.solve_points, vendor=not args.no_vendor_lu)
 
     if not args.no_serve:
-        result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax)
+        result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax,
+                                                cache_gib=args.serve_cache_gib)
 
     if args.gmres:
         result["rod_carpet_gmres"] = gmres_section(torch, fl, D)

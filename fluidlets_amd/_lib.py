@@ -47,6 +47,9 @@ _SIGNATURES = {
     "fl_refine_workspace_size": (c_size_t, [c_int64, c_int64]),
     "fl_refine": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
                           c_int64, c_double, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t, c_void_p]),
+    "fl_refine_frame": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, POINTER(c_double), c_void_p,
+                                c_void_p, c_int64, c_int64, c_double, c_int, POINTER(c_int), POINTER(c_double),
+                                c_void_p, c_size_t, c_void_p]),
     "fl_gmres_workspace_size": (c_size_t, [c_int64, c_int, c_int]),
     "fl_gmres": (c_int, [POINTER(fl_kernel), c_int64, c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p,
                          c_double, c_int, c_int, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t,

@@ -72,12 +72,14 @@ def nearest_groups(nearest: np.ndarray, n_force: int):
 
 
 def assemble_nearest_device(kernel, tgt, quad, eps_t, eps_stride, order_t, start_t, n_force, mu=1.0,
-                            K64=None, fp32=False, stream=None):
+                            K64=None, fp32=False, stream=None, fp64=True):
     t = D.torch()
     M, Q = int(tgt.shape[1]), int(quad.shape[1])
-    if K64 is None:
+    if not (fp64 or fp32):
+        raise ValueError("assemble_nearest_device needs fp64 or fp32 output")
+    if K64 is None and fp64:
         K64 = t.empty((3 * M, 3 * n_force), dtype=t.float64, device=tgt.device)
-    ld = int(K64.stride(0))
+    ld = int(K64.stride(0)) if K64 is not None else 3 * n_force
     K32 = t.empty((3 * M, 3 * n_force), dtype=t.float32, device=tgt.device) if fp32 else None
     st = lib().fl_assemble_nearest(kernel_struct(kernel), M, Q, int(n_force), D.ptr(tgt), D.ptr(quad), D.ptr(eps_t),
                                    eps_stride, D.ptr(order_t), D.ptr(start_t), float(mu), D.ptr(K64), ld,

@@ -227,10 +227,12 @@ __global__ __launch_bounds__(WG) void assemble_nearest_kernel(NearArgs a)
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int64_t row = 3 * i + r;
-            double* p = a.K64 + row * a.ld + 3 * j;
-            p[0] = a.w * S[3 * r];
-            p[1] = a.w * S[3 * r + 1];
-            p[2] = a.w * S[3 * r + 2];
+            if (a.K64) {
+                double* p = a.K64 + row * a.ld + 3 * j;
+                p[0] = a.w * S[3 * r];
+                p[1] = a.w * S[3 * r + 1];
+                p[2] = a.w * S[3 * r + 2];
+            }
             if (a.K32) {
                 float* q = a.K32 + row * a.ld32 + 3 * j;
                 q[0] = (float)(a.w * S[3 * r]);
@@ -340,8 +342,8 @@ int assemble_nearest_launch(int kind, double alpha, int64_t M, int64_t Q, int64_
     FL_CHECK(kind >= 0 && kind <= 2, "unknown kernel kind %d", kind);
     FL_CHECK(M >= 0 && Q >= 0 && n_force >= 0, "negative size");
     FL_CHECK(eps_stride == 0 || eps_stride == 1, "eps_stride must be 0 or 1");
-    FL_CHECK(K64 != nullptr, "K64 is required");
-    FL_CHECK(ld >= 3 * n_force, "ld too small");
+    FL_CHECK(K64 || K32, "K64 or K32 is required");
+    FL_CHECK(!K64 || ld >= 3 * n_force, "ld too small");
     FL_CHECK(!K32 || ld32 >= 3 * n_force, "ld32 too small");
     FL_CHECK(mu_re != 0.0, "mu must be nonzero");
     if (M == 0 || n_force == 0) return FL_OK;

@@ -655,7 +655,7 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 // Workgroups take tickets in dispatch order (ticket t solves block t forward,
 // nblk - 1 - t backward) and wait on a block's flag before reading it, so a
 // workgroup waits only on workgroups dispatched before it.  x_p is published
-// by a release store of flag[p] after every thread's stores are fenced.  A
+// by a release store of flag[p] after every wave's stores have completed.  A
 // wait is bounded (CH_SPIN polls): a workgroup that gives up sets sync[1] and
 // writes NaN for its block, then publishes its flag, so every workgroup ends.
 // sync = [ticket, error, flag[0 .. nblk)], zeroed before the launch.
@@ -732,8 +732,10 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
         const int64_t c0 = j * NB;
         const int nbj = (int)min((int64_t)NB, n - c0);
         if (tid == 0) {
+            // poll with relaxed loads (an acquire per poll would invalidate
+            // this XCD's L2 every time); one acquire fence once the flag is set
             int spins = 0;
-            while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+            while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
                 if (++spins > CH_SPIN) {
                     atomicOr(&sync[1], 1u);
                     gave_up = true;
@@ -741,6 +743,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         {
@@ -820,7 +823,9 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
 #pragma unroll
         for (int q = 0; q < NR; ++q)
             if (q < nrhs) x[(r0 + r) * nrhs + q] = bad ? T(NAN) : y[q];
-    __threadfence();
+    // every wave's stores complete (reach this XCD's L2), then one release
+    // store: its L2 write-back publishes all of them to the other XCDs
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) __hip_atomic_store(&flags[p], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -839,6 +844,46 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const TS* src, int64_t
     double v = (double)src[s * lds + q];
     if (inv_scale) v *= inv_scale[q];
     dst[idx] = (TD)v;
+}
+
+// Per-point 3x3 frame R (row-major, lab = R body) for refinement with factors
+// of the body-frame matrix: the residual goes to the body frame (R^T r)
+// before the FP32 solve and the correction back to the lab frame (R d).
+struct Frame3 {
+    double r[9];
+};
+
+// dst[i][q] = inv_scale[q] * (R^T src)[perm[i]][q]; rows are point-major
+// (row 3p + a is component a of point p).
+__global__ __launch_bounds__(256) void gather_rot_kernel(const double* src, float* dst, int64_t n,
+                                                         int nrhs, const int32_t* perm, const double* inv_scale,
+                                                         Frame3 R)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const int64_t i = idx / nrhs;
+    const int q = (int)(idx % nrhs);
+    const int64_t s = perm[i];
+    const int64_t p3 = s - s % 3;
+    const int a = (int)(s % 3);
+    const double v = fma(R.r[a], src[p3 * nrhs + q],
+                         fma(R.r[3 + a], src[(p3 + 1) * nrhs + q], R.r[6 + a] * src[(p3 + 2) * nrhs + q]));
+    dst[idx] = (float)(v * inv_scale[q]);
+}
+
+// x[3p + a][q] += scale[q] * (R d)[3p + a][q]; x and d have ld nrhs.
+__global__ __launch_bounds__(256) void axpy_rot_kernel(double* x, const float* d, int64_t n, int nrhs,
+                                                       const double* scale, Frame3 R)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const int64_t i = idx / nrhs;
+    const int q = (int)(idx % nrhs);
+    const int64_t p3 = i - i % 3;
+    const int a = (int)(i % 3);
+    const double v = fma(R.r[3 * a], (double)d[p3 * nrhs + q],
+                         fma(R.r[3 * a + 1], (double)d[(p3 + 1) * nrhs + q], R.r[3 * a + 2] * (double)d[(p3 + 2) * nrhs + q]));
+    x[idx] += scale[q] * v;
 }
 
 template <typename TS, typename TD>
@@ -1249,10 +1294,18 @@ size_t refine_ws_bytes(int64_t n, int64_t nrhs)
 
 int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux_c, int64_t n,
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
-                double* resid_out, void* ws, size_t ws_bytes, hipStream_t st)
+                double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st)
 {
     FL_CHECK(n >= 0 && nrhs >= 1 && nrhs <= 8, "need 1 <= nrhs <= 8");
     FL_CHECK(ldb >= nrhs, "ldb < nrhs");
+    FL_CHECK(!frame || n % 3 == 0, "a frame needs n = 3 x points");
+    Frame3 R{};
+    if (frame) {
+        for (int k = 0; k < 9; ++k) {
+            R.r[k] = frame[k];
+            FL_CHECK(std::isfinite(R.r[k]), "frame entries must be finite");
+        }
+    }
     if (n == 0) return FL_OK;
     if (ws_bytes < refine_ws_bytes(n, nrhs) || !ws)
         return set_error(FL_EWORKSPACE, "refine needs %zu workspace bytes", refine_ws_bytes(n, nrhs));
@@ -1300,13 +1353,21 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
         if ((it > 0 && worst <= tol) || it >= max_iter) break;
         // d = LU32^{-1} (r / s) s, with s = max |r| per column so FP32 sees O(1) values
         hipLaunchKernelGGL(scale_from_norms_kernel, dim3(1), dim3(8), 0, st, (const double*)norms, scale, inv_scale, q);
-        hipLaunchKernelGGL((gather_rows_kernel<double, float>), dim3(g), dim3(256), 0, st, (const double*)r,
-                           (int64_t)nrhs, y, n, q, (const int32_t*)FactorAux<float>::perm(aux),
-                           (const double*)inv_scale);
+        if (frame)
+            hipLaunchKernelGGL(gather_rot_kernel, dim3(g), dim3(256), 0, st, (const double*)r, y, n, q,
+                               (const int32_t*)FactorAux<float>::perm(aux), (const double*)inv_scale, R);
+        else
+            hipLaunchKernelGGL((gather_rows_kernel<double, float>), dim3(g), dim3(256), 0, st, (const double*)r,
+                               (int64_t)nrhs, y, n, q, (const int32_t*)FactorAux<float>::perm(aux),
+                               (const double*)inv_scale);
         FL_LAUNCHED();
         if (int s = solve_permuted<float>(LU32, n, lda32, aux, y, tmp, q, sync, st)) return s;
-        hipLaunchKernelGGL(axpy_rows_kernel<float>, dim3(g), dim3(256), 0, st, xc, (int64_t)nrhs, (const float*)y, n, q,
-                           (const double*)scale);
+        if (frame)
+            hipLaunchKernelGGL(axpy_rot_kernel, dim3(g), dim3(256), 0, st, xc, (const float*)y, n, q,
+                               (const double*)scale, R);
+        else
+            hipLaunchKernelGGL(axpy_rows_kernel<float>, dim3(g), dim3(256), 0, st, xc, (int64_t)nrhs, (const float*)y,
+                               n, q, (const double*)scale);
         FL_LAUNCHED();
         ++it;
         hipLaunchKernelGGL(residual_kernel, dim3((unsigned)div_up(n, 4 * RES_ROWS)), dim3(256), 0, st, A64, lda64, n, b, ldb,

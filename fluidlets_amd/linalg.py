@@ -87,9 +87,12 @@ def getrs(F: LUFactor, B, stream=None):
     return B
 
 
-def refine(A64, F32: LUFactor, b, tol: float = 1e-10, max_iter: int = 30, stream=None):
+def refine(A64, F32: LUFactor, b, tol: float = 1e-10, max_iter: int = 30, stream=None, frame=None):
     """x solving A64 x = b (FP64) from the FP32 factors by iterative
     refinement with FP64 residuals.  b: (n, nrhs) float64 device tensor.
+    With `frame` (3x3, lab = frame @ body) the factors are those of the
+    operator written in the body frame (A64 ~ R A' R^T per point), and every
+    correction is rotated accordingly (fl_refine_frame).
     Returns (x, iterations, per-column final relative residuals)."""
     t = D.torch()
     L = lib()
@@ -106,8 +109,16 @@ def refine(A64, F32: LUFactor, b, tol: float = 1e-10, max_iter: int = 30, stream
         ws = D.workspace(nbytes, stream)
         it = ctypes.c_int(0)
         res = (ctypes.c_double * 8)()
-        st = L.fl_refine(D.ptr(A64), n, D.ptr(F32.lu), n, D.ptr(F32.aux), n, D.ptr(bb), D.ptr(xx), q, q, float(tol),
-                         int(max_iter), ctypes.byref(it), res, D.ptr(ws), nbytes, D.stream_handle(stream))
+        if frame is None:
+            st = L.fl_refine(D.ptr(A64), n, D.ptr(F32.lu), n, D.ptr(F32.aux), n, D.ptr(bb), D.ptr(xx), q, q,
+                             float(tol), int(max_iter), ctypes.byref(it), res, D.ptr(ws), nbytes,
+                             D.stream_handle(stream))
+        else:
+            R = np.ascontiguousarray(np.asarray(frame, dtype=np.float64).reshape(3, 3))
+            st = L.fl_refine_frame(D.ptr(A64), n, D.ptr(F32.lu), n, D.ptr(F32.aux), n,
+                                   R.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), D.ptr(bb), D.ptr(xx), q, q,
+                                   float(tol), int(max_iter), ctypes.byref(it), res, D.ptr(ws), nbytes,
+                                   D.stream_handle(stream))
         check(st, "refine")
         x[:, c0:c0 + q] = xx
         iters.append(int(it.value))

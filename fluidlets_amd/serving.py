@@ -12,7 +12,11 @@ mix.
 Resident factors: resistance requests carry a geometry key; the assembled
 FP64 matrix and the factors stay on the device in an LRU cache (counted in
 the memory budget), so another right-hand side on the same geometry costs
-only the triangular solves (and refinement).
+only the triangular solves (and refinement).  Mixed-precision swimmers keep
+the FP32 factors of each body shape they meet (in that cache when the server
+has one, else in a cache of their own, counted in their footprint), so the
+steps of a periodic beat after the first period cost only the lab-frame
+assembly, refinement solves and the 6 x 6 Schur step.
 
 Latency is measured from a request's arrival to its completion (queueing
 included); the server reports p50/p99 and throughput.
@@ -23,13 +27,13 @@ import hashlib
 import heapq
 import threading
 import time
-from collections import OrderedDict
 from dataclasses import dataclass
 from typing import Any
 
 import numpy as np
 
 from . import _device as D
+from .cache import FactorCache
 
 
 @dataclass
@@ -55,15 +59,21 @@ class Result:
         return self.end - self.arrival
 
 
-def footprint_bytes(req: Request, precision: str) -> int:
+def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) -> int:
     """HBM bytes a request holds while it runs: the FP64 matrix (8 n^2), the
-    FP32 copy for the mixed path (4 n^2), plus O(n) vectors and workspaces."""
+    FP32 copy for the mixed path (4 n^2), plus O(n) vectors and workspaces;
+    a mixed-precision swimmer without the server's shared cache also holds
+    its own cache of body-frame factors."""
     if req.kind == "resistance":
         n = 3 * len(req.payload["points"])
     else:
         n = 3 * req.payload["n_points"]
     per = 8 + (4 if precision == "mixed" else 8)
-    return int(per * n * n + 512 * n + (64 << 20))
+    extra = 0
+    if req.kind == "swimmer" and precision == "mixed" and not shared_cache:
+        from .swimming import reuse_cache_bytes
+        extra = reuse_cache_bytes(req.payload["n_points"])   # the swimmer's own factor cache
+    return int(per * n * n + 512 * n + (64 << 20) + extra)
 
 
 def geometry_key(kernel, points, eps) -> str:
@@ -72,38 +82,6 @@ def geometry_key(kernel, points, eps) -> str:
     h.update(np.ascontiguousarray(points, dtype=np.float64).tobytes())
     h.update(np.ascontiguousarray(np.asarray(eps, dtype=np.float64)).tobytes())
     return h.hexdigest()
-
-
-class FactorCache:
-    """LRU of resident (K64, factors) per geometry key, bounded in bytes."""
-
-    def __init__(self, max_bytes: int):
-        self.max_bytes = max_bytes
-        self.bytes = 0
-        self.items: OrderedDict[str, tuple] = OrderedDict()
-        self.lock = threading.Lock()
-        self.hits = 0
-        self.misses = 0
-
-    def get(self, key):
-        with self.lock:
-            item = self.items.get(key)
-            if item is not None:
-                self.items.move_to_end(key)
-                self.hits += 1
-            else:
-                self.misses += 1
-            return item
-
-    def put(self, key, item, nbytes):
-        with self.lock:
-            if nbytes > self.max_bytes:
-                return
-            while self.bytes + nbytes > self.max_bytes and self.items:
-                _, old = self.items.popitem(last=False)
-                self.bytes -= old[-1]
-            self.items[key] = (*item, nbytes)
-            self.bytes += nbytes
 
 
 def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, stream=None):
@@ -148,10 +126,10 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     return R
 
 
-def run_swimmer(payload: dict, precision: str, stream=None):
+def run_swimmer(payload: dict, precision: str, cache: FactorCache | None = None, stream=None):
     from .swimming import swim_rk2
     tr = swim_rk2(payload["shape_at"], 0.0, payload["dt"], payload["steps"], payload["eps"],
-                  precision=precision, stream=stream)
+                  precision=precision, stream=stream, factor_cache=cache)
     return np.concatenate([tr.x0[-1], tr.B[-1].ravel()])
 
 
@@ -178,7 +156,7 @@ class SolveServer:
             if req.kind == "resistance":
                 out = solve_resistance(req.payload, self.precision, self.cache, stream=stream)
             elif req.kind == "swimmer":
-                out = run_swimmer(req.payload, self.precision, stream=stream)
+                out = run_swimmer(req.payload, self.precision, self.cache, stream=stream)
             else:
                 raise ValueError(f"unknown request kind {req.kind!r}")
             stream.synchronize()
@@ -224,7 +202,7 @@ class SolveServer:
                 admitted = False
                 while waiting and free_streams:
                     req = waiting[0][2]
-                    fp = footprint_bytes(req, self.precision)
+                    fp = footprint_bytes(req, self.precision, shared_cache=self.cache is not None)
                     if fp > self.budget:
                         heapq.heappop(waiting)
                         results.append(Result(req.rid, req.kind, None, req.arrival if realtime else 0.0, now(), now(),

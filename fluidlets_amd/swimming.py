@@ -12,10 +12,23 @@ K is assembled and factored on the GPU (FP64 LU, or FP32 LU with FP64
 refinement); the seven right-hand sides [v | C] are solved with the factors
 and the 6 x 6 Schur complement S = B K^{-1} C is formed and solved in FP64 by
 the fl_swim_schur kernels.
+
+Factor reuse across time steps ("mixed" precision, free-space kernels).  For
+a kernel with G(R d) = R G(d) R^T (Stokeslet3D, Brinkmanlet3D) the lab-frame
+matrix is the body-frame matrix rotated point by point, K = R K_body R^T,
+so the FP32 factors of K_body depend only on the body shape, which a
+periodic beat revisits.  They are kept in a FactorCache keyed by the body
+shape; every step still assembles the exact lab-frame FP64 K and refines
+against it (fl_refine_frame rotates each correction by the frame's
+orthogonal polar factor), so the answer is the lab-frame solution to the
+refinement tolerance whether the factors were reused or not.  On a miss the
+factors are made the same way (from the body-frame matrix), so a step's
+result does not depend on the cache's contents.
 """
 from __future__ import annotations
 
 import ctypes
+import hashlib
 from dataclasses import dataclass
 from typing import Callable
 
@@ -24,8 +37,9 @@ import numpy as np
 from . import _device as D
 from . import linalg
 from ._lib import check, lib
+from .cache import FactorCache
 from .core.assemble import assemble_nearest_device, nearest_groups
-from .kernels import Stokeslet3D
+from .kernels import Brinkmanlet3D, Stokeslet3D
 
 
 def _cross_matrix(v: np.ndarray) -> np.ndarray:
@@ -36,6 +50,34 @@ def _cross_matrix(v: np.ndarray) -> np.ndarray:
     C[:, 1, 0], C[:, 1, 2] = v[:, 2], -v[:, 0]
     C[:, 2, 0], C[:, 2, 1] = -v[:, 1], v[:, 0]
     return C
+
+
+def _frame_rotation(B: np.ndarray) -> np.ndarray:
+    """Orthogonal polar factor of the frame matrix (B itself when B is a
+    rotation; the integrators do not re-orthonormalise B)."""
+    u, _, vt = np.linalg.svd(B)
+    return u @ vt
+
+
+def _body_key(kernel, shape, nearest, eps, mu) -> str:
+    """Cache key of a body-frame matrix: the kernel, mu, eps, the nearest map
+    and the body shape on a grid of 2^-30 of its extent (shapes that differ
+    only by rounding, e.g. one beat period later, share their factors)."""
+    h = hashlib.sha1()
+    h.update(f"{kernel!r}|{float(mu)!r}".encode())
+    x = np.asarray(shape.x, np.float64)
+    X = np.asarray(shape.X, np.float64)
+    ext = max(float(np.abs(x).max(initial=0.0)), float(np.abs(X).max(initial=0.0)), 1e-300)
+    q = 2.0 ** 30 / ext
+    for a in (x, X):
+        h.update(np.ascontiguousarray(np.rint(a * q), dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(nearest, dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(np.asarray(eps, np.float64)).tobytes())
+    return h.hexdigest()
+
+
+def _rotation_invariant(kernel) -> bool:
+    return isinstance(kernel, (Stokeslet3D, Brinkmanlet3D))
 
 
 @dataclass
@@ -62,11 +104,13 @@ class SwimState:
 
 
 def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, nearest=None,
-                     precision: str = "fp64", stream=None) -> SwimState:
+                     precision: str = "fp64", stream=None, factor_cache: FactorCache | None = None) -> SwimState:
     """Solve the instantaneous free-swimming problem (reference signature).
 
     ``precision`` "fp64" (default) factors K in FP64; "mixed" uses FP32 MFMA
-    factors with FP64 iterative refinement of all seven right-hand sides."""
+    factors with FP64 iterative refinement of all seven right-hand sides.
+    With ``factor_cache`` ("mixed", Stokeslet3D or Brinkmanlet3D) the FP32
+    factors of the body-frame matrix are taken from, or added to, the cache."""
     t = D.torch()
     kernel = Stokeslet3D() if kernel is None else kernel
     if np.iscomplexobj(mu):
@@ -88,9 +132,10 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
     eps_arr = np.broadcast_to(np.asarray(eps, float), (Q,))
     e = eps_arr if not (Q > 1 and np.all(eps_arr == eps_arr[0])) else eps_arr[:1]
     eps_t, stride = D.eps_arg(e, Q, dev)
-    K64, K32 = assemble_nearest_device(kernel, x_soa, X_soa, eps_t, stride, t.from_numpy(order).to(dev),
-                                       t.from_numpy(start).to(dev), N, mu=mu, fp32=(precision == "mixed"),
-                                       stream=stream)
+    order_t, start_t = t.from_numpy(order).to(dev), t.from_numpy(start).to(dev)
+    body = precision == "mixed" and factor_cache is not None and _rotation_invariant(kernel)
+    K64, K32 = assemble_nearest_device(kernel, x_soa, X_soa, eps_t, stride, order_t, start_t, N, mu=mu,
+                                       fp32=(precision == "mixed" and not body), stream=stream)
     L = lib()
     # every device buffer stays referenced until its kernels are enqueued
     # (a temporary freed inside the argument list could be handed to the next
@@ -104,6 +149,20 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
         if F.info > 0:
             raise np.linalg.LinAlgError("Singular matrix")
         Y = linalg.getrs(F, R, stream=stream)
+    elif precision == "mixed" and body:
+        key = _body_key(kernel, shape, nearest, e, mu)
+        item = factor_cache.get(key)
+        if item is None:
+            _, Kb32 = assemble_nearest_device(kernel, D.soa(np.asarray(shape.x, float), dev),
+                                              D.soa(np.asarray(shape.X, float), dev), eps_t, stride, order_t,
+                                              start_t, N, mu=mu, fp32=True, fp64=False, stream=stream)
+            F = linalg.getrf(Kb32, stream=stream)
+            if F.info > 0:
+                raise np.linalg.LinAlgError("Singular matrix")
+            factor_cache.put(key, (F,), F.nbytes())
+        else:
+            F = item[0]
+        Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream, frame=_frame_rotation(B))
     elif precision == "mixed":
         F = linalg.getrf(K32, stream=stream)
         if F.info > 0:
@@ -164,23 +223,40 @@ def swim(shape_at: Callable[[float], SwimmerShape], t_span, eps, mu=1.0, x0=(0, 
     return Trajectory(sol.t, sol.y[:3].T, Bt)
 
 
+def reuse_cache_bytes(n_points: int) -> int:
+    """Bound of a swimmer's own factor cache: 128 body shapes' FP32 factors
+    (two solves per step, so a beat of 64 steps or fewer fits), at most 16 GiB."""
+    n = 3 * int(n_points)
+    return int(min(16 << 30, 128 * (4 * n * n + (1 << 20))))
+
+
 def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, steps: int, eps, mu=1.0,
              x0=(0, 0, 0), B0=None, kernel=None, fixed_nearest=True, precision: str = "fp64",
-             stream=None) -> Trajectory:
+             stream=None, factor_cache: FactorCache | None = None, reuse_factors: bool = True) -> Trajectory:
     """Fixed-step explicit midpoint (RK2) integration of the frame:
     z_{n+1} = z_n + dt F(t_n + dt/2, z_n + dt/2 F(t_n, z_n)),  two device
-    swimming_problem solves per step."""
+    swimming_problem solves per step.
+
+    With precision "mixed" and a free-space kernel the body-frame FP32
+    factors are reused whenever the beat revisits a shape (``factor_cache``,
+    or a cache of this call bounded by reuse_cache_bytes); reuse_factors=False
+    factors every step."""
     B0 = np.eye(3) if B0 is None else np.asarray(B0, float)
     nearest = None
+    s0 = shape_at(t0)
     if fixed_nearest:
         from ._nearest import nearest_neighbors
-        s0 = shape_at(t0)
         nearest = nearest_neighbors(s0.X, s0.x)
+    if not reuse_factors or precision != "mixed":
+        factor_cache = None
+    elif factor_cache is None:
+        factor_cache = FactorCache(reuse_cache_bytes(len(s0.x)))
 
     def F(t, z):
         b1, b2 = z[3:6], z[6:9]
         B = np.c_[b1, b2, np.cross(b1, b2)]
-        st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision, stream=stream)
+        st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision, stream=stream,
+                              factor_cache=factor_cache)
         return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
 
     z = np.concatenate([np.asarray(x0, float), B0[:, 0], B0[:, 1]])

@@ -131,7 +131,8 @@ int fl_assemble(const fl_kernel* kernel, int64_t M, int64_t N,
  * grouped by force point: order[] lists quadrature indices so that group j is
  * order[start[j] .. start[j+1]-1] (start has n_force+1 entries).  Each output
  * entry is written once, as the in-order sum over its group (no atomics).
- * quad is (3, Q).  K64 real FP64 (ld), optional K32 copy (ld32).
+ * quad is (3, Q).  K64 real FP64 (ld) and K32 FP32 (ld32): either may be
+ * NULL, not both.
  */
 int fl_assemble_nearest(const fl_kernel* kernel, int64_t M, int64_t Q, int64_t n_force,
                         const double* tgt, const double* quad, const double* eps, int64_t eps_stride,
@@ -202,6 +203,21 @@ size_t fl_refine_workspace_size(int64_t n, int64_t nrhs);
 int fl_refine(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32, int64_t n,
               const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
               double* resid_out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* fl_refine with factors of the same operator written in another frame: A64
+ * acts on n = 3P point-major rows (row 3p + a is component a of point p), and
+ * LU32 factors the FP32 copy of A' with A64 ~ R A' R^T, R = blockdiag of the
+ * 3x3 frame (host, 9 doubles, row-major, lab = R body) over the P points.
+ * Each correction is  x <- x + s R LU32^{-1} (R^T r / s).  Used by the free
+ * swimmer to reuse the factors of its body-frame matrix at every time step
+ * with the same body shape (src/fluidlets/swimming.py:90-101 rotates that
+ * matrix into the lab frame); the residual is always taken against the exact
+ * A64, so the answer does not depend on the reuse.  Workspace as fl_refine.
+ */
+int fl_refine_frame(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32, int64_t n,
+                    const double* frame, const double* b, double* x, int64_t nrhs, int64_t ldb, double tol,
+                    int max_iter, int* iters_out, double* resid_out, void* workspace, size_t workspace_bytes,
+                    void* stream);
 
 /* ------------------------------------------------------------------ GMRES
  * Restarted GMRES(restart) for  sum_j G(x_i, x_j) f_j / mu_re = U_i  with the

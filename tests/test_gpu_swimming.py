@@ -129,3 +129,73 @@ def test_rk2_is_second_order():
         errs.append(np.abs(tr.B[-1][:, 0] - [np.cos(angle), np.sin(angle), 0]).max())
     assert xerrs[0] < 5e-3 and 3.0 < xerrs[0] / xerrs[1] < 5.0
     assert 3.0 < errs[0] / errs[1] < 5.0
+
+
+def _stretched_frame(seed, delta):
+    Q = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))[0]
+    Q *= np.sign(np.linalg.det(Q))
+    return Q @ np.diag([1 + delta, 1 + delta, 1 + 2 * delta])
+
+
+@pytest.mark.parametrize("delta", [0.0, 1e-5])
+def test_refine_with_body_frame_factors(delta):
+    """fl_refine_frame: FP32 factors of the body-frame matrix, FP64 residuals
+    against the exact lab-frame matrix of the rotated (and slightly stretched)
+    points; the answer is the lab-frame dense solve to 1e-12."""
+    import torch
+    from fluidlets_amd import linalg
+    from fluidlets_amd.core.assemble import assemble_nearest_device, nearest_groups
+    from fluidlets_amd.swimming import _frame_rotation
+    print(f"case: refine_frame delta={delta}", flush=True)
+    sh = biflagellate(n_body=200, n_flag=60)(0.2)
+    N = len(sh.x)
+    nn = np.arange(N)
+    B = _stretched_frame(3, delta)
+    x_lab = sh.x @ B.T
+    K_lab = O.assemble_nearest("stokeslet", x_lab, x_lab, nn, N, 0.02)
+    order, start = nearest_groups(nn, N)
+    dev = torch.device("cuda")
+    eps_t = torch.tensor([0.02], dtype=torch.float64, device=dev)
+    soa = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+    o_t, s_t = torch.from_numpy(order).to(dev), torch.from_numpy(start).to(dev)
+    _, Kb32 = assemble_nearest_device(fl.Stokeslet3D(), soa(sh.x), soa(sh.x), eps_t, 0, o_t, s_t, N, fp32=True,
+                                      fp64=False)
+    K64, _ = assemble_nearest_device(fl.Stokeslet3D(), soa(x_lab), soa(x_lab), eps_t, 0, o_t, s_t, N)
+    assert rel_err(K64.cpu().numpy(), K_lab) < 1e-14
+    F = linalg.getrf(Kb32)
+    b = np.random.default_rng(5).normal(size=(3 * N, 7))
+    x, iters, res = linalg.refine(K64, F, torch.from_numpy(b).to(dev), tol=1e-13, max_iter=30,
+                                  frame=_frame_rotation(B))
+    ref = np.linalg.solve(K_lab, b)
+    assert rel_err(x.cpu().numpy(), ref) < 1e-12 and res.max() <= 1e-13
+    assert iters <= 6
+    # without the rotation the body-frame factors do not precondition the lab system
+    if delta == 0.0:
+        with pytest.raises(RuntimeError):
+            linalg.refine(K64, F, torch.from_numpy(b).to(dev), tol=1e-13, max_iter=4)
+
+
+def test_factor_reuse_over_a_periodic_beat():
+    """swim_rk2 ("mixed", Stokeslet3D) with dt = T/4: after the first beat the
+    body shapes repeat and their factors are reused; the trajectory equals
+    the one that factors every step to 1e-11, and a step's result is the
+    same whether its factors came from the cache or were just made."""
+    from fluidlets_amd.cache import FactorCache
+    from fluidlets_amd.swimming import swimming_problem
+    print("case: swimmer factor reuse", flush=True)
+    shape_at = biflagellate(n_body=200, n_flag=60)
+    cache = FactorCache(1 << 30)
+    tr = swim_rk2(shape_at, 0.0, 0.25, 8, 0.02, precision="mixed", factor_cache=cache)
+    assert cache.misses == 8 and cache.hits == 8
+    tr0 = swim_rk2(shape_at, 0.0, 0.25, 8, 0.02, precision="mixed", reuse_factors=False)
+    assert np.abs(tr.x0 - tr0.x0).max() < 1e-11 and np.abs(tr.B - tr0.B).max() < 1e-11
+    B = _stretched_frame(8, 1e-6)
+    x0 = np.array([0.3, -0.1, 0.2])
+    cold = swimming_problem(shape_at(0.25), x0, B, 0.02, precision="mixed", factor_cache=FactorCache(1 << 30))
+    hits = cache.hits
+    warm = swimming_problem(shape_at(0.25), x0, B, 0.02, precision="mixed", factor_cache=cache)
+    assert cache.hits == hits + 1
+    assert np.array_equal(cold.U, warm.U) and np.array_equal(cold.forces, warm.forces)
+    plain = swimming_problem(shape_at(0.25), x0, B, 0.02, precision="fp64")
+    assert rel_err(warm.U, plain.U) < 1e-10 and rel_err(warm.Omega, plain.Omega) < 1e-10
+    assert rel_err(warm.forces, plain.forces) < 1e-10
