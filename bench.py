@@ -147,7 +147,7 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
             A = K64.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            F = linalg.getrf(A)
+            F = linalg.getrf(A, refill=lambda X: X.copy_(K64))
             x64 = rhs.clone()
             linalg.getrs(F, x64)
             torch.cuda.synchronize()
@@ -156,6 +156,7 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
             del A
         row["fp64_lu_s"] = best64
         row["fp64_general_panels"] = F.general_panels
+        row["fp64_lu_async"] = bool(F.extra.get("async", False))
         x64h = x64.cpu().numpy()
         del F
         # mixed: FP32 MFMA LU + FP64 refinement (the FP32 copy is refactored each repeat)
@@ -164,7 +165,7 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
             A32 = K32.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            F32 = linalg.getrf(A32)
+            F32 = linalg.getrf(A32, refill=lambda X: X.copy_(K32))
             xm, steps, res = linalg.refine(K64, F32, rhs, tol=1e-10)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0

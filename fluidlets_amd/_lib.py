@@ -39,6 +39,10 @@ _SIGNATURES = {
                           c_size_t, c_void_p]),
     "fl_dgetrf": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, POINTER(c_int), POINTER(c_int), c_void_p,
                           c_size_t, c_void_p]),
+    "fl_sgetrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                c_void_p]),
+    "fl_dgetrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                c_void_p]),
     "fl_getrs_workspace_size": (c_size_t, [c_int64, c_int64, c_int]),
     "fl_sgetrs": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_size_t,
                           c_void_p]),

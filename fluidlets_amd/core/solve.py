@@ -53,8 +53,11 @@ def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-
     """Dense solve of K x = rhs (K assembled for mu = 1) on the device.
     rhs: (3N, q) float64 device tensor.  Returns (x, refinement_steps, factor)."""
     if precision == "fp64":
-        K64, _ = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=False, stream=stream)
-        F = linalg.getrf(K64, stream=stream)
+        def assemble64():
+            return assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=False, stream=stream)[0]
+
+        K64 = assemble64()
+        F = linalg.getrf(K64, stream=stream, refill=lambda A: A.copy_(assemble64()))
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
         x = rhs.clone()
@@ -62,7 +65,7 @@ def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-
         return x, 0, F
     if precision == "mixed":
         K64, K32 = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=True, stream=stream)
-        F = linalg.getrf(K32, stream=stream)
+        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64))
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
         x, steps, _ = linalg.refine(K64, F, rhs, tol=tol, max_iter=max_iter, stream=stream)

@@ -68,6 +68,10 @@ int sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
            hipStream_t st);
 int dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
            hipStream_t st);
+int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
+                 size_t wsb, hipStream_t st);
+int dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
+                 size_t wsb, hipStream_t st);
 int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb, void* ws,
            size_t wsb, hipStream_t st);
 int dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb, void* ws,
@@ -190,6 +194,20 @@ int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* 
 {
     clear_error();
     return dgetrf(A, n, lda, ipiv, aux, info, fallback_panels, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* workspace,
+                    size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return sgetrf_async(A, n, lda, ipiv, aux, status, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                    void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return dgetrf_async(A, n, lda, ipiv, aux, status, workspace, workspace_bytes, as_stream(stream));
 }
 
 size_t fl_getrs_workspace_size(int64_t n, int64_t nrhs, int precision)

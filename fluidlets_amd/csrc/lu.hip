@@ -1075,13 +1075,41 @@ size_t getrf_ws_bytes(int64_t n)
     return align_up((size_t)n * NB * sizeof(T), 256) + 256;
 }
 
+// End of the asynchronous factorization: status = 0 when no multiplier
+// exceeded 1 + 2^-20 (then the no-interchange pivots are partial pivoting's
+// and the factors are those of the verified path), 1 otherwise; the row
+// permutation is the identity.
+__global__ void spec_finish_kernel(const unsigned int* lmax_bits, unsigned int* status, int32_t* perm, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[i] = (int32_t)i;
+    if (i == 0) {
+        const float lmax = __uint_as_float(*lmax_bits);
+        status[0] = lmax <= 1.0f + 1.0f / (1 << 20) ? 0u : 1u;
+    }
+}
+
+// Blocked right-looking LU.  With status == nullptr (verified path) every
+// sub-panel tries the three tiers in turn, reading the panel's largest
+// multiplier back on the host; with a status word (asynchronous path) every
+// sub-panel is factored without interchanges, the largest multiplier of the
+// whole matrix is folded into one device word, and nothing is read back: the
+// call only enqueues work, and *status says afterwards whether the result is
+// valid (else A must be refactored from a copy on the verified path).
 template <typename T>
 int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info_host, int* fallback_panels,
-               void* ws, size_t ws_bytes, hipStream_t st)
+               void* ws, size_t ws_bytes, hipStream_t st, unsigned int* status = nullptr)
 {
+    const bool spec = status != nullptr;
     FL_CHECK(n >= 0, "negative n");
     FL_CHECK(lda >= n, "lda < n");
-    if (n == 0) return FL_OK;
+    if (n == 0) {
+        if (spec) {
+            hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, status, (int64_t)1);
+            FL_LAUNCHED();
+        }
+        return FL_OK;
+    }
     FL_CHECK(A && ipiv && aux, "null pointer argument");
     if (ws_bytes < getrf_ws_bytes<T>(n) || !ws)
         return set_error(FL_EWORKSPACE, "getrf needs %zu workspace bytes", getrf_ws_bytes<T>(n));
@@ -1090,10 +1118,15 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     unsigned int* flag = reinterpret_cast<unsigned int*>(tail);      // largest |multiplier| of a panel
     int32_t* info_blk = reinterpret_cast<int32_t*>(tail + 4);        // zero pivot found by tier 1
     int32_t* info_d = reinterpret_cast<int32_t*>(tail + 64);
-    FL_HIP(hipMemsetAsync(info_d, 0, sizeof(int32_t), st));
+    if (spec) {
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, flag, (int64_t)1);
+        FL_LAUNCHED();
+    } else {
+        FL_HIP(hipMemsetAsync(info_d, 0, sizeof(int32_t), st));
+    }
     int fallbacks = 0;
     auto swap_cols = [&](int64_t k0, int nb, int64_t c0, int64_t c1) -> int {
-        if (c1 <= c0) return FL_OK;
+        if (c1 <= c0 || spec) return FL_OK;   // the asynchronous path makes no interchanges
         hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)div_up(c1 - c0, 256)), dim3(256), 0, st, A, lda, k0, nb,
                            (const int32_t*)ipiv, c0, c1);
         FL_LAUNCHED();
@@ -1110,13 +1143,23 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             T* Akk = A + ks * lda + ks;
             T* invL = FactorAux<T>::invL(aux, n, p);
             T* invU = FactorAux<T>::invU(aux, n, p);
+            if (spec) {
+                hipLaunchKernelGGL(diag_nopiv_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag, invL,
+                                   invU);
+                FL_LAUNCHED();
+                if (m > nb)
+                    if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0),
+                                               Akk + nb * lda, lda, flag, st))
+                        return s;
+            }
             // Tier 0: no interchanges; tier 1: interchanges inside the diagonal
             // block; tier 2: pivot search over every row below the diagonal.
             // A tier is accepted when no multiplier of the panel exceeds 1
             // (then its pivots are the ones partial pivoting chooses).
-            FL_HIP(hipMemcpy2DAsync(backup, nb * sizeof(T), Akk, lda * sizeof(T), nb * sizeof(T), m,
-                                    hipMemcpyDeviceToDevice, st));
-            for (int tier = 0;; ++tier) {
+            if (!spec)
+                FL_HIP(hipMemcpy2DAsync(backup, nb * sizeof(T), Akk, lda * sizeof(T), nb * sizeof(T), m,
+                                        hipMemcpyDeviceToDevice, st));
+            for (int tier = 0; !spec; ++tier) {
                 if (tier == 2) {
                     ++fallbacks;
                     for (int64_t c = ks; c < ks + nb; ++c) {
@@ -1202,6 +1245,12 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                                            lda, st))
                 return s;
         }
+    }
+    if (spec) {
+        hipLaunchKernelGGL(spec_finish_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, (const unsigned*)flag,
+                           status, FactorAux<T>::perm(aux), n);
+        FL_LAUNCHED();
+        return FL_OK;
     }
     // fold the interchanges into a row permutation (host; ipiv is n int32)
     std::vector<int32_t> piv(n), perm(n);
@@ -1411,6 +1460,18 @@ int dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* inf
            hipStream_t st)
 {
     return getrf_impl<double>(A, n, lda, ipiv, aux, info, fb, ws, wsb, st);
+}
+int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
+                 size_t wsb, hipStream_t st)
+{
+    FL_CHECK(status, "null status");
+    return getrf_impl<float>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status);
+}
+int dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
+                 size_t wsb, hipStream_t st)
+{
+    FL_CHECK(status, "null status");
+    return getrf_impl<double>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status);
 }
 int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb, void* ws,
            size_t wsb, hipStream_t st)

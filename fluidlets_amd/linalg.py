@@ -4,6 +4,7 @@ fl_refine).  Device memory is torch tensors; the arithmetic is the HIP library.
 from __future__ import annotations
 
 import ctypes
+from contextlib import nullcontext
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -36,8 +37,14 @@ class LUFactor:
         return int(self.lu.numel() * self.lu.element_size() + self.aux.numel())
 
 
-def getrf(A, overwrite: bool = True, stream=None) -> LUFactor:
-    """Factor an (n, n) float32/float64 device tensor in place (or a copy)."""
+def getrf(A, overwrite: bool = True, stream=None, refill=None) -> LUFactor:
+    """Factor an (n, n) float32/float64 device tensor in place (or a copy).
+
+    With ``refill`` (a callable that rewrites A's original values into the
+    tensor it is given) the factorization first runs asynchronously without
+    interchanges (fl_?getrf_async, one read-back for the whole matrix); if
+    that is not partial pivoting's answer, A is refilled and factored on the
+    verified path.  The result is the same either way."""
     t = D.torch()
     if A.dim() != 2 or A.shape[0] != A.shape[1]:
         raise ValueError("getrf needs a square matrix")
@@ -54,6 +61,16 @@ def getrf(A, overwrite: bool = True, stream=None) -> LUFactor:
     aux = t.empty((max(int(L.fl_getrf_aux_size(n, prec)), 1),), dtype=t.uint8, device=A.device)
     nbytes = int(L.fl_getrf_workspace_size(n, prec))
     ws = D.workspace(nbytes, stream)
+    if refill is not None:
+        status = t.empty((1,), dtype=t.int32, device=A.device)
+        fa = L.fl_dgetrf_async if prec == FL_FP64 else L.fl_sgetrf_async
+        check(fa(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), D.ptr(status), D.ptr(ws), nbytes, D.stream_handle(stream)),
+              "getrf_async")
+        with t.cuda.stream(stream) if stream is not None else nullcontext():
+            ok = int(status.cpu().item()) == 0     # waits for the factorization on its stream
+            if ok:
+                return LUFactor(A, ipiv, aux, n, 0, 0, {"async": True})
+            refill(A)
     info = ctypes.c_int(0)
     fb = ctypes.c_int(0)
     fn = L.fl_dgetrf if prec == FL_FP64 else L.fl_sgetrf

@@ -145,7 +145,9 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
     x0c = (ctypes.c_double * 3)(*x0.tolist())
     check(L.fl_swim_rhs(N, D.ptr(x_soa), x0c, D.ptr(v_soa), D.ptr(R), D.stream_handle(stream)), "fl_swim_rhs")
     if precision == "fp64":
-        F = linalg.getrf(K64, stream=stream)
+        F = linalg.getrf(K64, stream=stream,
+                         refill=lambda A: assemble_nearest_device(kernel, x_soa, X_soa, eps_t, stride, order_t, start_t,
+                                                                  N, mu=mu, K64=A, stream=stream))
         if F.info > 0:
             raise np.linalg.LinAlgError("Singular matrix")
         Y = linalg.getrs(F, R, stream=stream)
@@ -153,10 +155,13 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
         key = _body_key(kernel, shape, nearest, e, mu)
         item = factor_cache.get(key)
         if item is None:
-            _, Kb32 = assemble_nearest_device(kernel, D.soa(np.asarray(shape.x, float), dev),
-                                              D.soa(np.asarray(shape.X, float), dev), eps_t, stride, order_t,
-                                              start_t, N, mu=mu, fp32=True, fp64=False, stream=stream)
-            F = linalg.getrf(Kb32, stream=stream)
+            xb, Xb = D.soa(np.asarray(shape.x, float), dev), D.soa(np.asarray(shape.X, float), dev)
+
+            def body32():
+                return assemble_nearest_device(kernel, xb, Xb, eps_t, stride, order_t, start_t, N, mu=mu, fp32=True,
+                                               fp64=False, stream=stream)[1]
+
+            F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()))
             if F.info > 0:
                 raise np.linalg.LinAlgError("Singular matrix")
             factor_cache.put(key, (F,), F.nbytes())
@@ -164,7 +169,7 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
             F = item[0]
         Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream, frame=_frame_rotation(B))
     elif precision == "mixed":
-        F = linalg.getrf(K32, stream=stream)
+        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64))
         if F.info > 0:
             raise np.linalg.LinAlgError("Singular matrix")
         Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream)

@@ -170,6 +170,24 @@ int fl_sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* i
 int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* general_panels,
               void* workspace, size_t workspace_bytes, void* stream);
 
+/* Asynchronous form of fl_?getrf for callers that keep the matrix's source
+ * (the FP64 matrix, or the assembly that made it): it only enqueues work and
+ * never waits on the stream.  Every diagonal block is factored without
+ * interchanges and the largest multiplier of the whole matrix is checked on
+ * the device.  When the stream has run the call, *status (one device word)
+ * is 0 if no multiplier exceeded 1 + 2^-20: the pivots are then those partial
+ * pivoting chooses, and A, ipiv and aux are exactly what fl_?getrf writes.
+ * Otherwise *status is 1 and A holds unusable values: refill it from its
+ * source and call fl_?getrf.  Same workspace as fl_?getrf.  Replaces, like
+ * fl_?getrf, the LU inside scipy.linalg.solve (src/fluidlets/core/solve.py:66)
+ * and numpy.linalg.solve (src/fluidlets/swimming.py:101) for matrices whose
+ * partial pivoting makes no interchanges, as the kernel matrices here do.
+ */
+int fl_sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                    void* workspace, size_t workspace_bytes, void* stream);
+int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
 /* C = alpha A B + beta C, row-major (M x K, K x N, M x N), on MFMA with FP32
  * (fl_sgemm) or FP64 (fl_dgemm) accumulation; C is not read when beta == 0.
  * The LU's update kernel.  It has no counterpart call in the reference (there

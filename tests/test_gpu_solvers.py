@@ -82,6 +82,51 @@ def test_resistance_matrix_takes_the_block_pivot_path(name, kernel):
     assert rel_err(Kg, K) < 1e-13
 
 
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_async_getrf_equals_the_verified_path(dtype):
+    """fl_?getrf_async: on a resistance matrix (no interchanges under partial
+    pivoting) its factors, pivots and permutation are bit-for-bit those of the
+    verified path and LAPACK's pivots; on random and singular matrices it
+    reports failure and the refilled verified factorization is used."""
+    import torch
+    from fluidlets_amd import linalg
+    print(f"case: async getrf {dtype}", flush=True)
+    n = 900
+    X = fibonacci_sphere(n // 3, 1.0, (0, 0, 1.5))
+    K = O.assemble("wall", X, X, 0.5 * np.sqrt(4 * np.pi / (n // 3))).astype(dtype)
+    ref = linalg.getrf(dev(K.copy()))
+    refills = []
+    Fa = linalg.getrf(dev(K.copy()), refill=lambda A: refills.append(1))
+    assert Fa.extra.get("async") and not refills
+    assert torch.equal(Fa.lu, ref.lu) and torch.equal(Fa.ipiv, ref.ipiv)
+    # aux: the permutation (n int32, padded to 256 bytes) and the inverted diagonal blocks
+    pb = -(-4 * n // 256) * 256
+    a, r = Fa.aux.cpu().numpy(), ref.aux.cpu().numpy()
+    assert np.array_equal(a[:4 * n], r[:4 * n])
+    assert np.array_equal(a[:4 * n].view(np.int32), np.arange(n))
+    diff = np.flatnonzero(a[pb:] != r[pb:])
+    assert diff.size == 0, f"inverse blocks differ from byte {pb + diff[0]} of {a.size}"
+    assert np.array_equal(Fa.ipiv.cpu().numpy(), np.arange(n))
+    b = np.random.default_rng(2).standard_normal((n, 3)).astype(dtype)
+    x1, x2 = dev(b.copy()), dev(b.copy())
+    linalg.getrs(Fa, x1)
+    linalg.getrs(ref, x2)
+    assert torch.equal(x1, x2)
+    # a random matrix needs interchanges: the async attempt fails, A is refilled
+    R = np.random.default_rng(3).standard_normal((300, 300)).astype(dtype)
+    src = dev(R.copy())
+    Fr = linalg.getrf(dev(R.copy()), refill=lambda A: A.copy_(src))
+    assert not Fr.extra.get("async")
+    Fv = linalg.getrf(dev(R.copy()))
+    assert torch.equal(Fr.lu, Fv.lu) and torch.equal(Fr.ipiv, Fv.ipiv)
+    # an exactly zero pivot also sends it to the verified path, which reports info
+    Z = np.eye(200, dtype=dtype)
+    Z[150, 150] = 0.0
+    zsrc = dev(Z.copy())
+    Fz = linalg.getrf(dev(Z.copy()), refill=lambda A: A.copy_(zsrc))
+    assert not Fz.extra.get("async") and Fz.info == 151
+
+
 def test_singular_matrix_reports_info():
     from fluidlets_amd import linalg
     A = np.eye(200)
