@@ -23,6 +23,7 @@
 // (src/fluidlets/core/solve.py:66) and numpy.linalg.solve
 // (src/fluidlets/swimming.py:101).
 #include "fl_internal.h"
+#include "mfma.h"
 
 #include <rocblas/rocblas.h>
 
@@ -172,11 +173,10 @@ struct Vec16<double> {
 };
 
 // Column j of the inverse of the 64 x 64 unit lower triangle of F (pitch lf),
-// by forward substitution; written to X[i * lx + j].
+// by forward substitution, into x[i] (row i).
 template <typename T>
-__device__ __forceinline__ void tri_inverse_lower(const T* F, int lf, int j, T* X, int lx)
+__device__ __forceinline__ void tri_inverse_lower(const T* F, int lf, int j, T (&x)[64])
 {
-    T x[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) {
         T s0 = T(0), s1 = T(0);
@@ -190,16 +190,13 @@ __device__ __forceinline__ void tri_inverse_lower(const T* F, int lf, int j, T* 
         if (i & 1) s0 = fma(F[i * lf + i - 1], x[i - 1], s0);
         x[i] = i < j ? T(0) : (i == j ? T(1) : -(s0 + s1));
     }
-#pragma unroll
-    for (int i = 0; i < 64; ++i) X[i * lx + j] = x[i];
 }
 
 // Column j of the inverse of the 64 x 64 upper triangle of F, by backward
 // substitution; x[q] holds row 63 - q.
 template <typename T>
-__device__ __forceinline__ void tri_inverse_upper(const T* F, int lf, int j, T* X, int lx)
+__device__ __forceinline__ void tri_inverse_upper(const T* F, int lf, int j, T (&x)[64])
 {
-    T x[64];
 #pragma unroll
     for (int q = 0; q < 64; ++q) {
         const int i = 63 - q;
@@ -215,70 +212,121 @@ __device__ __forceinline__ void tri_inverse_upper(const T* F, int lf, int j, T* 
         const T rd = T(1) / F[i * lf + i];
         x[q] = i > j ? T(0) : (i == j ? rd : -(s0 + s1) * rd);
     }
+}
+
+// The off-diagonal blocks of inv(L) and inv(U) on MFMA, from the packed
+// diagonal inverses in S (block dd: inv(L_dd) strictly below its diagonal,
+// 1 implied on it; inv(U_dd) on and above it):
+//   phase 1: T1 = L21 inv(L11), T2 = U12 inv(U22), written over L21 and U12;
+//   phase 2: inv(L)21 = -inv(L22) T1, inv(U)12 = -inv(U11) T2, to invL, invU.
+// Waves 0, 1 form the L products, waves 2, 3 the U products, each half of
+// the 64 x 64 output in MFMA blocks (32 x 32 FP32, 16 x 16 FP64, K = 64).
+template <typename T>
+__device__ __forceinline__ void block_inverse_products(T* S, T* invL, T* invU, int wave, int lane)
+{
+    using M = mfma::Mma<T>;
+    constexpr int H = NB / 2, LS = NB + 1, TB = M::B, TPE = H / TB, TW = TPE * TPE / 2;
+    const bool lside = wave < 2;
+    const int t0 = (wave & 1) * TW;
+    typename M::acc_t acc[TW];
 #pragma unroll
-    for (int q = 0; q < 64; ++q) X[(63 - q) * lx + j] = x[q];
+    for (int u = 0; u < TW; ++u) {
+        const int t = t0 + u, ar = (t / TPE) * TB + M::a_row(lane), bc = (t % TPE) * TB + M::d_col(lane);
+        acc[u] = M::zero();
+#pragma unroll 4
+        for (int k0 = 0; k0 < H; k0 += M::KS) {
+            const int k = k0 + M::a_k(lane);
+            T a, b;
+            if (lside) {
+                a = S[(H + ar) * LS + k];                                      // L21[ar][k]
+                b = k > bc ? S[k * LS + bc] : (k == bc ? T(1) : T(0));         // inv(L11)[k][bc]
+            } else {
+                a = S[ar * LS + H + k];                                        // U12[ar][k]
+                b = k <= bc ? S[(H + k) * LS + H + bc] : T(0);                 // inv(U22)[k][bc]
+            }
+            acc[u] = M::mma(a, b, acc[u]);
+        }
+    }
+    __syncthreads();  // every read of L21 and U12 is done
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+        const int t = t0 + u, r0 = (t / TPE) * TB, col = (t % TPE) * TB + M::d_col(lane);
+#pragma unroll
+        for (int i = 0; i < M::NACC; ++i) {
+            const int row = r0 + M::d_row(lane, i);
+            if (lside) S[(H + row) * LS + col] = acc[u][i];
+            else S[row * LS + H + col] = acc[u][i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+        const int t = t0 + u, ar = (t / TPE) * TB + M::a_row(lane), bc = (t % TPE) * TB + M::d_col(lane);
+        acc[u] = M::zero();
+#pragma unroll 4
+        for (int k0 = 0; k0 < H; k0 += M::KS) {
+            const int k = k0 + M::a_k(lane);
+            T a, b;
+            if (lside) {
+                a = k < ar ? S[(H + ar) * LS + H + k] : (k == ar ? T(1) : T(0));  // inv(L22)[ar][k]
+                b = S[(H + k) * LS + bc];                                      // T1[k][bc]
+            } else {
+                a = k >= ar ? S[ar * LS + k] : T(0);                           // inv(U11)[ar][k]
+                b = S[k * LS + H + bc];                                        // T2[k][bc]
+            }
+            acc[u] = M::mma(a, b, acc[u]);
+        }
+        const int r0 = (t / TPE) * TB, col = (t % TPE) * TB + M::d_col(lane);
+#pragma unroll
+        for (int i = 0; i < M::NACC; ++i) {
+            const int row = r0 + M::d_row(lane, i);
+            if (lside) invL[(H + row) * NB + col] = -acc[u][i];
+            else invU[row * NB + H + col] = -acc[u][i];
+        }
+    }
 }
 
 // inv(L) and inv(U) of the factored NB x NB block in S (pitch NB + 1), as
-// NB x NB row-major arrays (ld NB, zero outside the triangle): the four 64 x 64
-// diagonal triangles by substitution (waves 0, 1: L11, L22; waves 2, 3: U11,
-// U22), then inv(L)21 = -inv(L22) L21 inv(L11) and inv(U)12 = -inv(U11) U12
-// inv(U22).  Overwrites S's L21 and U12 blocks.  All 256 lanes call it.
+// NB x NB row-major arrays (ld NB, zero outside the triangle).  The four
+// 64 x 64 diagonal triangles are inverted by substitution (waves 0, 1: L11,
+// L22; waves 2, 3: U11, U22; one lane per column, in registers) and packed
+// back into S's diagonal blocks once every wave has read its triangle: block
+// dd holds inv(L_dd) strictly below its diagonal and inv(U_dd) on and above
+// it.  The off-diagonal blocks follow on MFMA (block_inverse_products),
+// overwriting S's L21 and U12.  All 256 lanes call it.
 template <typename T>
 __device__ __forceinline__ void block_inverses(T* S, T* invL, T* invU, int tid)
 {
     constexpr int H = NB / 2;
     constexpr int LS = NB + 1;
     const int lane = tid & 63, wave = tid >> 6;
-    if (wave < 2) tri_inverse_lower<T>(S + (wave * 64) * (LS + 1), LS, lane, invL + (wave * 64) * (NB + 1), NB);
-    else tri_inverse_upper<T>(S + ((wave - 2) * 64) * (LS + 1), LS, lane, invU + ((wave - 2) * 64) * (NB + 1), NB);
+    const int d = (wave & 1) * H;  // this wave's diagonal block
+    T x[64];
+    if (wave < 2) tri_inverse_lower<T>(S + d * (LS + 1), LS, lane, x);
+    else tri_inverse_upper<T>(S + d * (LS + 1), LS, lane, x);
+    __syncthreads();  // every triangle of S has been read
+    if (wave < 2) {
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+            if (i > lane) S[(d + i) * LS + d + lane] = x[i];
+            invL[(d + i) * NB + d + lane] = x[i];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 64; ++q) {
+            const int i = 63 - q;
+            if (i <= lane) S[(d + i) * LS + d + lane] = x[q];
+            invU[(d + i) * NB + d + lane] = x[q];
+        }
+    }
     if (wave == 0)
 #pragma unroll
         for (int i = 0; i < 64; ++i) invL[i * NB + 64 + lane] = T(0);
     if (wave == 3)
 #pragma unroll
         for (int i = 0; i < 64; ++i) invU[(64 + i) * NB + lane] = T(0);
-    __syncthreads();  // the diagonal inverse blocks are read back by the other waves
-    {
-        const int i = tid >> 2, jb = (tid & 3) * 16;
-        T tl[16], tu[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
-        for (int m = 0; m < H; ++m) {
-            const T l21 = S[(H + i) * LS + m];
-            const T u12 = S[i * LS + H + m];
-            const T* xl1 = invL + m * NB + jb;
-            const T* xu2 = invU + (H + m) * NB + H + jb;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                tl[q] = fma(l21, xl1[q], tl[q]);
-                tu[q] = fma(u12, xu2[q], tu[q]);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            S[(H + i) * LS + jb + q] = tl[q];  // L21 inv(L11) in place of L21
-            S[i * LS + H + jb + q] = tu[q];    // U12 inv(U22) in place of U12
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 16; ++q) tl[q] = tu[q] = T(0);
-        for (int m = 0; m < H; ++m) {
-            const T xl2 = invL[(H + i) * NB + H + m];
-            const T xu1 = invU[i * NB + m];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                tl[q] = fma(xl2, S[(H + m) * LS + jb + q], tl[q]);
-                tu[q] = fma(xu1, S[m * LS + H + jb + q], tu[q]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            invL[(H + i) * NB + jb + q] = -tl[q];
-            invU[i * NB + H + jb + q] = -tu[q];
-        }
-    }
+    __syncthreads();  // the packed inverses are read by every wave
+    block_inverse_products<T>(S, invL, invU, wave, lane);
 }
 
 template <typename T>
@@ -411,6 +459,42 @@ __global__ __launch_bounds__(256) void diag_lu_kernel(T* A, int64_t lda, int nb,
     block_inverses<T>(S, invL, invU, tid);
 }
 
+// S (pitch NB + 1) = the nb x nb block at A, padded to NB x NB with the
+// identity.  A full block with 16-byte aligned rows is read as 16-byte
+// vectors, in two rounds whose loads are all issued before any LDS store.
+template <typename T>
+__device__ __forceinline__ void load_diag_block(const T* A, int64_t lda, int nb, T* S, int tid)
+{
+    constexpr int LS = NB + 1;
+    constexpr int V = Vec16<T>::n;
+    constexpr int RV = NB / V;                 // vectors per row
+    constexpr int PER = NB * RV / 256 / 2;     // vectors per lane per round
+    using VT = typename Vec16<T>::type;
+    if (nb == NB && lda % V == 0 && ((uintptr_t)A % 16) == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            VT buf[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int idx = (h * PER + k) * 256 + tid;
+                buf[k] = *reinterpret_cast<const VT*>(A + (int64_t)(idx / RV) * lda + (idx % RV) * V);
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int idx = (h * PER + k) * 256 + tid;
+                const T* e = reinterpret_cast<const T*>(&buf[k]);
+#pragma unroll
+                for (int q = 0; q < V; ++q) S[(idx / RV) * LS + (idx % RV) * V + q] = e[q];
+            }
+        }
+    } else {
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            const int r = idx / NB, c = idx % NB;
+            S[r * LS + c] = (r < nb && c < nb) ? A[r * lda + c] : (r == c ? T(1) : T(0));
+        }
+    }
+}
+
 // ---------------------------------------------------------------- diagonal block without interchanges
 // The resistance matrices never need a row interchange (every pivot is the
 // largest entry left in its column), and then GEPP is LU without
@@ -435,10 +519,7 @@ __global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int 
     constexpr int LS = NB + 1;
     __shared__ T S[NB * LS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx / NB, c = idx % NB;
-        S[r * LS + c] = (r < nb && c < nb) ? A[r * lda + c] : (r == c ? T(1) : T(0));
-    }
+    load_diag_block<T>(A, lda, nb, S, tid);
     for (int c = tid; c < nb; c += 256) ipiv[k0 + c] = (int32_t)(k0 + c);
     __syncthreads();
     T lmax = T(0);
