@@ -768,9 +768,13 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
     const int64_t r0 = p * NB;
     const int nbp = (int)min((int64_t)NB, n - r0);
     bool gave_up = false;
-    T part[NR];
+    T part[NR], bq[NR];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) part[q] = T(0);
+    for (int q = 0; q < NR; ++q) {
+        part[q] = T(0);
+        // this row's right-hand side, read now: it is not produced by the chain
+        bq[q] = (hf == 0 && r < nbp && q < nrhs) ? b[(r0 + r) * nrhs + q] : T(0);
+    }
     const int64_t nsteps = LOWER ? p : nblk - 1 - p;
     // this thread's half row of T_pj: all H entries loaded at once (issued a
     // step ahead, so the loads overlap the previous block's products)
@@ -864,7 +868,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
     if (hf == 0)
 #pragma unroll
         for (int q = 0; q < NR; ++q)
-            xs[r * NR + q] = (r < nbp && q < nrhs) ? b[(r0 + r) * nrhs + q] - part[q] : T(0);
+            xs[r * NR + q] = (r < nbp && q < nrhs) ? bq[q] - part[q] : T(0);
     {
         VT* dst = reinterpret_cast<VT*>(&Ts[r * TP + hf * H]);
 #pragma unroll
