@@ -52,7 +52,7 @@ def eps_arg(eps, n: int, dev=None):
     e = np.asarray(eps, dtype=np.float64)
     if e.ndim == 0 or e.size == 1:
         return t.full((1,), float(e.reshape(-1)[0]) if e.size else 0.0, dtype=t.float64, device=dev or device()), 0
-    e = np.ascontiguousarray(np.broadcast_to(e, (n,)))
+    e = np.array(np.broadcast_to(e, (n,)), dtype=np.float64)   # a writable copy for torch
     return t.from_numpy(e).to(dev or device()), 1
 
 
