@@ -734,18 +734,65 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 //   acc = b_p - sum_j T_pj x_j     (j < p forward, j > p backward),
 //   x_p = inv(T_pp) acc            (the diagonal block's inverse kept in aux).
 // Workgroups take tickets in dispatch order (ticket t solves block t forward,
-// nblk - 1 - t backward) and wait on a block's flag before reading it, so a
-// workgroup waits only on workgroups dispatched before it.  x_p is published
-// by a release store of flag[p] after every wave's stores have completed.  A
-// wait is bounded (CH_SPIN polls): a workgroup that gives up sets sync[1] and
-// writes NaN for its block, then publishes its flag, so every workgroup ends.
-// sync = [ticket, error, flag[0 .. nblk)], zeroed before the launch.
+// nblk - 1 - t backward), so a workgroup waits only on workgroups dispatched
+// before it.  The output x is filled with a signalling-NaN sentinel before
+// the launch (arithmetic never produces one); a workgroup publishes x_p by
+// storing it, waiting for its stores and issuing one agent-scope release
+// (the L2 write-back that makes them visible to the other XCDs), and a
+// reader polls the entries of x_j it needs with coherent loads until none is
+// the sentinel: the data is the flag, so a block step costs one round trip
+// through memory.  A wait is bounded (CH_SPIN polls): a workgroup that gives
+// up sets sync[1] and writes quiet NaN for its block, so every workgroup ends.
+// sync = [ticket, error], zeroed before the launch.
 constexpr int CH_SPIN = 1 << 22;
 
 __global__ void zero_u32_kernel(unsigned* p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+// The chain's "not yet written" value: a signalling NaN of each width.
+template <typename T>
+struct ChainBits;
+template <>
+struct ChainBits<float> {
+    using U = unsigned int;
+    static constexpr U sentinel = 0x7FA5A5A5u;
+};
+template <>
+struct ChainBits<double> {
+    using U = unsigned long long;
+    static constexpr U sentinel = 0x7FF4A5A5A5A5A5A5ull;
+};
+
+template <typename T>
+__global__ void fill_sentinel_kernel(T* x, int64_t n)
+{
+    using U = typename ChainBits<T>::U;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) reinterpret_cast<U*>(x)[i] = ChainBits<T>::sentinel;
+}
+
+// Bounded wait for one entry of x written by another workgroup.
+template <typename T>
+__device__ __forceinline__ T chain_read(const T* p, bool& gave_up)
+{
+    using U = typename ChainBits<T>::U;
+    const U* q = reinterpret_cast<const U*>(p);
+    U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (v == ChainBits<T>::sentinel) {
+        if (++spins > CH_SPIN) {
+            gave_up = true;
+            return T(NAN);
+        }
+        __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    T out;
+    __builtin_memcpy(&out, &v, sizeof(T));
+    return out;
 }
 
 template <typename T, bool LOWER, int NR>
@@ -759,7 +806,6 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
     __shared__ __attribute__((aligned(16))) T Ts[NB * TP];
     __shared__ __attribute__((aligned(16))) T xs[NB * NR];
     __shared__ int tk;
-    unsigned* flags = sync + 2;
     const int tid = threadIdx.x, r = tid >> 1, hf = tid & 1;
     const int64_t nblk = (n + NB - 1) / NB;
     if (tid == 0) tk = (int)atomicAdd(&sync[0], 1u);
@@ -816,21 +862,6 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
         const int64_t j = LOWER ? s : nblk - 1 - s;
         const int64_t c0 = j * NB;
         const int nbj = (int)min((int64_t)NB, n - c0);
-        if (tid == 0) {
-            // poll with relaxed loads (an acquire per poll would invalidate
-            // this XCD's L2 every time); one acquire fence once the flag is set
-            int spins = 0;
-            while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-                if (++spins > CH_SPIN) {
-                    atomicOr(&sync[1], 1u);
-                    gave_up = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        __syncthreads();
         {
             VT* dst = reinterpret_cast<VT*>(&Ts[r * TP + hf * H]);
 #pragma unroll
@@ -844,7 +875,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
         }
         for (int idx = tid; idx < NB * NR; idx += 256) {
             const int c = idx / NR, q = idx % NR;
-            xs[idx] = (c < nbj && q < nrhs) ? x[(c0 + c) * nrhs + q] : T(0);
+            xs[idx] = (c < nbj && q < nrhs) ? chain_read(&x[(c0 + c) * nrhs + q], gave_up) : T(0);
         }
         __syncthreads();
         if (s + 1 < nsteps) load_row(LOWER ? s + 1 : nblk - 2 - s);
@@ -900,19 +931,17 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) y[q] += dpp_mov<0xB1>(y[q]);
-    if (tid == 0 && gave_up) atomicOr(&sync[1], 2u);
-    __shared__ int bad;
-    if (tid == 0) bad = gave_up;
-    __syncthreads();
+    const int bad = __syncthreads_or(gave_up);
+    if (tid == 0 && bad) atomicOr(&sync[1], 1u);
     if (hf == 0 && r < nbp)
 #pragma unroll
         for (int q = 0; q < NR; ++q)
             if (q < nrhs) x[(r0 + r) * nrhs + q] = bad ? T(NAN) : y[q];
-    // every wave's stores complete (reach this XCD's L2), then one release
-    // store: its L2 write-back publishes all of them to the other XCDs
+    // every wave's stores complete (reach this XCD's L2), then one release:
+    // its L2 write-back publishes all of them to the other XCDs
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[p], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 // dst[i][:] = scale_c * src[perm[i]][:] with a type conversion (gather of the
@@ -1386,7 +1415,13 @@ int solve_permuted(const T* LU, int64_t n, int64_t lda, void* aux, T* y, T* tmp,
     FL_LAUNCHED();
     const T* invs = FactorAux<T>::invL(aux, n, 0);
     const int64_t half = div_up(n, NB) + 2;
+    const int64_t len = n * nrhs;
+    const dim3 fg((unsigned)div_up(len, 256));
+    hipLaunchKernelGGL(fill_sentinel_kernel<T>, fg, dim3(256), 0, st, tmp, len);
+    FL_LAUNCHED();
     launch_chain<T, true>(LU, lda, n, invs, (const T*)y, tmp, nrhs, sync, st);
+    FL_LAUNCHED();
+    hipLaunchKernelGGL(fill_sentinel_kernel<T>, fg, dim3(256), 0, st, y, len);   // y was the forward sweep's input
     FL_LAUNCHED();
     launch_chain<T, false>(LU, lda, n, invs, (const T*)tmp, y, nrhs, sync + half, st);
     FL_LAUNCHED();
