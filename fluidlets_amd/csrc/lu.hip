@@ -1011,21 +1011,25 @@ __global__ __launch_bounds__(256) void copy_rows_kernel(const TS* src, TD* dst, 
 }
 
 // ---------------------------------------------------------------- refinement
-// r = b - A x (FP64), 8 rows per wave (each x entry loaded once for the 8
-// rows), lanes striding along the rows; x and r are n x nrhs (ld nrhs), b has ld ldb.
+// r = b - A x (FP64), 8 rows per workgroup: each x entry is loaded once for
+// the 8 rows, the four waves take interleaved quarters of the columns (lanes
+// striding along a row, so A is read in 256-lane coalesced runs), and the
+// partial sums are combined in a fixed order (wave 0 .. 3) through LDS.  At
+// n = 5,400 that is 675 workgroups, enough to fill the chip.  x and r are
+// n x nrhs (ld nrhs), b has ld ldb.
 constexpr int RES_ROWS = 8;
 __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t lda, int64_t n, const double* b,
                                                        int64_t ldb, const double* x, double* r, int nrhs)
 {
-    const int lane = threadIdx.x % 64;
-    const int64_t row0 = ((int64_t)blockIdx.x * 4 + threadIdx.x / 64) * RES_ROWS;
-    if (row0 >= n) return;
+    __shared__ double part[4][RES_ROWS * 8];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int64_t row0 = (int64_t)blockIdx.x * RES_ROWS;
     double acc[RES_ROWS][8];
 #pragma unroll
     for (int i = 0; i < RES_ROWS; ++i)
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[i][q] = 0.0;
-    for (int64_t c = lane; c < n; c += 64) {
+    for (int64_t c = threadIdx.x; c < n; c += 256) {
         double xv[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) xv[q] = q < nrhs ? x[c * nrhs + q] : 0.0;
@@ -1045,15 +1049,22 @@ __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t 
             for (int off = 32; off > 0; off /= 2) s += __shfl_xor(s, off);
             acc[i][q] = s;
         }
-    if (lane < RES_ROWS * nrhs) {
-        const int i = lane / nrhs, q = lane % nrhs;
+    if (lane < RES_ROWS * 8) {
+        const int i = lane / 8, q = lane % 8;
         double s = 0.0;
 #pragma unroll
         for (int ii = 0; ii < RES_ROWS; ++ii)
 #pragma unroll
             for (int qq = 0; qq < 8; ++qq)
                 if (ii == i && qq == q) s = acc[ii][qq];
-        if (row0 + i < n) r[(row0 + i) * nrhs + q] = b[(row0 + i) * ldb + q] - s;
+        part[wave][lane] = s;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < RES_ROWS * 8) {
+        const int i = t / 8, q = t % 8;
+        const double s = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+        if (q < nrhs && row0 + i < n) r[(row0 + i) * nrhs + q] = b[(row0 + i) * ldb + q] - s;
     }
 }
 
@@ -1502,26 +1513,34 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, bnorms);
     FL_LAUNCHED();
     FL_HIP(hipMemsetAsync(xc, 0, (size_t)n * nrhs * sizeof(double), st));
-    std::vector<double> hb(16), hr(16);
-    FL_HIP(hipMemcpyAsync(hb.data(), bnorms, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
+    // small = norms[16], scale[8], inv_scale[8], bnorms[16]: one read-back
+    // brings the residual's and the right-hand side's norms to the host
+    std::vector<double> hs(48);
+    const double* hr = hs.data();
+    const double* hb = hs.data() + 32;
     int it = 0;
     double worst = 0.0;
     for (;;) {
-        // r holds the current residual b - A x (x = 0 on entry)
-        hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, norms);
-        FL_LAUNCHED();
-        FL_HIP(hipMemcpyAsync(hr.data(), norms, 16 * sizeof(double), hipMemcpyDeviceToHost, st));
-        FL_HIP(hipStreamSynchronize(st));
-        worst = 0.0;
-        for (int c = 0; c < q; ++c) {
-            const double bn = std::sqrt(hb[c]);
-            const double rel = bn > 0 ? std::sqrt(hr[c]) / bn : std::sqrt(hr[c]);
-            if (resid_out) resid_out[c] = rel;
-            worst = std::max(worst, rel);
+        // r holds the current residual b - A x.  Before the first correction
+        // x = 0 and r = b, whose norms are already known (relative residual 1),
+        // so the host looks only from the first correction on.
+        if (it > 0 || max_iter <= 0) {
+            hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, norms);
+            FL_LAUNCHED();
+            FL_HIP(hipMemcpyAsync(hs.data(), small, 48 * sizeof(double), hipMemcpyDeviceToHost, st));
+            FL_HIP(hipStreamSynchronize(st));
+            worst = 0.0;
+            for (int c = 0; c < q; ++c) {
+                const double bn = std::sqrt(hb[c]);
+                const double rel = bn > 0 ? std::sqrt(hr[c]) / bn : std::sqrt(hr[c]);
+                if (resid_out) resid_out[c] = rel;
+                worst = std::max(worst, rel);
+            }
+            if ((it > 0 && worst <= tol) || it >= max_iter) break;
         }
-        if ((it > 0 && worst <= tol) || it >= max_iter) break;
         // d = LU32^{-1} (r / s) s, with s = max |r| per column so FP32 sees O(1) values
-        hipLaunchKernelGGL(scale_from_norms_kernel, dim3(1), dim3(8), 0, st, (const double*)norms, scale, inv_scale, q);
+        hipLaunchKernelGGL(scale_from_norms_kernel, dim3(1), dim3(8), 0, st, (const double*)(it == 0 ? bnorms : norms),
+                           scale, inv_scale, q);
         if (frame)
             hipLaunchKernelGGL(gather_rot_kernel, dim3(g), dim3(256), 0, st, (const double*)r, y, n, q,
                                (const int32_t*)FactorAux<float>::perm(aux), (const double*)inv_scale, R);
@@ -1539,7 +1558,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
                                n, q, (const double*)scale);
         FL_LAUNCHED();
         ++it;
-        hipLaunchKernelGGL(residual_kernel, dim3((unsigned)div_up(n, 4 * RES_ROWS)), dim3(256), 0, st, A64, lda64, n, b, ldb,
+        hipLaunchKernelGGL(residual_kernel, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, A64, lda64, n, b, ldb,
                            (const double*)xc, r, q);
         FL_LAUNCHED();
     }

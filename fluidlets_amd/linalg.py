@@ -37,14 +37,16 @@ class LUFactor:
         return int(self.lu.numel() * self.lu.element_size() + self.aux.numel())
 
 
-def getrf(A, overwrite: bool = True, stream=None, refill=None) -> LUFactor:
+def getrf(A, overwrite: bool = True, stream=None, refill=None, hint: dict | None = None) -> LUFactor:
     """Factor an (n, n) float32/float64 device tensor in place (or a copy).
 
     With ``refill`` (a callable that rewrites A's original values into the
     tensor it is given) the factorization first runs asynchronously without
     interchanges (fl_?getrf_async, one read-back for the whole matrix); if
     that is not partial pivoting's answer, A is refilled and factored on the
-    verified path.  The result is the same either way."""
+    verified path.  The result is the same either way.  ``hint`` (a dict
+    shared by a sequence of similar factorizations) records a failed attempt,
+    after which the sequence goes straight to the verified path."""
     t = D.torch()
     if A.dim() != 2 or A.shape[0] != A.shape[1]:
         raise ValueError("getrf needs a square matrix")
@@ -61,7 +63,7 @@ def getrf(A, overwrite: bool = True, stream=None, refill=None) -> LUFactor:
     aux = t.empty((max(int(L.fl_getrf_aux_size(n, prec)), 1),), dtype=t.uint8, device=A.device)
     nbytes = int(L.fl_getrf_workspace_size(n, prec))
     ws = D.workspace(nbytes, stream)
-    if refill is not None:
+    if refill is not None and (hint is None or hint.get("async", True)):
         status = t.empty((1,), dtype=t.int32, device=A.device)
         fa = L.fl_dgetrf_async if prec == FL_FP64 else L.fl_sgetrf_async
         check(fa(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), D.ptr(status), D.ptr(ws), nbytes, D.stream_handle(stream)),
@@ -70,6 +72,8 @@ def getrf(A, overwrite: bool = True, stream=None, refill=None) -> LUFactor:
             ok = int(status.cpu().item()) == 0     # waits for the factorization on its stream
             if ok:
                 return LUFactor(A, ipiv, aux, n, 0, 0, {"async": True})
+            if hint is not None:
+                hint["async"] = False
             refill(A)
     info = ctypes.c_int(0)
     fb = ctypes.c_int(0)

@@ -104,13 +104,17 @@ class SwimState:
 
 
 def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, nearest=None,
-                     precision: str = "fp64", stream=None, factor_cache: FactorCache | None = None) -> SwimState:
+                     precision: str = "fp64", stream=None, factor_cache: FactorCache | None = None,
+                     lu_hint: dict | None = None) -> SwimState:
     """Solve the instantaneous free-swimming problem (reference signature).
 
     ``precision`` "fp64" (default) factors K in FP64; "mixed" uses FP32 MFMA
     factors with FP64 iterative refinement of all seven right-hand sides.
     With ``factor_cache`` ("mixed", Stokeslet3D or Brinkmanlet3D) the FP32
-    factors of the body-frame matrix are taken from, or added to, the cache."""
+    factors of the body-frame matrix are taken from, or added to, the cache.
+    ``lu_hint`` is shared by the solves of one trajectory (linalg.getrf): a
+    swimmer whose matrix needs row interchanges tries the asynchronous LU
+    once."""
     t = D.torch()
     kernel = Stokeslet3D() if kernel is None else kernel
     if np.iscomplexobj(mu):
@@ -145,7 +149,7 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
     x0c = (ctypes.c_double * 3)(*x0.tolist())
     check(L.fl_swim_rhs(N, D.ptr(x_soa), x0c, D.ptr(v_soa), D.ptr(R), D.stream_handle(stream)), "fl_swim_rhs")
     if precision == "fp64":
-        F = linalg.getrf(K64, stream=stream,
+        F = linalg.getrf(K64, stream=stream, hint=lu_hint,
                          refill=lambda A: assemble_nearest_device(kernel, x_soa, X_soa, eps_t, stride, order_t, start_t,
                                                                   N, mu=mu, K64=A, stream=stream))
         if F.info > 0:
@@ -161,7 +165,7 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
                 return assemble_nearest_device(kernel, xb, Xb, eps_t, stride, order_t, start_t, N, mu=mu, fp32=True,
                                                fp64=False, stream=stream)[1]
 
-            F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()))
+            F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()), hint=lu_hint)
             if F.info > 0:
                 raise np.linalg.LinAlgError("Singular matrix")
             factor_cache.put(key, (F,), F.nbytes())
@@ -169,7 +173,7 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
             F = item[0]
         Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream, frame=_frame_rotation(B))
     elif precision == "mixed":
-        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64))
+        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), hint=lu_hint)
         if F.info > 0:
             raise np.linalg.LinAlgError("Singular matrix")
         Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream)
@@ -197,10 +201,12 @@ class Trajectory:
 
 
 def _frame_rhs(shape_at, eps, mu, kernel, nearest, precision):
+    lu_hint: dict = {}
+
     def rhs(t, z):
         b1, b2 = z[3:6], z[6:9]
         B = np.c_[b1, b2, np.cross(b1, b2)]
-        st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision)
+        st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision, lu_hint=lu_hint)
         return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
     return rhs
 
@@ -256,12 +262,13 @@ def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, st
         factor_cache = None
     elif factor_cache is None:
         factor_cache = FactorCache(reuse_cache_bytes(len(s0.x)))
+    lu_hint: dict = {}
 
     def F(t, z):
         b1, b2 = z[3:6], z[6:9]
         B = np.c_[b1, b2, np.cross(b1, b2)]
         st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision, stream=stream,
-                              factor_cache=factor_cache)
+                              factor_cache=factor_cache, lu_hint=lu_hint)
         return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
 
     z = np.concatenate([np.asarray(x0, float), B0[:, 0], B0[:, 1]])

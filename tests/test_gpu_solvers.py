@@ -119,6 +119,12 @@ def test_async_getrf_equals_the_verified_path(dtype):
     assert not Fr.extra.get("async")
     Fv = linalg.getrf(dev(R.copy()))
     assert torch.equal(Fr.lu, Fv.lu) and torch.equal(Fr.ipiv, Fv.ipiv)
+    # a hint shared by a sequence remembers the failure: the next call does not try again
+    hint, refills = {}, []
+    linalg.getrf(dev(R.copy()), refill=lambda A: (refills.append(1), A.copy_(src)), hint=hint)
+    assert hint == {"async": False} and len(refills) == 1
+    F2 = linalg.getrf(dev(R.copy()), refill=lambda A: refills.append(1), hint=hint)
+    assert len(refills) == 1 and torch.equal(F2.lu, Fv.lu)
     # an exactly zero pivot also sends it to the verified path, which reports info
     Z = np.eye(200, dtype=dtype)
     Z[150, 150] = 0.0
