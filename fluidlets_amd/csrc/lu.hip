@@ -781,7 +781,7 @@ __device__ __forceinline__ T chain_read(const T* p, bool& gave_up)
     using U = typename ChainBits<T>::U;
     const U* q = reinterpret_cast<const U*>(p);
     U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int spins = 0;
+    int spins = gave_up ? CH_SPIN : 0;   // a thread waits out the limit at most once
     while (v == ChainBits<T>::sentinel) {
         if (++spins > CH_SPIN) {
             gave_up = true;
@@ -938,7 +938,8 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
         for (int q = 0; q < NR; ++q)
             if (q < nrhs) x[(r0 + r) * nrhs + q] = bad ? T(NAN) : y[q];
     // every wave's stores complete (reach this XCD's L2), then one release:
-    // its L2 write-back publishes all of them to the other XCDs
+    // its L2 write-back publishes all of them to the other XCDs (cheaper than
+    // agent-scope atomic stores entry by entry, measured)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
