@@ -795,18 +795,30 @@ __device__ __forceinline__ T chain_read(const T* p, bool& gave_up)
     return out;
 }
 
-template <typename T, bool LOWER, int NR>
-__global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t lda, int64_t n, const T* invs,
-                                                         const T* b, T* x, int nrhs, unsigned* sync)
+// Sum over the TPR consecutive lanes that share a row (TPR = 2 or 4), by DPP.
+template <int TPR, typename T>
+__device__ __forceinline__ T row_sum(T v)
 {
-    constexpr int H = NB / 2;
+    v += dpp_mov<0xB1>(v);
+    if (TPR == 4) v += dpp_mov<0x4E>(v);
+    return v;
+}
+
+// TPR lanes per row of the block (NB x TPR lanes per workgroup), each
+// covering NB / TPR columns of every block product.
+template <typename T, bool LOWER, int NR, int TPR>
+__global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_t lda, int64_t n, const T* invs,
+                                                             const T* b, T* x, int nrhs, unsigned* sync)
+{
+    constexpr int H = NB / TPR;
+    constexpr int NT = NB * TPR;
     constexpr int V = Vec16<T>::n;
     constexpr int TP = NB + (sizeof(T) == 4 ? 4 : 2);  // LDS row pitch: rows 16-byte aligned, 4 banks apart
     using VT = typename Vec16<T>::type;
     __shared__ __attribute__((aligned(16))) T Ts[NB * TP];
     __shared__ __attribute__((aligned(16))) T xs[NB * NR];
     __shared__ int tk;
-    const int tid = threadIdx.x, r = tid >> 1, hf = tid & 1;
+    const int tid = threadIdx.x, r = tid / TPR, hf = tid % TPR;
     const int64_t nblk = (n + NB - 1) / NB;
     if (tid == 0) tk = (int)atomicAdd(&sync[0], 1u);
     __syncthreads();
@@ -822,8 +834,8 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
         bq[q] = (hf == 0 && r < nbp && q < nrhs) ? b[(r0 + r) * nrhs + q] : T(0);
     }
     const int64_t nsteps = LOWER ? p : nblk - 1 - p;
-    // this thread's half row of T_pj: all H entries loaded at once (issued a
-    // step ahead, so the loads overlap the previous block's products)
+    // this thread's part of the row of T_pj: all H entries loaded at once
+    // (issued a step ahead, so the loads overlap the previous block's products)
     T trow[H];
     auto load_row = [&](int64_t j) {
         const int64_t c0 = j * NB;
@@ -855,52 +867,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
             for (int q = 0; q < V; ++q) trow[k + q] = up[q];
         }
     };
-    if (nsteps > 0) load_row(LOWER ? 0 : nblk - 1);
-    else load_inv();
-#pragma unroll 1
-    for (int64_t s = 0; s < nsteps; ++s) {
-        const int64_t j = LOWER ? s : nblk - 1 - s;
-        const int64_t c0 = j * NB;
-        const int nbj = (int)min((int64_t)NB, n - c0);
-        {
-            VT* dst = reinterpret_cast<VT*>(&Ts[r * TP + hf * H]);
-#pragma unroll
-            for (int k = 0; k < H; k += V) {
-                VT u;
-                T* up = reinterpret_cast<T*>(&u);
-#pragma unroll
-                for (int q = 0; q < V; ++q) up[q] = trow[k + q];
-                dst[k / V] = u;
-            }
-        }
-        for (int idx = tid; idx < NB * NR; idx += 256) {
-            const int c = idx / NR, q = idx % NR;
-            xs[idx] = (c < nbj && q < nrhs) ? chain_read(&x[(c0 + c) * nrhs + q], gave_up) : T(0);
-        }
-        __syncthreads();
-        if (s + 1 < nsteps) load_row(LOWER ? s + 1 : nblk - 2 - s);
-        else load_inv();
-        const T* tr = &Ts[r * TP + hf * H];
-#pragma unroll 1
-        for (int k0 = 0; k0 < H; k0 += V) {
-            const VT u = *reinterpret_cast<const VT*>(tr + k0);
-            const T* up = reinterpret_cast<const T*>(&u);
-#pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const T* xk = &xs[(hf * H + k0 + k) * NR];
-#pragma unroll
-                for (int q = 0; q < NR; ++q) part[q] = fma(up[k], xk[q], part[q]);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < NR; ++q) part[q] += dpp_mov<0xB1>(part[q]);  // the two halves of row r
-    if (hf == 0)
-#pragma unroll
-        for (int q = 0; q < NR; ++q)
-            xs[r * NR + q] = (r < nbp && q < nrhs) ? bq[q] - part[q] : T(0);
-    {
+    auto stage_row = [&]() {
         VT* dst = reinterpret_cast<VT*>(&Ts[r * TP + hf * H]);
 #pragma unroll
         for (int k = 0; k < H; k += V) {
@@ -910,12 +877,9 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
             for (int q = 0; q < V; ++q) up[q] = trow[k + q];
             dst[k / V] = u;
         }
-    }
-    __syncthreads();
-    T y[NR];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) y[q] = T(0);
-    {
+    };
+    // acc[q] += (this thread's columns of the staged row) . xs[:, q]
+    auto row_product = [&](T* acc) {
         const T* tr = &Ts[r * TP + hf * H];
 #pragma unroll 1
         for (int k0 = 0; k0 < H; k0 += V) {
@@ -925,12 +889,42 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(const T* LU, int64_t ld
             for (int k = 0; k < V; ++k) {
                 const T* xk = &xs[(hf * H + k0 + k) * NR];
 #pragma unroll
-                for (int q = 0; q < NR; ++q) y[q] = fma(up[k], xk[q], y[q]);
+                for (int q = 0; q < NR; ++q) acc[q] = fma(up[k], xk[q], acc[q]);
             }
         }
+    };
+    if (nsteps > 0) load_row(LOWER ? 0 : nblk - 1);
+    else load_inv();
+#pragma unroll 1
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const int64_t j = LOWER ? s : nblk - 1 - s;
+        const int64_t c0 = j * NB;
+        const int nbj = (int)min((int64_t)NB, n - c0);
+        stage_row();
+        for (int idx = tid; idx < NB * NR; idx += NT) {
+            const int c = idx / NR, q = idx % NR;
+            xs[idx] = (c < nbj && q < nrhs) ? chain_read(&x[(c0 + c) * nrhs + q], gave_up) : T(0);
+        }
+        __syncthreads();
+        if (s + 1 < nsteps) load_row(LOWER ? s + 1 : nblk - 2 - s);
+        else load_inv();
+        row_product(part);
+        __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < NR; ++q) y[q] += dpp_mov<0xB1>(y[q]);
+    for (int q = 0; q < NR; ++q) part[q] = row_sum<TPR>(part[q]);  // the TPR parts of row r
+    if (hf == 0)
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            xs[r * NR + q] = (r < nbp && q < nrhs) ? bq[q] - part[q] : T(0);
+    stage_row();
+    __syncthreads();
+    T y[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) y[q] = T(0);
+    row_product(y);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) y[q] = row_sum<TPR>(y[q]);
     const int bad = __syncthreads_or(gave_up);
     if (tid == 0 && bad) atomicOr(&sync[1], 1u);
     if (hf == 0 && r < nbp)
@@ -1402,17 +1396,24 @@ size_t getrs_ws_bytes(int64_t n, int64_t nrhs)
 // Words of the chained solves' sync area (two sweeps).
 size_t chain_sync_bytes(int64_t n) { return align_up((size_t)2 * (div_up(n, NB) + 2) * sizeof(unsigned), 256); }
 
+// Two lanes per row (256-lane workgroups).  Four lanes per row (512-lane
+// workgroups, half the products per lane) measured slower: 11.0 against
+// 8.4 us per FP32 block step at n = 5,400, the wider barriers and staging
+// costing more than the shorter products save.
+constexpr int CH_TPR = 2;
+
 template <typename T, bool LOWER>
 void launch_chain(const T* LU, int64_t lda, int64_t n, const T* invs, const T* b, T* x, int nrhs, unsigned* sync,
                   hipStream_t st)
 {
-    const dim3 grid((unsigned)div_up(n, NB)), blk(256);
-    if (nrhs <= 1) hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 1>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+    const dim3 grid((unsigned)div_up(n, NB)), blk(NB * CH_TPR);
+    if (nrhs <= 1)
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 1, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
     else if (nrhs <= 2)
-        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 2>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 2, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
     else if (nrhs <= 4)
-        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 4>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
-    else hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 8>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 4, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+    else hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 8, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
 }
 
 // Solves in place on a contiguous n x nrhs buffer `y` that already holds the
