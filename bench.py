@@ -261,12 +261,14 @@ def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4,
     return out
 
 
-def gmres_section(torch, fl, D, precision_list=("fp64", "mixed"), n_side=32, per_rod=128, restart=50,
-                  rtol=1e-8, maxiter=40):
+def gmres_section(torch, fl, D, precision_list=("fp64", "mixed", "fp64_block_jacobi"), n_side=32, per_rod=128,
+                  restart=50, rtol=1e-8, maxiter=40):
     """Matrix-free GMRES(restart) on the fixed-rod carpet: 32 x 32 rods of
     128 helical-lattice points (131,072 points), WallStokeslet3D, eps = 0.01,
     rods held fixed in the background shear u = (z, 0, 0) (right-hand side
-    -(z, 0, 0)); FP64 GMRES and flexible GMRES with FP32C inner cycles."""
+    -(z, 0, 0)); FP64 GMRES, flexible GMRES with FP32C inner cycles, and FP64
+    GMRES preconditioned by the rods' diagonal blocks (block-Jacobi, setup
+    included in the time)."""
     from fluidlets_amd.core.gmres import gmres_device
     from fluidlets_amd.geometry import rod_carpet
 
@@ -283,8 +285,10 @@ def gmres_section(torch, fl, D, precision_list=("fp64", "mixed"), n_side=32, per
     for prec in precision_list:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        bj = prec == "fp64_block_jacobi"
         f, its, res = gmres_device(fl.WallStokeslet3D(), x_soa, eps_t, stride, U_soa, rtol=rtol, restart=restart,
-                                   maxiter=maxiter, precision=prec)
+                                   maxiter=maxiter, precision="fp64" if bj else prec,
+                                   block_points=per_rod if bj else 0)
         torch.cuda.synchronize()
         row = {"seconds": time.perf_counter() - t0, "iterations": its, "residual": res}
         if ref is None:

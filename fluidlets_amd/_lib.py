@@ -58,6 +58,14 @@ _SIGNATURES = {
     "fl_gmres": (c_int, [POINTER(fl_kernel), c_int64, c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p,
                          c_double, c_int, c_int, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t,
                          c_void_p]),
+    "fl_gmres_bj_workspace_size": (c_size_t, [c_int64, c_int, c_int]),
+    "fl_gmres_bj": (c_int, [POINTER(fl_kernel), c_int64, c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p,
+                            c_double, c_int, c_int, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t,
+                            c_void_p]),
+    "fl_assemble_blocks": (c_int, [POINTER(fl_kernel), c_int64, c_int, c_void_p, c_void_p, c_int64, c_double,
+                                   c_void_p, c_void_p]),
+    "fl_block_lu": (c_int, [c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fl_block_apply": (c_int, [c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fl_sgemm": (c_int, [c_int64, c_int64, c_int64, c_float, c_void_p, c_int64, c_void_p, c_int64, c_float,
                          c_void_p, c_int64, c_void_p]),
     "fl_dgemm": (c_int, [c_int64, c_int64, c_int64, c_double, c_void_p, c_int64, c_void_p, c_int64, c_double,

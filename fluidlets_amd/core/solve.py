@@ -75,13 +75,15 @@ def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-
 
 
 def resistance(kernel, points, velocities, eps, mu=1.0, method="auto", rtol=1e-10, restart=60, maxiter=200,
-               precision="fp64"):
+               precision="fp64", block_points: int = 0):
     """Forces that produce the given velocities at the given points.
 
     Same signature, results and exceptions as the reference's
     ``fluidlets.resistance``; ``precision`` ("fp64" or "mixed") selects the
     dense factorization (LU) or the inner matvec precision (GMRES:
-    "mixed" runs flexible GMRES with FP32C inner matvecs).
+    "mixed" runs flexible GMRES with FP32C inner matvecs).  With GMRES,
+    ``block_points`` > 0 (points per body, the bodies consecutive) adds
+    block-Jacobi preconditioning by the bodies' diagonal blocks.
 
     Returns
     -------
@@ -114,7 +116,7 @@ def resistance(kernel, points, velocities, eps, mu=1.0, method="auto", rtol=1e-1
         iterations = 0
         for c in range(cols.shape[1]):
             fc, its = gmres_solve(kernel, x_soa, eps_t, stride, cols[:, c], rtol=rtol, restart=restart,
-                                  maxiter=maxiter, precision=precision)
+                                  maxiter=maxiter, precision=precision, block_points=block_points)
             fcols.append(fc)
             iterations += its
         f = _from_columns(np.stack(fcols, axis=1), dtype, mu)

@@ -278,6 +278,36 @@ __global__ __launch_bounds__(WG) void nearest_kernel(int64_t Q, int64_t N, const
     if (k < Q) out[k] = arg;
 }
 
+// Diagonal blocks of the resistance matrix for bodies of P consecutive points:
+// KB[b][3i + a][3j + c] = w G_ac(x_{bP+i}, x_{bP+j}; eps_{bP+j}), i, j < P,
+// each block 3P x 3P row-major (the block-Jacobi preconditioner of GMRES).
+// One lane per (block, target, source); consecutive lanes write consecutive
+// sources of a row.
+template <int KIND>
+__global__ __launch_bounds__(WG) void assemble_blocks_kernel(const double* X, int64_t N, int P, int64_t nblk,
+                                                             const double* eps, int64_t eps_stride, double alpha,
+                                                             double inv_alpha, double w, double* KB)
+{
+    const int64_t t = (int64_t)blockIdx.x * WG + threadIdx.x;
+    const int64_t PP = (int64_t)P * P;
+    const int64_t b = t / PP;
+    if (b >= nblk) return;
+    const int rem = (int)(t % PP), i = rem / P, j = rem % P;
+    const int64_t gi = b * P + i, gj = b * P + j;
+    const double e = eps[gj * eps_stride];
+    double G[9];
+    block3x3<KIND>(X[gi], X[N + gi], X[2 * N + gi], X[gj], X[N + gj], X[2 * N + gj], e * e, alpha, inv_alpha, G);
+    const int64_t m = 3 * (int64_t)P;
+    double* K = KB + b * m * m;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double* row = K + (3 * i + a) * m + 3 * j;
+        row[0] = w * G[3 * a];
+        row[1] = w * G[3 * a + 1];
+        row[2] = w * G[3 * a + 2];
+    }
+}
+
 }  // namespace
 
 int nearest_launch(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* out, hipStream_t stream)
@@ -370,6 +400,29 @@ int assemble_nearest_launch(int kind, double alpha, int64_t M, int64_t Q, int64_
     else if (kind == FL_WALL_STOKESLET3D)
         hipLaunchKernelGGL(assemble_nearest_kernel<FL_WALL_STOKESLET3D>, grid, dim3(WG), 0, stream, a);
     else hipLaunchKernelGGL(assemble_nearest_kernel<FL_BRINKMANLET3D>, grid, dim3(WG), 0, stream, a);
+    FL_LAUNCHED();
+    return FL_OK;
+}
+
+int assemble_blocks_launch(int kind, double alpha, int64_t N, int P, const double* X, const double* eps,
+                           int64_t eps_stride, double w, double* KB, hipStream_t stream)
+{
+    FL_CHECK(kind >= 0 && kind <= 2, "unknown kernel kind %d", kind);
+    FL_CHECK(P >= 1 && N >= 0 && N % P == 0, "N must be a multiple of the block size");
+    FL_CHECK(eps_stride == 0 || eps_stride == 1, "eps_stride must be 0 or 1");
+    const int64_t nblk = N / P;
+    const int64_t total = nblk * (int64_t)P * P;
+    if (total == 0) return FL_OK;
+    const dim3 grid((unsigned)((total + WG - 1) / WG));
+    const double inv_alpha = alpha > 0 ? 1.0 / alpha : 0.0;
+    if (kind == FL_STOKESLET3D)
+        hipLaunchKernelGGL(assemble_blocks_kernel<FL_STOKESLET3D>, grid, dim3(WG), 0, stream, X, N, P, nblk, eps,
+                           eps_stride, alpha, inv_alpha, w, KB);
+    else if (kind == FL_WALL_STOKESLET3D)
+        hipLaunchKernelGGL(assemble_blocks_kernel<FL_WALL_STOKESLET3D>, grid, dim3(WG), 0, stream, X, N, P, nblk, eps,
+                           eps_stride, alpha, inv_alpha, w, KB);
+    else hipLaunchKernelGGL(assemble_blocks_kernel<FL_BRINKMANLET3D>, grid, dim3(WG), 0, stream, X, N, P, nblk, eps,
+                            eps_stride, alpha, inv_alpha, w, KB);
     FL_LAUNCHED();
     return FL_OK;
 }

@@ -80,10 +80,11 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
                 double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st);
 
-size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision);
+size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision, int block_points);
 int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double* eps, int64_t eps_stride,
                double mu_re, const double* U, double* f, double rtol, int restart, int maxiter, int inner_precision,
-               int* iters_out, double* resid_out, void* ws, size_t ws_bytes, hipStream_t st);
+               int* iters_out, double* resid_out, void* ws, size_t ws_bytes, hipStream_t st, int block_points);
+size_t bj_bytes(int64_t N, int P);
 
 template <typename T>
 int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
@@ -253,7 +254,7 @@ int fl_refine_frame(const double* A64, int64_t lda64, const float* LU32, int64_t
 
 size_t fl_gmres_workspace_size(int64_t N, int restart, int inner_precision)
 {
-    return gmres_workspace_bytes(N, restart, inner_precision);
+    return gmres_workspace_bytes(N, restart, inner_precision, 0);
 }
 
 int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* eps, int64_t eps_stride, double mu_re,
@@ -263,7 +264,48 @@ int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* 
     clear_error();
     if (int s = check_kernel(kernel)) return s;
     return gmres_impl(kernel->kind, kernel->alpha, N, X, eps, eps_stride, mu_re, U, f, rtol, restart, maxiter,
-                      inner_precision, iters_out, resid_out, workspace, workspace_bytes, as_stream(stream));
+                      inner_precision, iters_out, resid_out, workspace, workspace_bytes, as_stream(stream), 0);
+}
+
+size_t fl_gmres_bj_workspace_size(int64_t N, int restart, int block_points)
+{
+    return gmres_workspace_bytes(N, restart, FL_FP64, block_points);
+}
+
+int fl_gmres_bj(const fl_kernel* kernel, int64_t N, const double* X, const double* eps, int64_t eps_stride,
+                double mu_re, const double* U, double* f, double rtol, int restart, int maxiter, int block_points,
+                int* iters_out, double* resid_out, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    if (block_points <= 0) return set_error(FL_EINVAL, "fl_gmres_bj needs a positive block size");
+    return gmres_impl(kernel->kind, kernel->alpha, N, X, eps, eps_stride, mu_re, U, f, rtol, restart, maxiter, FL_FP64,
+                      iters_out, resid_out, workspace, workspace_bytes, as_stream(stream), block_points);
+}
+
+int fl_block_lu(int64_t m, int64_t nblk, double* blocks, int32_t* ipiv, int32_t* perm, int32_t* info, void* stream)
+{
+    clear_error();
+    if (nblk > 0 && !(blocks && ipiv && perm && info)) return set_error(FL_EINVAL, "null pointer argument");
+    return block_lu_launch(m, nblk, blocks, ipiv, perm, info, as_stream(stream));
+}
+
+int fl_block_apply(int64_t N, int block_points, const double* blocks, const int32_t* perm, const double* v, double* z,
+                   void* stream)
+{
+    clear_error();
+    if (N > 0 && !(blocks && perm && v && z)) return set_error(FL_EINVAL, "null pointer argument");
+    return block_apply_launch(N, block_points, blocks, perm, v, z, as_stream(stream));
+}
+
+int fl_assemble_blocks(const fl_kernel* kernel, int64_t N, int block_points, const double* X, const double* eps,
+                       int64_t eps_stride, double mu_re, double* blocks, void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    FL_CHECK(mu_re != 0.0, "mu must be nonzero");
+    return assemble_blocks_launch(kernel->kind, kernel->alpha, N, block_points, X, eps, eps_stride,
+                                  kernel_scale(kernel->kind) / mu_re, blocks, as_stream(stream));
 }
 
 int fl_sgemm(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,

@@ -59,4 +59,20 @@ int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt,
 // 1/(4 pi) for the Brinkmanlet.
 double kernel_scale(int kind);
 
+// Diagonal blocks (bodies of P consecutive points) of the matrix w G, each
+// 3P x 3P row-major, written to KB (N / P blocks).
+int assemble_blocks_launch(int kind, double alpha, int64_t N, int P, const double* X, const double* eps,
+                           int64_t eps_stride, double w, double* KB, hipStream_t stream);
+
+// Block-Jacobi preconditioner: batched LU with partial pivoting of nblk
+// m x m row-major blocks in place (ipiv, perm: nblk x m int32), and
+// z = blockdiag(K_b)^{-1} v on (3, N) structure-of-arrays vectors whose block
+// b holds points b P .. b P + P - 1 (m = 3P).
+int block_lu_launch(int64_t m, int64_t nblk, double* KB, int32_t* ipiv, int32_t* perm, int32_t* info,
+                    hipStream_t stream);
+int block_apply_launch(int64_t N, int P, const double* KB, const int32_t* perm, const double* v, double* z,
+                       hipStream_t stream);
+// Bytes of GMRES's block-Jacobi storage for N points in bodies of P.
+size_t bj_bytes(int64_t N, int P);
+
 }  // namespace fl

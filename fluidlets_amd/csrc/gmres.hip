@@ -13,6 +13,10 @@
 // Flexible variant (inner_precision = FL_FP32C): FGMRES(m) in FP64 whose
 // preconditioner z = M^{-1} v is one inner GMRES(m_in) cycle with FP32C
 // matvecs; the outer matvecs, residuals and solution stay FP64.
+//
+// Block-Jacobi variant (block_points = P > 0): the same right-preconditioned
+// loop with z = blockdiag(K_b)^{-1} v, K_b the FP64 matrix of body b (points
+// b P .. b P + P - 1), assembled and factored once (bjacobi.hip).
 #include "fl_internal.h"
 
 #include <algorithm>
@@ -201,12 +205,22 @@ int inner_cycle(const Op& op, const double* v, double* z, int m, double* V, doub
 
 }  // namespace
 
-size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision)
+// Block-Jacobi storage: the factored blocks, then ipiv, perm and info.
+size_t bj_bytes(int64_t N, int P)
+{
+    if (P <= 0 || N % P != 0) return 0;
+    const int64_t m = 3 * (int64_t)P, nblk = N / P;
+    return align_up((size_t)nblk * m * m * sizeof(double), 256) + 2 * align_up((size_t)nblk * m * sizeof(int32_t), 256) +
+           align_up((size_t)nblk * sizeof(int32_t), 256);
+}
+
+size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision, int block_points)
 {
     const int64_t n = 3 * N;
     const size_t vec = align_up((size_t)n * sizeof(double), 256);
     size_t b = (size_t)(restart + 1) * vec + 2 * vec + 4096;  // V, w, r, dots
     if (inner_precision == FL_FP32C) b += (size_t)restart * vec + (size_t)(kInnerRestart + 1) * vec + vec;
+    else if (block_points > 0) b += (size_t)restart * vec + bj_bytes(N, block_points);
     b += align_up(std::max({eval_workspace_size(N, N, 1, inner_precision), eval_workspace_size(N, N, 1, FL_FP64),
                             eval_sym_workspace_size(N, 1, FL_FP64)}),
                   256);
@@ -215,13 +229,17 @@ size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision)
 
 int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double* eps, int64_t eps_stride,
                double mu_re, const double* U, double* f, double rtol, int restart, int maxiter, int inner_precision,
-               int* iters_out, double* resid_out, void* ws, size_t ws_bytes, hipStream_t st)
+               int* iters_out, double* resid_out, void* ws, size_t ws_bytes, hipStream_t st, int block_points)
 {
     FL_CHECK(N >= 0 && restart >= 1 && maxiter >= 1, "invalid GMRES sizes");
     FL_CHECK(inner_precision == FL_FP64 || inner_precision == FL_FP32C, "unknown inner precision");
     FL_CHECK(mu_re != 0.0, "mu must be nonzero");
+    FL_CHECK(block_points >= 0, "negative block size");
+    FL_CHECK(block_points == 0 || inner_precision == FL_FP64, "block-Jacobi runs with FP64 matvecs");
+    FL_CHECK(block_points == 0 || (N % block_points == 0 && 3 * (int64_t)block_points <= 512),
+             "block-Jacobi needs N a multiple of the block size and at most 170 points per block");
     if (N == 0) return FL_OK;
-    const size_t need = gmres_workspace_bytes(N, restart, inner_precision);
+    const size_t need = gmres_workspace_bytes(N, restart, inner_precision, block_points);
     if (ws_bytes < need || !ws) return set_error(FL_EWORKSPACE, "gmres needs %zu workspace bytes", need);
     const int64_t n = 3 * N;
     const size_t vec = align_up((size_t)n * sizeof(double), 256);
@@ -238,7 +256,9 @@ int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double*
     p += 4096;
     FL_CHECK(restart + 2 <= 512, "restart too large");
     const bool flexible = inner_precision == FL_FP32C;
-    double *Z = nullptr, *Vin = nullptr, *win = nullptr;
+    const bool bj = block_points > 0;
+    double *Z = nullptr, *Vin = nullptr, *win = nullptr, *KB = nullptr;
+    int32_t *bj_ipiv = nullptr, *bj_perm = nullptr, *bj_info = nullptr;
     if (flexible) {
         Z = reinterpret_cast<double*>(p);
         p += (size_t)restart * vec;
@@ -246,11 +266,36 @@ int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double*
         p += (size_t)(kInnerRestart + 1) * vec;
         win = reinterpret_cast<double*>(p);
         p += vec;
+    } else if (bj) {
+        const int64_t m = 3 * (int64_t)block_points, nblk = N / block_points;
+        Z = reinterpret_cast<double*>(p);
+        p += (size_t)restart * vec;
+        KB = reinterpret_cast<double*>(p);
+        p += align_up((size_t)nblk * m * m * sizeof(double), 256);
+        bj_ipiv = reinterpret_cast<int32_t*>(p);
+        p += align_up((size_t)nblk * m * sizeof(int32_t), 256);
+        bj_perm = reinterpret_cast<int32_t*>(p);
+        p += align_up((size_t)nblk * m * sizeof(int32_t), 256);
+        bj_info = reinterpret_cast<int32_t*>(p);
+        p += align_up((size_t)nblk * sizeof(int32_t), 256);
     }
     Op op{kind, alpha, N, X, eps, eps_stride, kernel_scale(kind) / mu_re, nullptr, 0, st};
     op.ews = p;
     op.ews_bytes = std::max({eval_workspace_size(N, N, 1, FL_FP64), eval_workspace_size(N, N, 1, inner_precision),
                              eval_sym_workspace_size(N, 1, FL_FP64)});
+    if (bj) {
+        // the bodies' diagonal blocks of the operator (same scale), factored once
+        const int64_t m = 3 * (int64_t)block_points, nblk = N / block_points;
+        if (int e = assemble_blocks_launch(kind, alpha, N, block_points, X, eps, eps_stride, op.w_re, KB, st)) return e;
+        if (int e = block_lu_launch(m, nblk, KB, bj_ipiv, bj_perm, bj_info, st)) return e;
+        std::vector<int32_t> hinfo(nblk);
+        FL_HIP(hipMemcpyAsync(hinfo.data(), bj_info, nblk * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        FL_HIP(hipStreamSynchronize(st));
+        for (int64_t q = 0; q < nblk; ++q)
+            if (hinfo[q] != 0)
+                return set_error(FL_ESINGULAR, "block-Jacobi: block %lld is singular (zero pivot %d)", (long long)q,
+                                 hinfo[q]);
+    }
 
     double bnorm = 0.0;
     if (int e = dot_host(U, U, n, s, st, bnorm)) return e;
@@ -287,6 +332,9 @@ int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double*
                                         inner_its))
                     return e;
                 if (int e = op.apply(Z + (int64_t)j * n, w, FL_FP64)) return e;
+            } else if (bj) {
+                if (int e = block_apply_launch(N, block_points, KB, bj_perm, vj, Z + (int64_t)j * n, st)) return e;
+                if (int e = op.apply(Z + (int64_t)j * n, w, FL_FP64)) return e;
             } else {
                 if (int e = op.apply(vj, w, FL_FP64)) return e;
             }
@@ -304,7 +352,8 @@ int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double*
         }
         std::vector<double> y = L.solve(k);
         FL_HIP(hipMemcpyAsync(s.dots, y.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(multiaxpy_kernel, dim3((unsigned)blocks(n)), dim3(WG), 0, st, (const double*)(flexible ? Z : V),
+        hipLaunchKernelGGL(multiaxpy_kernel, dim3((unsigned)blocks(n)), dim3(WG), 0, st,
+                           (const double*)(flexible || bj ? Z : V),
                            n, (const double*)s.dots, k, 1.0, f, n);
         FL_LAUNCHED();
         rnorm = -1.0;

@@ -260,6 +260,40 @@ int fl_gmres(const fl_kernel* kernel, int64_t N, const double* X, const double* 
              int inner_precision, int* iters_out, double* resid_out,
              void* workspace, size_t workspace_bytes, void* stream);
 
+/* Block-Jacobi preconditioned GMRES for several bodies: the N points form
+ * N / block_points bodies of block_points consecutive points, and
+ * z = blockdiag(K_b)^{-1} v (K_b the FP64 matrix of body b, exactly the
+ * diagonal block of the resistance matrix) preconditions fl_gmres's loop from
+ * the right.  Same operator, stopping rule (true residual <= rtol ||U||),
+ * arguments and results as fl_gmres with inner_precision FL_FP64; the
+ * reference has no preconditioner (src/fluidlets/core/solve.py:68-79), so
+ * this is an addition for many-body problems, which converge in far fewer
+ * iterations.  3 block_points <= 512.  Returns FL_ESINGULAR when a block has
+ * a zero pivot.  Synchronises on stream.
+ */
+size_t fl_gmres_bj_workspace_size(int64_t N, int restart, int block_points);
+int fl_gmres_bj(const fl_kernel* kernel, int64_t N, const double* X, const double* eps, int64_t eps_stride,
+                double mu_re, const double* U, double* f, double rtol, int restart, int maxiter, int block_points,
+                int* iters_out, double* resid_out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The preconditioner's pieces, for callers with their own Krylov loop.
+ * fl_assemble_blocks: the N / block_points diagonal blocks (body b: points
+ *   b P .. b P + P - 1) of the matrix of fl_assemble, each 3P x 3P row-major,
+ *   one after the other in `blocks` (device).
+ * fl_block_lu: partial-pivoting LU (getrf: largest |entry|, the first on
+ *   ties) of nblk m x m row-major blocks in place, m <= 512; ipiv (LAPACK
+ *   order, 0-based within each block) and perm (row i of the factors is row
+ *   perm[i] of the block), nblk x m int32 each; info[b] = first zero pivot
+ *   (1-based) of block b or 0.  Asynchronous.
+ * fl_block_apply: z = blockdiag^{-1} v on (3, N) structure-of-arrays vectors
+ *   with the factors of fl_block_lu (m = 3 block_points).  Asynchronous.
+ */
+int fl_assemble_blocks(const fl_kernel* kernel, int64_t N, int block_points, const double* X, const double* eps,
+                       int64_t eps_stride, double mu_re, double* blocks, void* stream);
+int fl_block_lu(int64_t m, int64_t nblk, double* blocks, int32_t* ipiv, int32_t* perm, int32_t* info, void* stream);
+int fl_block_apply(int64_t N, int block_points, const double* blocks, const int32_t* perm, const double* v, double* z,
+                   void* stream);
+
 /* ------------------------------------------------------------------ free swimmer
  * The bordered system of src/fluidlets/swimming.py:90-101 (no slip, zero net
  * force, zero net torque), which the reference solves with numpy.linalg.solve
