@@ -89,13 +89,56 @@ def cpu_baseline(X, F, n_targets: int):
     t0 = time.perf_counter()
     O.evaluate("wall", X[:n_targets], X, F, EPS)
     dt = time.perf_counter() - t0
+    # the same loop on one core, on a sixteenth of the targets
+    n1 = max(16, n_targets // 16)
+    O.set_threads(1)
+    t1 = time.perf_counter()
+    O.evaluate("wall", X[:n1], X, F, EPS)
+    dt1 = time.perf_counter() - t1
+    O.set_threads(threads)
     return {"value": n_targets * N_POINTS / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+            "host_cores": cores, "value_1core": n1 * N_POINTS / dt1,
             "sample": f"WallStokeslet3D FP64, first {n_targets} targets x all {N_POINTS} sources "
-                      f"({n_targets * N_POINTS:.3e} pairs, {dt:.1f} s); C/OpenMP restatement of "
-                      f"core/matvec.py:_apply with the reference's pow()-based block, -O3, no fast-math"}
+                      f"({n_targets * N_POINTS:.3e} pairs, {dt:.1f} s; one core: {n1} targets, {dt1:.1f} s); "
+                      f"C/OpenMP restatement of core/matvec.py:_apply with the reference's pow()-based block, "
+                      f"-O3, no fast-math"}
 
 
-def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=True):
+def c1_section(torch, fl):
+    """The reference's own CPU case: a level-3 icosphere (642 points),
+    eps = 0.25 x mean nearest-neighbour spacing, Stokeslet3D, U = (1, 0, 0),
+    dense LU (1926 x 1926), drag / (6 pi), then the forces evaluated at
+    10,000 targets in the shell 1.5 < r < 5 (seed 1).  The same solve through
+    scipy.linalg.solve (LAPACK, the reference's call at core/solve.py:66) is
+    timed on the host with the kernel matrix this library assembled."""
+    import scipy.linalg
+    from fluidlets_amd.geometry import icosphere, mean_nearest_spacing
+
+    X = icosphere(3)
+    eps = 0.25 * mean_nearest_spacing(X)
+    U = np.tile([1.0, 0.0, 0.0], (len(X), 1))
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=(10000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    T = d * (rng.uniform(1.5 ** 3, 5.0 ** 3, 10000) ** (1.0 / 3.0))[:, None]
+    K = fl.Stokeslet3D()
+    fl.resistance(K, X, U, eps, method="lu")                       # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f, info = fl.resistance(K, X, U, eps, method="lu")
+    u = fl.evaluate(K, T, X, f, eps)
+    t_gpu = time.perf_counter() - t0
+    A = fl.assemble(K, X, X, eps)
+    t1 = time.perf_counter()
+    fc = scipy.linalg.solve(A, U.reshape(-1))
+    t_cpu = time.perf_counter() - t1
+    return {"points": len(X), "eps": eps, "drag_over_6pi": float(f[:, 0].sum() / (6 * np.pi)),
+            "residual": info.residual, "gpu_solve_and_evaluate_s": t_gpu, "shell_velocity_max": float(np.abs(u).max()),
+            "cpu_lapack_solve_s": t_cpu, "cpu_threads": len(os.sched_getaffinity(0)),
+            "cpu_vs_gpu_force_relerr": float(np.linalg.norm(fc - f.reshape(-1)) / np.linalg.norm(fc))}
+
+
+def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=True, cpu_lapack=True):
     """Mixed-precision wall resistance solve: Fibonacci sphere (radius 1,
     centre (0,0,1.5)), eps = 0.5 sqrt(4 pi / N), six rigid-body right-hand
     sides.  Times FP32-MFMA LU + FP64 refinement to 1e-10, the FP64 LU of this
@@ -210,6 +253,21 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
         row["fp64_lu_tflops"] = flops / best64 / 1e12
         row["mixed_lu_tflops_equiv"] = flops / bestm / 1e12
         row["solves_per_s_mixed"] = 1.0 / bestm
+        if cpu_lapack and name == kernels[0]:
+            # the reference's dense solve, scipy.linalg.solve (LAPACK getrf + getrs, threads = host cores)
+            import scipy.linalg
+            try:
+                Kh = K64.cpu().numpy()
+                bh = rhs.cpu().numpy()
+                t1 = time.perf_counter()
+                xc = scipy.linalg.solve(Kh, bh, overwrite_a=True, check_finite=False)
+                row["cpu_lapack_fp64_solve_s"] = time.perf_counter() - t1
+                row["cpu_lapack_threads"] = len(os.sched_getaffinity(0))
+                row["cpu_lapack_vs_fp64_force_relerr"] = float(np.linalg.norm(xc - x64h) / np.linalg.norm(x64h))
+                row["speedup_mixed_vs_cpu_lapack"] = row["cpu_lapack_fp64_solve_s"] / bestm
+                del Kh, xc
+            except Exception as exc:  # report, do not fail the bench
+                row["cpu_lapack_error"] = repr(exc)[:200]
         out[name] = row
         del K64, K32
         torch.cuda.empty_cache()
@@ -433,8 +491,10 @@ def main():
         result["c2_table"] = table
 
     if not args.no_solve:
+        result["icosphere_c1"] = c1_section(torch, fl)
         result["wall_sphere_solve"] = c3_solves(torch, fl, D, tuple(args.solve_kernels.split(",")),
-                                                args.solve_points, vendor=not args.no_vendor_lu)
+                                                args.solve_points, vendor=not args.no_vendor_lu,
+                                                cpu_lapack=not args.no_cpu)
 
     if not args.no_serve:
         result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax,
