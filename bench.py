@@ -202,27 +202,42 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
         row["fp64_lu_async"] = bool(F.extra.get("async", False))
         x64h = x64.cpu().numpy()
         del F
-        # mixed: FP32 MFMA LU + FP64 refinement (the FP32 copy is refactored each repeat)
-        bestm = None
-        for _ in range(repeats):
-            A32 = K32.clone()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            F32 = linalg.getrf(A32, refill=lambda X: X.copy_(K32))
-            xm, steps, res = linalg.refine(K64, F32, rhs, tol=1e-10)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            bestm = dt if bestm is None else min(bestm, dt)
-            t1 = time.perf_counter()
-            xm2, _, _ = linalg.refine(K64, F32, rhs, tol=1e-10)
-            torch.cuda.synchronize()
-            row["mixed_resolve_s"] = time.perf_counter() - t1
-            del F32, A32
+        # mixed: FP32 LU + FP64 refinement (the FP32 copy is refactored each
+        # repeat).  The library's mixed solves take split-bf16 trailing
+        # products ("bf16x3"); FP32 MFMA products are timed beside them.
+        # residual tolerance 1e-12: the 1e-10 residual the workload asks for
+        # leaves the forces' error against FP64 near 1e-10 on this matrix
+        # (condition ~70); one more correction puts it near 1e-14
+        refine_tol = 1e-12
+        row["refinement_tol"] = refine_tol
+        for products in ("fp32", "bf16x3"):
+            bestm = None
+            for _ in range(repeats):
+                A32 = K32.clone()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                F32 = linalg.getrf(A32, refill=lambda X: X.copy_(K32), products=products)
+                xm, steps, res = linalg.refine(K64, F32, rhs, tol=refine_tol)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                bestm = dt if bestm is None else min(bestm, dt)
+                t1 = time.perf_counter()
+                xm2, _, _ = linalg.refine(K64, F32, rhs, tol=refine_tol)
+                torch.cuda.synchronize()
+                resolve = time.perf_counter() - t1
+                del F32, A32
+            xmh = xm.cpu().numpy()
+            err = float(np.linalg.norm(xmh - x64h) / np.linalg.norm(x64h))
+            if products == "fp32":
+                row["mixed_fp32_products_lu_refine_s"] = bestm
+                row["mixed_fp32_products_refinement_steps"] = steps
+                row["mixed_fp32_products_vs_fp64_force_relerr"] = err
+        row["mixed_products"] = "bf16x3"
         row["mixed_lu_refine_s"] = bestm
+        row["mixed_resolve_s"] = resolve
         row["refinement_steps"] = steps
         row["refinement_residual_max"] = float(res.max())
-        xmh = xm.cpu().numpy()
-        row["mixed_vs_fp64_force_relerr"] = float(np.linalg.norm(xmh - x64h) / np.linalg.norm(x64h))
+        row["mixed_vs_fp64_force_relerr"] = err
         R64, Rm = grand(X, center, x64h), grand(X, center, xmh)
         row["grand_resistance_fp64"] = R64.round(6).tolist()
         row["grand_resistance_max_entry_relerr"] = float(np.abs(Rm - R64).max() / np.abs(R64).max())

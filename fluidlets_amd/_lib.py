@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfluidlets_hip.so"
 
 FL_OK, FL_EINVAL, FL_EHIP, FL_ENOCONV, FL_ESINGULAR, FL_EWORKSPACE = 0, -1, -2, -3, -4, -5
 FL_FP64, FL_FP32C = 0, 1
+FL_LU_FP32, FL_LU_BF16X3 = 0, 1
 
 
 class fl_kernel(ctypes.Structure):
@@ -41,6 +42,10 @@ _SIGNATURES = {
                           c_size_t, c_void_p]),
     "fl_sgetrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
+    "fl_sgetrf_ex": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, POINTER(c_int), POINTER(c_int), c_int,
+                             c_void_p, c_size_t, c_void_p]),
+    "fl_sgetrf_async_ex": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                   c_size_t, c_void_p]),
     "fl_dgetrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
     "fl_getrs_workspace_size": (c_size_t, [c_int64, c_int64, c_int]),

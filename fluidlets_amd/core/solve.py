@@ -65,7 +65,7 @@ def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-
         return x, 0, F
     if precision == "mixed":
         K64, K32 = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=True, stream=stream)
-        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64))
+        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3")
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
         x, steps, _ = linalg.refine(K64, F, rhs, tol=tol, max_iter=max_iter, stream=stream)

@@ -64,12 +64,12 @@ size_t getrf_aux_bytes(int64_t n, int precision);
 size_t getrf_workspace_bytes(int64_t n, int precision);
 size_t getrs_workspace_bytes(int64_t n, int64_t nrhs, int precision);
 size_t refine_workspace_bytes(int64_t n, int64_t nrhs);
-int sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
-           hipStream_t st);
+int sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, int products, void* ws,
+           size_t wsb, hipStream_t st);
 int dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
            hipStream_t st);
-int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
-                 size_t wsb, hipStream_t st);
+int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, int products,
+                 void* ws, size_t wsb, hipStream_t st);
 int dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
                  size_t wsb, hipStream_t st);
 int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb, void* ws,
@@ -187,7 +187,16 @@ int fl_sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* i
               void* workspace, size_t workspace_bytes, void* stream)
 {
     clear_error();
-    return sgetrf(A, n, lda, ipiv, aux, info, fallback_panels, workspace, workspace_bytes, as_stream(stream));
+    return sgetrf(A, n, lda, ipiv, aux, info, fallback_panels, FL_LU_FP32, workspace, workspace_bytes,
+                  as_stream(stream));
+}
+
+int fl_sgetrf_ex(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fallback_panels,
+                 int products, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return sgetrf(A, n, lda, ipiv, aux, info, fallback_panels, products, workspace, workspace_bytes,
+                  as_stream(stream));
 }
 
 int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fallback_panels,
@@ -201,7 +210,14 @@ int fl_sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, 
                     size_t workspace_bytes, void* stream)
 {
     clear_error();
-    return sgetrf_async(A, n, lda, ipiv, aux, status, workspace, workspace_bytes, as_stream(stream));
+    return sgetrf_async(A, n, lda, ipiv, aux, status, FL_LU_FP32, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_sgetrf_async_ex(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                       int products, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return sgetrf_async(A, n, lda, ipiv, aux, status, products, workspace, workspace_bytes, as_stream(stream));
 }
 
 int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,

@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace fl {
@@ -43,6 +44,11 @@ namespace {
 
 constexpr int NB = 128;   // diagonal block (LDS-resident GEPP, inverted for the solves)
 constexpr int NBO = 256;  // outer panel width: the trailing update runs with K = NBO
+// Outer panel width with split-bf16 trailing products: the cheaper products
+// leave the update bound by its passes over A22, which a wider panel halves.
+// (Measured at n = 24,576: 256 -> 512 takes the mixed solve from 85 to 75 ms;
+// 768 and 1024 gain 1 ms more but the factors' error grows with the width.)
+constexpr int NBO_SPLIT = 512;
 
 // ---------------------------------------------------------------- triangular inverses (general path)
 // Used after a panel has been refactored column by column with full-column
@@ -1164,10 +1170,60 @@ inline rocblas_status rb_gemm(rocblas_handle h, int m, int n, int k, const doubl
     return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-// C (M x N, row-major, ldc) -= A (M x K, lda) B (K x N, ldb).
+// ---------------------------------------------------------------- split-bf16 products
+// An FP32 value x is written as hi + lo, hi = bf16(x) and lo = bf16(x - hi)
+// (round to nearest even; x - hi is exact in FP32), so x agrees with hi + lo
+// to 2^-16 relative.  A product a b is then taken as
+//   a_hi b_hi + a_hi b_lo + a_lo b_hi
+// (the dropped terms are below 2^-16 |a b|), each on bf16 MFMA with FP32
+// sums: one GEMM over K' = 3K with A' = [A_hi | A_hi | A_lo] and
+// B' = [B_hi ; B_lo ; B_hi].  gfx950 runs bf16 MFMA at 16 times the FP32
+// MFMA rate, so the three products cost less than one FP32 product; the FP32
+// factors they give serve as FP64 refinement's preconditioner, whose
+// convergence they leave unchanged on these matrices.
+__device__ inline uint16_t bf16_rne(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));  // inf, nan
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_value(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// A' (M x 3K, ld 3K) = [hi | hi | lo] of A (M x K, lda); one workgroup per row.
+__global__ void split_rows_kernel(const float* A, int64_t lda, int K, uint16_t* Ap)
+{
+    const int64_t i = blockIdx.x;
+    uint16_t* o = Ap + i * 3 * K;
+    for (int c = threadIdx.x; c < K; c += blockDim.x) {
+        const float x = A[i * lda + c];
+        const uint16_t h = bf16_rne(x);
+        o[c] = h;
+        o[K + c] = h;
+        o[2 * K + c] = bf16_rne(x - bf16_value(h));
+    }
+}
+
+// B' (3K x N, ld N) = [hi ; lo ; hi] of B (K x N, ldb); blockIdx.y = row of B.
+__global__ void split_cols_kernel(const float* B, int64_t ldb, int64_t N, int K, uint16_t* Bp)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = blockIdx.y;
+    if (j >= N) return;
+    const float x = B[k * ldb + j];
+    const uint16_t h = bf16_rne(x);
+    Bp[k * N + j] = h;
+    Bp[(K + k) * N + j] = bf16_rne(x - bf16_value(h));
+    Bp[(2 * K + k) * N + j] = h;
+}
+
+size_t split_bytes(int64_t n) { return align_up((size_t)n * 3 * NBO_SPLIT * sizeof(uint16_t), 256); }
+
+// C (M x N, row-major, ldc) -= A (M x K, lda) B (K x N, ldb).  With split
+// (FP32 only: two buffers of split_bytes(max(M, N))) a large update takes the
+// split-bf16 products.
 template <typename T>
 int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
-                    hipStream_t st)
+                    uint16_t* split, hipStream_t st)
 {
     static const bool use_rb = [] {
         const char* e = std::getenv("FL_LU_ROCBLAS");
@@ -1180,6 +1236,25 @@ int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, co
         if (int e = rocblas_handle_for_device(h)) return e;
         if (rocblas_set_stream(h, st) != rocblas_status_success) return set_error(FL_EHIP, "rocblas_set_stream failed");
         const T alpha = T(-1), beta = T(1);
+        if constexpr (std::is_same_v<T, float>) {
+            if (split && K <= NBO_SPLIT) {
+                uint16_t* Ap = split;
+                uint16_t* Bp = split + split_bytes(std::max(M, N)) / sizeof(uint16_t);
+                hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)M), dim3(256), 0, st, A, lda, (int)K, Ap);
+                FL_LAUNCHED();
+                hipLaunchKernelGGL(split_cols_kernel, dim3((unsigned)div_up(N, 256), (unsigned)K), dim3(256), 0, st, B,
+                                   ldb, N, (int)K, Bp);
+                FL_LAUNCHED();
+                // column-major C^T = alpha B'^T A'^T + beta C^T, bf16 operands, FP32 C and sums
+                if (rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, (int)N, (int)M, (int)(3 * K),
+                                    &alpha, Bp, rocblas_datatype_bf16_r, (int)N, Ap, rocblas_datatype_bf16_r,
+                                    (int)(3 * K), &beta, C, rocblas_datatype_f32_r, (int)ldc, C,
+                                    rocblas_datatype_f32_r, (int)ldc, rocblas_datatype_f32_r,
+                                    rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
+                    return set_error(FL_EHIP, "rocBLAS gemm_ex (split bf16) failed");
+                return FL_OK;
+            }
+        }
         // row-major C = alpha A B + beta C  <=>  column-major C^T = alpha B^T A^T + beta C^T
         if (rb_gemm(h, (int)N, (int)M, (int)K, &alpha, B, (int)ldb, A, (int)lda, &beta, C, (int)ldc) !=
             rocblas_status_success)
@@ -1189,10 +1264,13 @@ int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, co
     return gemm_launch<T>(M, N, K, T(-1), A, lda, B, ldb, T(1), C, ldc, nullptr, st);
 }
 
+// Workspace: the sub-panel backup (n x NB), flags, and for FP32 the two
+// split-bf16 operand buffers of the trailing update.
 template <typename T>
 size_t getrf_ws_bytes(int64_t n)
 {
-    return align_up((size_t)n * NB * sizeof(T), 256) + 256;
+    const size_t base = align_up((size_t)n * NB * sizeof(T), 256) + 256;
+    return std::is_same_v<T, float> ? base + 2 * split_bytes(n) : base;
 }
 
 // End of the asynchronous factorization: status = 0 when no multiplier
@@ -1218,7 +1296,7 @@ __global__ void spec_finish_kernel(const unsigned int* lmax_bits, unsigned int* 
 // valid (else A must be refactored from a copy on the verified path).
 template <typename T>
 int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info_host, int* fallback_panels,
-               void* ws, size_t ws_bytes, hipStream_t st, unsigned int* status = nullptr)
+               void* ws, size_t ws_bytes, hipStream_t st, unsigned int* status = nullptr, bool split_products = false)
 {
     const bool spec = status != nullptr;
     FL_CHECK(n >= 0, "negative n");
@@ -1238,6 +1316,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     unsigned int* flag = reinterpret_cast<unsigned int*>(tail);      // largest |multiplier| of a panel
     int32_t* info_blk = reinterpret_cast<int32_t*>(tail + 4);        // zero pivot found by tier 1
     int32_t* info_d = reinterpret_cast<int32_t*>(tail + 64);
+    uint16_t* split = split_products && std::is_same_v<T, float> ? reinterpret_cast<uint16_t*>(tail + 256) : nullptr;
     if (spec) {
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, flag, (int64_t)1);
         FL_LAUNCHED();
@@ -1252,10 +1331,11 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         FL_LAUNCHED();
         return FL_OK;
     };
-    // Outer panels of NBO columns (the trailing update has K = NBO); each is
+    // Outer panels of nbo_w columns (the trailing update has K = nbo_w); each is
     // factored as sub-panels of NB columns.
-    for (int64_t k = 0; k < n; k += NBO) {
-        const int64_t pe = std::min<int64_t>(n, k + NBO);
+    const int64_t nbo_w = split ? NBO_SPLIT : NBO;
+    for (int64_t k = 0; k < n; k += nbo_w) {
+        const int64_t pe = std::min<int64_t>(n, k + nbo_w);
         for (int64_t ks = k; ks < pe; ks += NB) {
             const int nb = (int)std::min<int64_t>(NB, pe - ks);
             const int64_t m = n - ks;
@@ -1360,9 +1440,9 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                                                A + (ks + nb) * lda + pe, lda, nullptr, st))
                         return s;
             }
-            // trailing update with K = NBO
+            // trailing update with K = nbo_w
             if (int s = trailing_update<T>(nr, nr, nbo, A + pe * lda + k, lda, A + k * lda + pe, lda, A + pe * lda + pe,
-                                           lda, st))
+                                           lda, split, st))
                 return s;
         }
     }
@@ -1592,21 +1672,23 @@ size_t getrs_workspace_bytes(int64_t n, int64_t nrhs, int precision)
 }
 size_t refine_workspace_bytes(int64_t n, int64_t nrhs) { return refine_ws_bytes(n, nrhs); }
 
-int sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
-           hipStream_t st)
+int sgetrf(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, int products, void* ws,
+           size_t wsb, hipStream_t st)
 {
-    return getrf_impl<float>(A, n, lda, ipiv, aux, info, fb, ws, wsb, st);
+    FL_CHECK(products == FL_LU_FP32 || products == FL_LU_BF16X3, "unknown products %d", products);
+    return getrf_impl<float>(A, n, lda, ipiv, aux, info, fb, ws, wsb, st, nullptr, products == FL_LU_BF16X3);
 }
 int dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* fb, void* ws, size_t wsb,
            hipStream_t st)
 {
     return getrf_impl<double>(A, n, lda, ipiv, aux, info, fb, ws, wsb, st);
 }
-int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
-                 size_t wsb, hipStream_t st)
+int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, int products,
+                 void* ws, size_t wsb, hipStream_t st)
 {
     FL_CHECK(status, "null status");
-    return getrf_impl<float>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status);
+    FL_CHECK(products == FL_LU_FP32 || products == FL_LU_BF16X3, "unknown products %d", products);
+    return getrf_impl<float>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status, products == FL_LU_BF16X3);
 }
 int dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
                  size_t wsb, hipStream_t st)

@@ -103,7 +103,7 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
         K64, K32 = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=(precision == "mixed"),
                                    stream=stream)
         if precision == "mixed":
-            F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64))
+            F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3")
         else:
             F = linalg.getrf(K64.clone(), stream=stream, refill=lambda A: A.copy_(K64))
         if F.info > 0:

@@ -165,7 +165,8 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
                 return assemble_nearest_device(kernel, xb, Xb, eps_t, stride, order_t, start_t, N, mu=mu, fp32=True,
                                                fp64=False, stream=stream)[1]
 
-            F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()), hint=lu_hint)
+            F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()), hint=lu_hint,
+                             products="bf16x3")
             if F.info > 0:
                 raise np.linalg.LinAlgError("Singular matrix")
             factor_cache.put(key, (F,), F.nbytes())
@@ -173,7 +174,7 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
             F = item[0]
         Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream, frame=_frame_rotation(B))
     elif precision == "mixed":
-        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), hint=lu_hint)
+        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), hint=lu_hint, products="bf16x3")
         if F.info > 0:
             raise np.linalg.LinAlgError("Singular matrix")
         Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream)

@@ -188,6 +188,32 @@ int fl_sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, 
 int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* Products of the FP32 LU's large trailing updates (A22 -= L21 U12 with both
+ * dimensions >= 2048), for fl_sgetrf_ex and fl_sgetrf_async_ex.
+ *   FL_LU_FP32    FP32 MFMA products, FP32 sums (what fl_sgetrf does).
+ *   FL_LU_BF16X3  split bf16: every FP32 operand x is written as hi + lo
+ *                 (hi = bf16(x), lo = bf16(x - hi), x = hi + lo to 2^-16
+ *                 relative) and a b is taken as a_hi b_hi + a_hi b_lo + a_lo b_hi
+ *                 on bf16 MFMA with FP32 sums: about twice as fast on gfx950,
+ *                 with products good to about 2^-16 relative, for factors
+ *                 that precondition FP64 iterative refinement (fl_refine). */
+typedef enum fl_lu_products {
+    FL_LU_FP32 = 0,
+    FL_LU_BF16X3 = 1
+} fl_lu_products;
+
+/* fl_sgetrf and fl_sgetrf_async with a choice of trailing-update products
+ * (fl_lu_products).  Same arguments and workspace (fl_getrf_workspace_size
+ * with FL_FP32C covers both); FL_LU_FP32 gives exactly fl_sgetrf /
+ * fl_sgetrf_async.  Replace, like them, the LU inside scipy.linalg.solve
+ * (src/fluidlets/core/solve.py:66) and numpy.linalg.solve
+ * (src/fluidlets/swimming.py:101), here as the factorization that the
+ * mixed-precision solves refine. */
+int fl_sgetrf_ex(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info, int* general_panels,
+                 int products, void* workspace, size_t workspace_bytes, void* stream);
+int fl_sgetrf_async_ex(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                       int products, void* workspace, size_t workspace_bytes, void* stream);
+
 /* C = alpha A B + beta C, row-major (M x K, K x N, M x N), on MFMA with FP32
  * (fl_sgemm) or FP64 (fl_dgemm) accumulation; C is not read when beta == 0.
  * The LU's update kernel.  It has no counterpart call in the reference (there

@@ -168,6 +168,58 @@ def test_mixed_precision_refinement_reaches_1e_10():
     assert rel_err(x.cpu().numpy(), xref) < 1e-10
 
 
+@pytest.mark.parametrize("n_points", [1024, 1500])
+def test_bf16x3_products_lu_refines_to_the_fp64_solution(n_points):
+    """FP32 LU whose trailing updates (>= 2048 on a side) take split-bf16
+    products: a backward-stable factorization to the split's accuracy, in the
+    same number of refinement steps as FP32 products, and the refined
+    solution within 1e-10 of LAPACK's FP64 solve."""
+    from fluidlets_amd import linalg
+    import torch
+    print(f"case: sgetrf bf16x3 wall sphere N={n_points}", flush=True)
+    X = fibonacci_sphere(n_points, 1.0, (0, 0, 1.5))
+    eps = 0.5 * np.sqrt(4 * np.pi / n_points)
+    K = O.assemble("wall", X, X, eps)
+    n = 3 * n_points
+    assert n - 512 >= 2048            # at least one split trailing update (outer panels of 512)
+    b = np.random.default_rng(5).standard_normal((n, 6))
+    xref = sl.solve(K, b)
+    K64 = dev(K)
+    out = {}
+    for products in ("fp32", "bf16x3"):
+        F = linalg.getrf(K64.to(torch.float32), products=products)
+        assert F.extra["products"] == products and F.info == 0
+        LU = F.lu.cpu().numpy().astype(np.float64)
+        L, U = np.tril(LU, -1) + np.eye(n), np.triu(LU)
+        perm = F.aux.cpu().numpy()[:4 * n].view(np.int32)
+        berr = np.linalg.norm(L @ U - K[perm]) / np.linalg.norm(K)
+        x, steps, res = linalg.refine(K64, F, dev(b), tol=1e-12)
+        out[products] = (LU, berr, steps)
+        assert res.max() <= 1e-12
+        assert rel_err(x.cpu().numpy(), xref) < 1e-10
+    (lu32, berr32, steps32), (lu3, berr3, steps3) = out["fp32"], out["bf16x3"]
+    assert berr32 < 2e-6 and berr3 < 2e-5
+    assert steps3 <= steps32 + 1
+    # the split products ran: the factors differ from the FP32-product ones,
+    # by about the split's 2^-16 rather than FP32's 2^-24
+    d = np.abs(lu3 - lu32).max() / np.abs(lu32).max()
+    assert 1e-8 < d < 1e-4
+
+
+def test_bf16x3_products_are_ignored_for_fp64_and_small_matrices():
+    from fluidlets_amd import linalg
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((700, 700))
+    F64a = linalg.getrf(dev(A.copy()))
+    F64b = linalg.getrf(dev(A.copy()), products="bf16x3")
+    assert np.array_equal(F64a.lu.cpu().numpy(), F64b.lu.cpu().numpy())
+    F32a = linalg.getrf(dev(A.astype(np.float32)))
+    F32b = linalg.getrf(dev(A.astype(np.float32)), products="bf16x3")
+    assert np.array_equal(F32a.lu.cpu().numpy(), F32b.lu.cpu().numpy())
+    with pytest.raises(ValueError):
+        linalg.getrf(dev(A.copy()), products="fp16")
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_resistance_lu_matches_oracle_and_legacy_drag(precision):
     """tests/test_kernels.py:61-73 and the MATLAB sphere drags (sphere_drag.mat)."""
