@@ -44,6 +44,11 @@ FLOPS_TRANSPOSED_APPLY = {"wall": 18, "stokeslet": 17, "brinkman": 17}
 SYM_TILE = 512
 
 
+def progress(msg: str) -> None:
+    """One line per finished section or pass, on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def sym_flops(n: int, fam: str) -> float:
     """Algorithmic flops of one reciprocal evaluation over n points: pairs in
     different 512-point tiles are evaluated once and applied both ways; pairs
@@ -284,12 +289,14 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
             except Exception as exc:  # report, do not fail the bench
                 row["cpu_lapack_error"] = repr(exc)[:200]
         out[name] = row
+        progress(f"wall-sphere solve {name}: mixed {row['mixed_lu_refine_s']:.4f} s, FP64 {row['fp64_lu_s']:.4f} s")
         del K64, K32
         torch.cuda.empty_cache()
     return out
 
 
-def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4, seed=5, cache_gib=48):
+def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4, seed=5, cache_gib=48,
+                    rates=(0.9,)):
     """Serving mix (70 % six-mode resistance solves on ellipsoids, 30 %
     biflagellate RK2 swimmers), reduced in size so the default run stays
     short: n_requests requests, N <= n_max, swimmer_steps RK2 steps (50 = one
@@ -297,7 +304,9 @@ def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4,
     baseline: every step factors its matrix) and by the continuous-batching
     server (mixed precision, `slots` streams, a resident factor cache of
     cache_gib GiB that starts empty): offline (all requests queued at t = 0)
-    and online (Poisson arrivals at 0.9 x the baseline's measured capacity)."""
+    and online (Poisson arrivals at each fraction in `rates` of the
+    baseline's measured capacity; the first fraction's results are also
+    reported under the top-level online keys)."""
     from fluidlets_amd.serving import Request, SolveServer, serving_mix, summarize
 
     reqs = serving_mix(n_requests, seed=seed, n_max=n_max, swimmer_steps=swimmer_steps, rate=1.0)
@@ -309,27 +318,47 @@ def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4,
     def fresh():
         return SolveServer(slots=slots, precision="mixed", cache_bytes=int(cache_gib) << 30)
 
+    def tick(tag):
+        done = [0]
+
+        def on_result(res):
+            done[0] += 1
+            if res.kind == "swimmer" or done[0] % 10 == 0:
+                progress(f"serving: {tag} {done[0]}/{n_requests} done ({res.kind}, {res.latency:.2f} s)")
+        return on_result
+
     out = {"requests": n_requests, "slots": slots, "swimmer_steps": swimmer_steps, "n_max": n_max,
            "server_cache_gib": cache_gib}
-    off_base = summarize(base_srv.serve(reqs, realtime=False))
+    off_base = summarize(base_srv.serve(reqs, realtime=False, on_result=tick("offline baseline")))
+    progress(f"serving: offline baseline {off_base['solves_per_s']:.3f} requests/s")
     fast_srv = fresh()
-    off_fast = summarize(fast_srv.serve(reqs, realtime=False))
+    off_fast = summarize(fast_srv.serve(reqs, realtime=False, on_result=tick("offline server")))
+    progress(f"serving: offline server {off_fast['solves_per_s']:.3f} requests/s")
     off_fast["cache_hits"], off_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
     out["offline_baseline_fp64_one_at_a_time"] = off_base
     out["offline_server_mixed"] = off_fast
-    lam = 0.9 * off_base["solves_per_s"]
-    t = 0.0
-    rng = np.random.default_rng(seed + 1)
-    trace = []
-    for r in reqs:
-        t += rng.exponential(1.0 / lam)
-        trace.append(Request(r.kind, r.payload, t, r.rid))
-    out["arrival_rate_per_s"] = lam
-    out["online_baseline_fp64_one_at_a_time"] = summarize(base_srv.serve(trace, realtime=True))
-    fast_srv = fresh()
-    on_fast = summarize(fast_srv.serve(trace, realtime=True))
-    on_fast["cache_hits"], on_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
-    out["online_server_mixed"] = on_fast
+    sweep = []
+    for frac in rates:
+        lam = frac * off_base["solves_per_s"]
+        t = 0.0
+        rng = np.random.default_rng(seed + 1)
+        trace = []
+        for r in reqs:
+            t += rng.exponential(1.0 / lam)
+            trace.append(Request(r.kind, r.payload, t, r.rid))
+        on_base = summarize(base_srv.serve(trace, realtime=True, on_result=tick(f"online {frac} baseline")))
+        progress(f"serving: online {frac} x capacity, baseline p99 {on_base['latency_p99_s']:.3f} s")
+        fast_srv = fresh()
+        on_fast = summarize(fast_srv.serve(trace, realtime=True, on_result=tick(f"online {frac} server")))
+        progress(f"serving: online {frac} x capacity, server p99 {on_fast['latency_p99_s']:.3f} s")
+        on_fast["cache_hits"], on_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
+        sweep.append({"capacity_fraction": frac, "arrival_rate_per_s": lam,
+                      "baseline_fp64_one_at_a_time": on_base, "server_mixed": on_fast})
+    out["arrival_rate_per_s"] = sweep[0]["arrival_rate_per_s"]
+    out["online_baseline_fp64_one_at_a_time"] = sweep[0]["baseline_fp64_one_at_a_time"]
+    out["online_server_mixed"] = sweep[0]["server_mixed"]
+    if len(sweep) > 1:
+        out["online_sweep"] = sweep
     out["offline_throughput_speedup"] = off_fast["solves_per_s"] / off_base["solves_per_s"]
     return out
 
@@ -389,6 +418,8 @@ def main():
     ap.add_argument("--serve-swimmer-steps", type=int, default=50)
     ap.add_argument("--serve-cache-gib", type=int, default=48, help="resident factor cache of the mixed server")
     ap.add_argument("--serve-nmax", type=int, default=3000)
+    ap.add_argument("--serve-rates", default="0.9",
+                    help="comma list of online arrival rates as fractions of the baseline's capacity")
     ap.add_argument("--gmres", action="store_true", help="run the 131,072-point GMRES rod-carpet section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (the reference point for HBM byte counters)")
@@ -504,19 +535,24 @@ def main():
                     row["rel_err_vs_fp64"] = (torch.linalg.norm(uu - ref[name]) / torch.linalg.norm(ref[name])).item()
                 table[f"{name}/{prec}"] = row
         result["c2_table"] = table
+        progress("kernel-sum table done")
 
     if not args.no_solve:
         result["icosphere_c1"] = c1_section(torch, fl)
+        progress("icosphere solve done")
         result["wall_sphere_solve"] = c3_solves(torch, fl, D, tuple(args.solve_kernels.split(",")),
                                                 args.solve_points, vendor=not args.no_vendor_lu,
                                                 cpu_lapack=not args.no_cpu)
+        progress("wall-sphere solves done")
 
     if not args.no_serve:
+        rates = tuple(float(v) for v in args.serve_rates.split(",") if v.strip())
         result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax,
-                                                cache_gib=args.serve_cache_gib)
+                                                cache_gib=args.serve_cache_gib, rates=rates or (0.9,))
 
     if args.gmres:
         result["rod_carpet_gmres"] = gmres_section(torch, fl, D)
+        progress("rod-carpet GMRES done")
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(X, F, args.cpu_targets)

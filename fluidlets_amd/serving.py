@@ -162,9 +162,11 @@ class SolveServer:
             stream.synchronize()
             return out
 
-    def serve(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+    def serve(self, requests: list[Request], realtime: bool = True, on_result=None) -> list[Result]:
         """Serve a trace.  With realtime=True requests become visible at their
-        arrival times (Poisson traces); otherwise all are queued at t = 0."""
+        arrival times (Poisson traces); otherwise all are queued at t = 0.
+        ``on_result(result)``, if given, is called from the worker thread as
+        each request completes (not for requests rejected at admission)."""
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         results: list[Result] = []
         lock = threading.Condition()
@@ -182,12 +184,15 @@ class SolveServer:
             except Exception as exc:  # a failing request is reported, the server keeps running
                 err = f"{type(exc).__name__}: {exc}"
             end = now()
+            res = Result(req.rid, req.kind, value, req.arrival if realtime else 0.0, start, end, err)
             with lock:
-                results.append(Result(req.rid, req.kind, value, req.arrival if realtime else 0.0, start, end, err))
+                results.append(res)
                 state["in_use"] -= fp
                 state["running"] -= 1
                 free_streams.append(sid)
                 lock.notify_all()
+            if on_result is not None:
+                on_result(res)
 
         state["t0"] = time.perf_counter()
         waiting: list = []
