@@ -42,6 +42,10 @@ _SIGNATURES = {
                           c_size_t, c_void_p]),
     "fl_sgetrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
+    "fl_sgetri": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64, c_void_p]),
+    "fl_refine_inv": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                              c_int64, c_double, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t,
+                              c_void_p]),
     "fl_sgetrf_ex": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, POINTER(c_int), POINTER(c_int), c_int,
                              c_void_p, c_size_t, c_void_p]),
     "fl_sgetrf_async_ex": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

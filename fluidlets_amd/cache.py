@@ -1,7 +1,7 @@
 """Byte-bounded LRU of device-resident matrices and factors.
 
 The solve server keeps the assembled FP64 matrix and the factors of each
-rigid-body geometry here, and the free swimmer keeps the FP32 factors of its
+rigid-body geometry here, and the free swimmer keeps the FP32 inverse of its
 body-frame matrix for every body shape it has met (a periodic beat revisits
 the same shapes), so repeated right-hand sides and repeated shapes cost only
 triangular solves and refinement.
@@ -22,6 +22,72 @@ class FactorCache:
         self.lock = threading.Lock()
         self.hits = 0
         self.misses = 0
+        self.inflight: dict = {}   # key (or (key, "upgrade")) -> Event of the thread making it
+        self.key_hits: dict = {}   # hits per stored key since it was stored
+
+    def get_or_make(self, key, make, made: list | None = None):
+        """The item for `key`, made once: on a miss ``make()`` returns
+        ``(item, nbytes)`` and is stored; a caller that finds another thread
+        making the same key waits for it instead of making it again (and
+        makes it itself if that attempt failed or the item was not kept).
+        ``make`` must leave the item complete (its device work finished)
+        before returning, since callers use it on their own streams.  With
+        ``made`` (a list), True is appended when this call made the item."""
+        while True:
+            with self.lock:
+                item = self.items.get(key)
+                if item is not None:
+                    self.items.move_to_end(key)
+                    self.hits += 1
+                    self.key_hits[key] = self.key_hits.get(key, 0) + 1
+                    if made is not None:
+                        made.append(False)
+                    return item
+                ev = self.inflight.get(key)
+                if ev is None:
+                    self.misses += 1
+                    ev = threading.Event()
+                    self.inflight[key] = ev
+                    break
+            ev.wait()   # then look again: stored (a hit), or the maker failed (make it here)
+        try:
+            item, nbytes = make()
+            self.put(key, item, nbytes)
+            if made is not None:
+                made.append(True)
+            with self.lock:
+                stored = self.items.get(key)   # the stored tuple itself, for upgrade()'s identity check
+            same = stored is not None and len(stored) == len(item) + 1 and all(a is b for a, b in zip(stored, item))
+            return stored if same else (*item, nbytes)
+        finally:
+            with self.lock:
+                self.inflight.pop(key, None)
+            ev.set()
+
+    def upgrade(self, key, old, make):
+        """Replace the stored item `old` (as returned for `key`) by
+        ``make()`` -> ``(item, nbytes)``, once: if another thread is already
+        upgrading this key, or the stored item is no longer `old`, return the
+        current item (or `old` if the key is gone) without waiting.  ``make``
+        must leave its item complete before returning."""
+        tag = (key, "upgrade")
+        with self.lock:
+            cur = self.items.get(key)
+            if cur is not old or tag in self.inflight:
+                return cur if cur is not None else old
+            ev = threading.Event()
+            self.inflight[tag] = ev
+        try:
+            item, nbytes = make()
+            with self.lock:
+                keep = self.items.get(key) is old
+            if keep:
+                self.put(key, item, nbytes)
+            return (*item, nbytes)
+        finally:
+            with self.lock:
+                self.inflight.pop(tag, None)
+            ev.set()
 
     def get(self, key):
         with self.lock:
@@ -43,7 +109,8 @@ class FactorCache:
             if old is not None:
                 self.bytes -= old[-1]
             while self.bytes + nbytes > self.max_bytes and self.items:
-                _, ev = self.items.popitem(last=False)
+                gone, ev = self.items.popitem(last=False)
                 self.bytes -= ev[-1]
+                self.key_hits.pop(gone, None)
             self.items[key] = (*item, nbytes)
             self.bytes += nbytes

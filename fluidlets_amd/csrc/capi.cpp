@@ -76,9 +76,11 @@ int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, i
            size_t wsb, hipStream_t st);
 int dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb, void* ws,
            size_t wsb, hipStream_t st);
+int getri_impl(const float* LU, int64_t n, int64_t lda, const void* aux, float* X, int64_t ldx, hipStream_t st);
 int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux, int64_t n,
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
-                double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st);
+                double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st,
+                const float* inv32 = nullptr, int64_t ldi = 0);
 
 size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision, int block_points);
 int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double* eps, int64_t eps_stride,
@@ -266,6 +268,22 @@ int fl_refine_frame(const double* A64, int64_t lda64, const float* LU32, int64_t
     if (!frame) return set_error(FL_EINVAL, "fl_refine_frame needs a frame");
     return refine_impl(A64, lda64, LU32, lda32, aux32, n, b, x, nrhs, ldb, tol, max_iter, iters_out, resid_out,
                        frame, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_sgetri(const float* LU, int64_t n, int64_t lda, const void* aux, float* Ainv, int64_t ldi, void* stream)
+{
+    clear_error();
+    return getri_impl(LU, n, lda, aux, Ainv, ldi, as_stream(stream));
+}
+
+int fl_refine_inv(const double* A64, int64_t lda64, const float* Ainv32, int64_t ldi, int64_t n, const double* frame,
+                  const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
+                  double* resid_out, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (!Ainv32 && n > 0) return set_error(FL_EINVAL, "fl_refine_inv needs the inverse");
+    return refine_impl(A64, lda64, nullptr, 0, nullptr, n, b, x, nrhs, ldb, tol, max_iter, iters_out, resid_out,
+                       frame, workspace, workspace_bytes, as_stream(stream), Ainv32, ldi);
 }
 
 size_t fl_gmres_workspace_size(int64_t N, int restart, int inner_precision)

@@ -968,8 +968,8 @@ struct Frame3 {
     double r[9];
 };
 
-// dst[i][q] = inv_scale[q] * (R^T src)[perm[i]][q]; rows are point-major
-// (row 3p + a is component a of point p).
+// dst[i][q] = inv_scale[q] * (R^T src)[perm[i]][q] (perm == nullptr: the
+// identity); rows are point-major (row 3p + a is component a of point p).
 __global__ __launch_bounds__(256) void gather_rot_kernel(const double* src, float* dst, int64_t n,
                                                          int nrhs, const int32_t* perm, const double* inv_scale,
                                                          Frame3 R)
@@ -978,7 +978,7 @@ __global__ __launch_bounds__(256) void gather_rot_kernel(const double* src, floa
     if (idx >= n * nrhs) return;
     const int64_t i = idx / nrhs;
     const int q = (int)(idx % nrhs);
-    const int64_t s = perm[i];
+    const int64_t s = perm ? perm[i] : i;
     const int64_t p3 = s - s % 3;
     const int a = (int)(s % 3);
     const double v = fma(R.r[a], src[p3 * nrhs + q],
@@ -1067,6 +1067,70 @@ __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t 
         const double s = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
         if (q < nrhs && row0 + i < n) r[(row0 + i) * nrhs + q] = b[(row0 + i) * ldb + q] - s;
     }
+}
+
+// y = X t for an explicit FP32 inverse X (n x n, ldx) and nrhs <= 8 FP32
+// columns t, y (ld nrhs): the correction of a refinement step with an
+// explicit inverse.  One workgroup per RES_ROWS rows, columns split over the
+// lanes, the four waves' partial sums combined in a fixed order.  Bound by
+// reading X once (4 n^2 bytes).
+__global__ __launch_bounds__(256) void inv_apply_kernel(const float* X, int64_t ldx, int64_t n, const float* t,
+                                                        float* y, int nrhs)
+{
+    __shared__ float part[4][RES_ROWS * 8];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int64_t row0 = (int64_t)blockIdx.x * RES_ROWS;
+    float acc[RES_ROWS][8];
+#pragma unroll
+    for (int i = 0; i < RES_ROWS; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[i][q] = 0.0f;
+    for (int64_t c = threadIdx.x; c < n; c += 256) {
+        float tv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tv[q] = q < nrhs ? t[c * nrhs + q] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < RES_ROWS; ++i) {
+            const float a = row0 + i < n ? X[(row0 + i) * ldx + c] : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[i][q] = fmaf(a, tv[q], acc[i][q]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RES_ROWS; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            float v = acc[i][q];
+#pragma unroll
+            for (int off = 32; off > 0; off /= 2) v += __shfl_xor(v, off);
+            acc[i][q] = v;
+        }
+    if (lane < RES_ROWS * 8) {
+        const int i = lane / 8, q = lane % 8;
+        float v = 0.0f;
+#pragma unroll
+        for (int ii = 0; ii < RES_ROWS; ++ii)
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq)
+                if (ii == i && qq == q) v = acc[ii][qq];
+        part[wave][lane] = v;
+    }
+    __syncthreads();
+    const int tt = threadIdx.x;
+    if (tt < RES_ROWS * 8) {
+        const int i = tt / 8, q = tt % 8;
+        const float v = ((part[0][tt] + part[1][tt]) + part[2][tt]) + part[3][tt];
+        if (q < nrhs && row0 + i < n) y[(row0 + i) * nrhs + q] = v;
+    }
+}
+
+// X = P (the permutation of the factors: row i holds a one in column perm[i]).
+__global__ __launch_bounds__(256) void perm_identity_kernel(float* X, int64_t ldx, int64_t n, const int32_t* perm)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n * n) return;
+    const int64_t i = idx / n, j = idx % n;
+    X[i * ldx + j] = j == perm[i] ? 1.0f : 0.0f;
 }
 
 // Per column q of an n x nrhs (ld) array: out[q] = sum_i a_iq^2 and
@@ -1556,8 +1620,10 @@ size_t refine_ws_bytes(int64_t n, int64_t nrhs)
 
 int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux_c, int64_t n,
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
-                double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st)
+                double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st,
+                const float* inv32 = nullptr, int64_t ldi = 0)
 {
+    FL_CHECK(!inv32 || ldi >= n, "ldi < n");
     FL_CHECK(n >= 0 && nrhs >= 1 && nrhs <= 8, "need 1 <= nrhs <= 8");
     FL_CHECK(ldb >= nrhs, "ldb < nrhs");
     FL_CHECK(!frame || n % 3 == 0, "a frame needs n = 3 x points");
@@ -1623,15 +1689,24 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
         // d = LU32^{-1} (r / s) s, with s = max |r| per column so FP32 sees O(1) values
         hipLaunchKernelGGL(scale_from_norms_kernel, dim3(1), dim3(8), 0, st, (const double*)(it == 0 ? bnorms : norms),
                            scale, inv_scale, q);
+        // with an explicit inverse (which includes the row permutation) the
+        // scaled residual goes to tmp unpermuted and y = inv32 tmp
+        const int32_t* perm = inv32 ? nullptr : (const int32_t*)FactorAux<float>::perm(aux);
+        float* yin = inv32 ? tmp : y;
         if (frame)
-            hipLaunchKernelGGL(gather_rot_kernel, dim3(g), dim3(256), 0, st, (const double*)r, y, n, q,
-                               (const int32_t*)FactorAux<float>::perm(aux), (const double*)inv_scale, R);
+            hipLaunchKernelGGL(gather_rot_kernel, dim3(g), dim3(256), 0, st, (const double*)r, yin, n, q, perm,
+                               (const double*)inv_scale, R);
         else
             hipLaunchKernelGGL((gather_rows_kernel<double, float>), dim3(g), dim3(256), 0, st, (const double*)r,
-                               (int64_t)nrhs, y, n, q, (const int32_t*)FactorAux<float>::perm(aux),
-                               (const double*)inv_scale);
+                               (int64_t)nrhs, yin, n, q, perm, (const double*)inv_scale);
         FL_LAUNCHED();
-        if (int s = solve_permuted<float>(LU32, n, lda32, aux, y, tmp, q, sync, st)) return s;
+        if (inv32) {
+            hipLaunchKernelGGL(inv_apply_kernel, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, inv32, ldi, n,
+                               (const float*)tmp, y, q);
+            FL_LAUNCHED();
+        } else if (int s = solve_permuted<float>(LU32, n, lda32, aux, y, tmp, q, sync, st)) {
+            return s;
+        }
         if (frame)
             hipLaunchKernelGGL(axpy_rot_kernel, dim3(g), dim3(256), 0, st, xc, (const float*)y, n, q,
                                (const double*)scale, R);
@@ -1647,13 +1722,62 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     hipLaunchKernelGGL((copy_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, (const double*)xc, x, ldb, n, q);
     FL_LAUNCHED();
     unsigned sync_err[2] = {0u, 0u};
-    const int64_t half = div_up(n, NB) + 2;
-    FL_HIP(hipMemcpyAsync(&sync_err[0], sync + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    FL_HIP(hipMemcpyAsync(&sync_err[1], sync + half + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    if (!inv32 && it > 0) {   // the chained solves' wait flags (an explicit inverse has none)
+        const int64_t half = div_up(n, NB) + 2;
+        FL_HIP(hipMemcpyAsync(&sync_err[0], sync + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        FL_HIP(hipMemcpyAsync(&sync_err[1], sync + half + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    }
     FL_HIP(hipStreamSynchronize(st));
     if (iters_out) *iters_out = it;
     if (sync_err[0] | sync_err[1]) return set_error(FL_EHIP, "triangular solve: a block wait timed out");
     if (!(worst <= tol)) return set_error(FL_ENOCONV, "refinement reached %.3e after %d steps (tol %.1e)", worst, it, tol);
+    return FL_OK;
+}
+
+// Explicit inverse from the FP32 factors of P A = L U: X = U^{-1} L^{-1} P,
+// by blocked substitution on X = P.  Forward, block row p = 0, 1, ...:
+//   X_p = inv(L_pp) X_p,  X_below -= L_below,p X_p;
+// backward, p = last, ..., 0:
+//   X_p = inv(U_pp) X_p,  X_above -= U_above,p X_p,
+// with the inverted diagonal blocks kept in aux and the LU's update GEMMs
+// (2 n^3 flop).  The refinement then corrects with one pass over X
+// (inv_apply_kernel) instead of two chained triangular sweeps, which pays
+// when one factorization serves many solves.
+int getri_impl(const float* LU, int64_t n, int64_t lda, const void* aux_c, float* X, int64_t ldx, hipStream_t st)
+{
+    FL_CHECK(n >= 0, "negative n");
+    FL_CHECK(lda >= n && ldx >= n, "leading dimension < n");
+    if (n == 0) return FL_OK;
+    FL_CHECK(LU && aux_c && X, "null pointer argument");
+    void* aux = const_cast<void*>(aux_c);
+    hipLaunchKernelGGL(perm_identity_kernel, dim3((unsigned)div_up(n * n, 256)), dim3(256), 0, st, X, ldx, n,
+                       (const int32_t*)FactorAux<float>::perm(aux));
+    FL_LAUNCHED();
+    const int64_t nblk = div_up(n, NB);
+    for (int64_t p = 0; p < nblk; ++p) {
+        const int64_t k0 = p * NB;
+        const int nb = (int)std::min<int64_t>(NB, n - k0);
+        float* Xp = X + k0 * ldx;
+        // in place: M = nb <= 128 rows, C on B's memory
+        if (int s = gemm_launch<float>(nb, n, nb, 1.0f, FactorAux<float>::invL(aux, n, p), NB, Xp, ldx, 0.0f, Xp, ldx,
+                                       nullptr, st))
+            return s;
+        const int64_t below = n - k0 - nb;
+        if (below > 0)
+            if (int s = trailing_update<float>(below, n, nb, LU + (k0 + nb) * lda + k0, lda, Xp, ldx, Xp + nb * ldx, ldx,
+                                               nullptr, st))
+                return s;
+    }
+    for (int64_t p = nblk - 1; p >= 0; --p) {
+        const int64_t k0 = p * NB;
+        const int nb = (int)std::min<int64_t>(NB, n - k0);
+        float* Xp = X + k0 * ldx;
+        if (int s = gemm_launch<float>(nb, n, nb, 1.0f, FactorAux<float>::invU(aux, n, p), NB, Xp, ldx, 0.0f, Xp, ldx,
+                                       nullptr, st))
+            return s;
+        if (k0 > 0)
+            if (int s = trailing_update<float>(k0, n, nb, LU + k0, lda, Xp, ldx, X, ldx, nullptr, st)) return s;
+    }
     return FL_OK;
 }
 

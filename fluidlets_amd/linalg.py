@@ -124,12 +124,27 @@ def getrs(F: LUFactor, B, stream=None):
     return B
 
 
-def refine(A64, F32: LUFactor, b, tol: float = 1e-10, max_iter: int = 30, stream=None, frame=None):
+def getri(F: LUFactor, stream=None):
+    """Explicit inverse (n x n float32 device tensor) from FP32 factors
+    (fl_sgetri): for many refinements against one factorization."""
+    t = D.torch()
+    if F.precision != FL_FP32C:
+        raise ValueError("getri needs FP32 factors")
+    n = F.n
+    X = t.empty((n, n), dtype=t.float32, device=F.lu.device)
+    check(lib().fl_sgetri(D.ptr(F.lu), n, n, D.ptr(F.aux), D.ptr(X), n, D.stream_handle(stream)), "getri")
+    return X
+
+
+def refine(A64, F32: LUFactor | None, b, tol: float = 1e-10, max_iter: int = 30, stream=None, frame=None,
+           inverse=None):
     """x solving A64 x = b (FP64) from the FP32 factors by iterative
     refinement with FP64 residuals.  b: (n, nrhs) float64 device tensor.
     With `frame` (3x3, lab = frame @ body) the factors are those of the
     operator written in the body frame (A64 ~ R A' R^T per point), and every
-    correction is rotated accordingly (fl_refine_frame).
+    correction is rotated accordingly (fl_refine_frame).  With `inverse` (an
+    explicit FP32 inverse from ``getri``; F32 may then be None) each
+    correction is one product with it (fl_refine_inv).
     Returns (x, iterations, per-column final relative residuals)."""
     t = D.torch()
     L = lib()
@@ -146,7 +161,12 @@ def refine(A64, F32: LUFactor, b, tol: float = 1e-10, max_iter: int = 30, stream
         ws = D.workspace(nbytes, stream)
         it = ctypes.c_int(0)
         res = (ctypes.c_double * 8)()
-        if frame is None:
+        if inverse is not None:
+            R = None if frame is None else np.ascontiguousarray(np.asarray(frame, dtype=np.float64).reshape(3, 3))
+            st = L.fl_refine_inv(D.ptr(A64), n, D.ptr(inverse), int(inverse.stride(0)), n,
+                                 None if R is None else R.ctypes.data, D.ptr(bb), D.ptr(xx), q, q, float(tol),
+                                 int(max_iter), ctypes.byref(it), res, D.ptr(ws), nbytes, D.stream_handle(stream))
+        elif frame is None:
             st = L.fl_refine(D.ptr(A64), n, D.ptr(F32.lu), n, D.ptr(F32.aux), n, D.ptr(bb), D.ptr(xx), q, q,
                              float(tol), int(max_iter), ctypes.byref(it), res, D.ptr(ws), nbytes,
                              D.stream_handle(stream))

@@ -13,10 +13,11 @@ Resident factors: resistance requests carry a geometry key; the assembled
 FP64 matrix and the factors stay on the device in an LRU cache (counted in
 the memory budget), so another right-hand side on the same geometry costs
 only the triangular solves (and refinement).  Mixed-precision swimmers keep
-the FP32 factors of each body shape they meet (in that cache when the server
-has one, else in a cache of their own, counted in their footprint), so the
-steps of a periodic beat after the first period cost only the lab-frame
-assembly, refinement solves and the 6 x 6 Schur step.
+the FP32 inverse of each body shape's matrix they meet (in that cache when the
+server has one, else in a cache of their own, counted in their footprint), so
+the steps of a periodic beat after the first period cost only the lab-frame
+assembly, refinement with one pass over the inverse per correction, and the
+6 x 6 Schur step.
 
 Latency is measured from a request's arrival to its completion (queueing
 included); the server reports p50/p99 and throughput.
@@ -62,17 +63,20 @@ class Result:
 def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) -> int:
     """HBM bytes a request holds while it runs: the FP64 matrix (8 n^2), the
     FP32 copy for the mixed path (4 n^2), plus O(n) vectors and workspaces;
-    a mixed-precision swimmer without the server's shared cache also holds
-    its own cache of body-frame factors."""
+    a mixed-precision swimmer also holds the FP32 inverse it makes next to
+    its factors (4 n^2) and, without the server's shared cache, its own cache
+    of body-frame inverses."""
     if req.kind == "resistance":
         n = 3 * len(req.payload["points"])
     else:
         n = 3 * req.payload["n_points"]
     per = 8 + (4 if precision == "mixed" else 8)
     extra = 0
+    if req.kind == "swimmer" and precision == "mixed":
+        extra += 4 * n * n
     if req.kind == "swimmer" and precision == "mixed" and not shared_cache:
         from .swimming import reuse_cache_bytes
-        extra = reuse_cache_bytes(req.payload["n_points"])   # the swimmer's own factor cache
+        extra += reuse_cache_bytes(req.payload["n_points"])   # the swimmer's own cache
     return int(per * n * n + 512 * n + (64 << 20) + extra)
 
 
@@ -247,6 +251,9 @@ def summarize(results: list[Result]) -> dict:
         "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
         "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
         "latency_mean_s": float(lat.mean()) if len(lat) else None,
+        # the three slowest requests: (request id, kind, wait before start, latency)
+        "slowest": [(r.rid, r.kind, round(r.start - r.arrival, 4), round(r.latency, 4))
+                    for r in sorted(ok, key=lambda r: -r.latency)[:3]],
     }
 
 

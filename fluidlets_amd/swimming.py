@@ -21,9 +21,14 @@ periodic beat revisits.  They are kept in a FactorCache keyed by the body
 shape; every step still assembles the exact lab-frame FP64 K and refines
 against it (fl_refine_frame rotates each correction by the frame's
 orthogonal polar factor), so the answer is the lab-frame solution to the
-refinement tolerance whether the factors were reused or not.  On a miss the
-factors are made the same way (from the body-frame matrix), so a step's
-result does not depend on the cache's contents.
+refinement tolerance whether the factors were reused or not.  When a shape
+comes back a second time (its third visit), its factors are traded for the
+explicit FP32 inverse (fl_sgetri, the same memory), so every later
+correction is one pass over it (fl_refine_inv) instead of two chained
+triangular sweeps.  Up to then the factors serve, since the inverse costs
+40 % of a factorization more.  Concurrent swimmers sharing a cache make each shape once
+(FactorCache.get_or_make).  A step's result therefore depends on the cache's
+history only within the refinement tolerance.
 """
 from __future__ import annotations
 
@@ -157,8 +162,8 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
         Y = linalg.getrs(F, R, stream=stream)
     elif precision == "mixed" and body:
         key = _body_key(kernel, shape, nearest, e, mu)
-        item = factor_cache.get(key)
-        if item is None:
+
+        def make_factors():
             xb, Xb = D.soa(np.asarray(shape.x, float), dev), D.soa(np.asarray(shape.X, float), dev)
 
             def body32():
@@ -166,13 +171,28 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
                                                fp64=False, stream=stream)[1]
 
             F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()), hint=lu_hint,
-                             products="bf16x3")
+                             products="bf16x3")        # returns with the factors complete
             if F.info > 0:
                 raise np.linalg.LinAlgError("Singular matrix")
-            factor_cache.put(key, (F,), F.nbytes())
-        else:
-            F = item[0]
-        Y, _, _ = linalg.refine(K64, F, R, tol=1e-13, max_iter=30, stream=stream, frame=_frame_rotation(B))
+            return (F, None), F.nbytes()
+
+        made: list = []
+        item = factor_cache.get_or_make(key, make_factors, made)
+        if item[1] is None and not made[0] and factor_cache.key_hits.get(key, 0) >= 2:
+            # the shape keeps coming back (third visit): trade its factors for
+            # the explicit inverse, so this and later visits correct with one
+            # pass over it instead of two chained triangular sweeps
+            F0 = item[0]
+
+            def make_inverse():
+                X_inv = linalg.getri(F0, stream=stream)
+                (stream if stream is not None else t.cuda.current_stream()).synchronize()   # complete before shared
+                return (None, X_inv), int(X_inv.numel() * X_inv.element_size())
+
+            item = factor_cache.upgrade(key, item, make_inverse)
+        F, Xinv = item[0], item[1]
+        Y, _, _ = linalg.refine(K64, None if Xinv is not None else F, R, tol=1e-13, max_iter=30, stream=stream,
+                                frame=_frame_rotation(B), inverse=Xinv)
     elif precision == "mixed":
         F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), hint=lu_hint, products="bf16x3")
         if F.info > 0:
@@ -236,7 +256,7 @@ def swim(shape_at: Callable[[float], SwimmerShape], t_span, eps, mu=1.0, x0=(0, 
 
 
 def reuse_cache_bytes(n_points: int) -> int:
-    """Bound of a swimmer's own factor cache: 128 body shapes' FP32 factors
+    """Bound of a swimmer's own factor cache: 128 body shapes' FP32 inverses
     (two solves per step, so a beat of 64 steps or fewer fits), at most 16 GiB."""
     n = 3 * int(n_points)
     return int(min(16 << 30, 128 * (4 * n * n + (1 << 20))))

@@ -188,6 +188,26 @@ int fl_sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, 
 int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* Explicit inverse from the factors of fl_sgetrf: Ainv = A^{-1} =
+ * U^{-1} L^{-1} P (n x n row-major, ldi >= n), by blocked substitution on the
+ * permuted identity with the stored inverted diagonal blocks and the LU's
+ * update GEMMs (2 n^3 flop).  For callers that refine many right-hand sides
+ * against one factorization: fl_refine_inv then corrects with one pass over
+ * Ainv instead of two chained triangular sweeps.  No counterpart in the
+ * reference (numpy.linalg.solve at src/fluidlets/swimming.py:101 refactors
+ * every call); used for the swimmer's body-frame factor reuse. */
+int fl_sgetri(const float* LU, int64_t n, int64_t lda, const void* aux, float* Ainv, int64_t ldi, void* stream);
+
+/* fl_refine (frame == NULL) or fl_refine_frame with the corrections taken
+ * from an explicit FP32 inverse (fl_sgetri) of the operator:
+ *   x <- x + s Ainv32 (r / s)      (rotated by the frame when given).
+ * Same arguments, workspace (fl_refine_workspace_size), stopping rule and
+ * results as fl_refine.  Replaces, like fl_refine, the dense solve of
+ * src/fluidlets/swimming.py:101 (numpy.linalg.solve). */
+int fl_refine_inv(const double* A64, int64_t lda64, const float* Ainv32, int64_t ldi, int64_t n, const double* frame,
+                  const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
+                  double* resid_out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Products of the FP32 LU's large trailing updates (A22 -= L21 U12 with both
  * dimensions >= 2048), for fl_sgetrf_ex and fl_sgetrf_async_ex.
  *   FL_LU_FP32    FP32 MFMA products, FP32 sums (what fl_sgetrf does).

@@ -220,6 +220,51 @@ def test_bf16x3_products_are_ignored_for_fp64_and_small_matrices():
         linalg.getrf(dev(A.copy()), products="fp16")
 
 
+@pytest.mark.parametrize("n", [1, 129, 300, 777, 2600])
+def test_sgetri_matches_lapack_float32_inverse(n):
+    """fl_sgetri on partial-pivoting FP32 factors of a random matrix (rows
+    interchanged everywhere): A X = I as closely as LAPACK's own FP32 inverse
+    (sgetrf + sgetri through scipy) achieves, and X within the conditioning
+    of the FP64 inverse."""
+    from fluidlets_amd import linalg
+    print(f"case: sgetri random n={n}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    F = linalg.getrf(dev(A.copy()))
+    X = linalg.getri(F).cpu().numpy().astype(np.float64)
+    A64 = A.astype(np.float64)
+    X_lapack = sl.inv(A).astype(np.float64)
+    I = np.eye(n)
+    ours = np.linalg.norm(A64 @ X - I) / np.sqrt(n)
+    lapack = np.linalg.norm(A64 @ X_lapack - I) / np.sqrt(n)
+    assert ours <= 10 * lapack + 1e-6
+    assert rel_err(X, np.linalg.inv(A64)) < 1e-7 * np.linalg.cond(A64) * np.sqrt(n)
+
+
+def test_refine_with_explicit_inverse_matches_lu_refinement():
+    """fl_refine_inv: corrections from the explicit FP32 inverse reach the
+    FP64 solution as the LU corrections do, in about as many steps."""
+    from fluidlets_amd import linalg
+    import torch
+    print("case: refine with explicit inverse, wall sphere", flush=True)
+    n_points = 900
+    X = fibonacci_sphere(n_points, 1.0, (0, 0, 1.5))
+    eps = 0.5 * np.sqrt(4 * np.pi / n_points)
+    K = O.assemble("wall", X, X, eps)
+    b = np.random.default_rng(8).standard_normal((3 * n_points, 7))
+    xref = sl.solve(K, b)
+    K64 = dev(K)
+    F = linalg.getrf(K64.to(torch.float32))
+    x_lu, steps_lu, res_lu = linalg.refine(K64, F, dev(b), tol=1e-12)
+    Xinv = linalg.getri(F)
+    x_inv, steps_inv, res_inv = linalg.refine(K64, None, dev(b), tol=1e-12, inverse=Xinv)
+    assert res_lu.max() <= 1e-12 and res_inv.max() <= 1e-12
+    assert rel_err(x_lu.cpu().numpy(), xref) < 1e-10 and rel_err(x_inv.cpu().numpy(), xref) < 1e-10
+    assert steps_inv <= steps_lu + 1
+    with pytest.raises(RuntimeError):
+        linalg.refine(K64, None, dev(b), tol=1e-12, inverse=torch.zeros_like(Xinv), max_iter=3)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_resistance_lu_matches_oracle_and_legacy_drag(precision):
     """tests/test_kernels.py:61-73 and the MATLAB sphere drags (sphere_drag.mat)."""

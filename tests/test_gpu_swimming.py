@@ -169,6 +169,12 @@ def test_refine_with_body_frame_factors(delta):
     ref = np.linalg.solve(K_lab, b)
     assert rel_err(x.cpu().numpy(), ref) < 1e-12 and res.max() <= 1e-13
     assert iters <= 6
+    # the same corrections from the explicit inverse of the body-frame factors
+    Xinv = linalg.getri(F)
+    xi, iters_i, res_i = linalg.refine(K64, None, torch.from_numpy(b).to(dev), tol=1e-13, max_iter=30,
+                                       frame=_frame_rotation(B), inverse=Xinv)
+    assert rel_err(xi.cpu().numpy(), ref) < 1e-12 and res_i.max() <= 1e-13
+    assert iters_i <= iters + 1
     # without the rotation the body-frame factors do not precondition the lab system
     if delta == 0.0:
         with pytest.raises(RuntimeError):
@@ -177,9 +183,10 @@ def test_refine_with_body_frame_factors(delta):
 
 def test_factor_reuse_over_a_periodic_beat():
     """swim_rk2 ("mixed", Stokeslet3D) with dt = T/4: after the first beat the
-    body shapes repeat and their factors are reused; the trajectory equals
-    the one that factors every step to 1e-11, and a step's result is the
-    same whether its factors came from the cache or were just made."""
+    body shapes repeat and their factors, then their explicit inverses, are
+    reused; the trajectory equals the one that factors every step to 1e-11,
+    and a step's result is the same to 1e-12 whether it refined with
+    just-made factors or a cached inverse."""
     from fluidlets_amd.cache import FactorCache
     from fluidlets_amd.swimming import swimming_problem
     print("case: swimmer factor reuse", flush=True)
@@ -195,7 +202,9 @@ def test_factor_reuse_over_a_periodic_beat():
     hits = cache.hits
     warm = swimming_problem(shape_at(0.25), x0, B, 0.02, precision="mixed", factor_cache=cache)
     assert cache.hits == hits + 1
-    assert np.array_equal(cold.U, warm.U) and np.array_equal(cold.forces, warm.forces)
+    assert any(v[0] is None and v[1] is not None for v in cache.items.values())   # inverses in use
+    assert rel_err(cold.U, warm.U) < 1e-12 and rel_err(cold.Omega, warm.Omega) < 1e-12
+    assert rel_err(cold.forces, warm.forces) < 1e-12
     plain = swimming_problem(shape_at(0.25), x0, B, 0.02, precision="fp64")
     assert rel_err(warm.U, plain.U) < 1e-10 and rel_err(warm.Omega, plain.Omega) < 1e-10
     assert rel_err(warm.forces, plain.forces) < 1e-10

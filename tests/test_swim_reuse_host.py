@@ -60,3 +60,88 @@ def test_factor_cache_replaces_a_key_without_leaking_bytes():
     assert c.bytes == 50 and c.items["a"][0] == 2
     c.put("b", (3,), 60)
     assert "a" not in c.items and c.bytes == 60
+
+
+def test_factor_cache_makes_each_key_once_across_threads():
+    """get_or_make: concurrent callers of one key wait for the first maker
+    (one make, the rest hits); if that maker fails, its caller sees the
+    error and one waiter makes the item instead."""
+    import threading
+    import time
+    from fluidlets_amd.cache import FactorCache
+    c = FactorCache(1000)
+    calls = []
+
+    def make():
+        calls.append(1)
+        time.sleep(0.05)
+        return ("v",), 10
+
+    out = []
+    ths = [threading.Thread(target=lambda: out.append(c.get_or_make("k", make))) for _ in range(8)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert len(calls) == 1 and c.misses == 1 and c.hits == 7
+    assert all(o[0] == "v" and o[-1] == 10 for o in out)
+
+    c2 = FactorCache(1000)
+    attempts = []
+
+    def flaky():
+        attempts.append(1)
+        time.sleep(0.02)
+        if len(attempts) == 1:
+            raise RuntimeError("first attempt fails")
+        return ("w",), 5
+
+    res = []
+
+    def run():
+        try:
+            res.append(c2.get_or_make("k", flaky)[0])
+        except RuntimeError:
+            res.append("err")
+
+    ths = [threading.Thread(target=run) for _ in range(4)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert len(attempts) == 2 and sorted(res) == ["err", "w", "w", "w"]
+
+
+def test_factor_cache_upgrade_replaces_an_item_once():
+    """upgrade: the first caller replaces the stored item; a caller holding a
+    stale item gets the current one; a caller arriving during an upgrade
+    keeps its item without waiting."""
+    import threading
+    from fluidlets_amd.cache import FactorCache
+    c = FactorCache(1000)
+    made = []
+    old = c.get_or_make("k", lambda: (("lu", None), 10), made)
+    assert made == [True] and old[:2] == ("lu", None)
+    new = c.upgrade("k", old, lambda: ((None, "inv"), 10))
+    assert new[:2] == (None, "inv") and c.items["k"][:2] == (None, "inv") and c.bytes == 10
+    assert c.upgrade("k", old, lambda: ((None, "again"), 10))[:2] == (None, "inv")   # stale: current item
+    made2 = []
+    assert c.get_or_make("k", lambda: (("x", None), 1), made2)[:2] == (None, "inv") and made2 == [False]
+
+    c2 = FactorCache(1000)
+    first = c2.get_or_make("k", lambda: (("lu", None), 10))
+    started, release = threading.Event(), threading.Event()
+
+    def slow():
+        started.set()
+        release.wait()
+        return (None, "inv"), 10
+
+    out = []
+    th = threading.Thread(target=lambda: out.append(c2.upgrade("k", first, slow)))
+    th.start()
+    started.wait()
+    assert c2.upgrade("k", first, lambda: ((None, "dup"), 10)) is first   # in flight: no wait
+    release.set()
+    th.join()
+    assert out[0][:2] == (None, "inv") and c2.items["k"][:2] == (None, "inv")
