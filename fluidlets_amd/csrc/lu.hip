@@ -736,9 +736,11 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 // Forward (LOWER: L z = y, unit diagonal) or backward (U x = z) substitution
 // by NB-row blocks in one launch, for NR-padded right-hand sides stored
 // row-major with ld nrhs <= NR.  Block row p needs every block solved before
-// it in the sweep:
-//   acc = b_p - sum_j T_pj x_j     (j < p forward, j > p backward),
-//   x_p = inv(T_pp) acc            (the diagonal block's inverse kept in aux).
+// it in the sweep, written with its nearest block q = p -/+ 1 apart:
+//   c_p = inv(T_pp) (b_p - sum_{j != q} T_pj x_j)   (j < p forward, j > p backward),
+//   x_p = c_p - M_p x_q,   M_p = inv(T_pp) T_pq      (inverse and link kept in aux),
+// so c_p is formed while block q is still being solved, and the step from
+// x_q to x_p is one NB x NB product.
 // Workgroups take tickets in dispatch order (ticket t solves block t forward,
 // nblk - 1 - t backward), so a workgroup waits only on workgroups dispatched
 // before it.  The output x is filled with a signalling-NaN sentinel before
@@ -861,13 +863,15 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
             for (int k = 0; k < H; ++k) trow[k] = (r < nbp && hf * H + k < nbj) ? src[k] : T(0);
         }
     };
-    // the diagonal block's inverse row (always NB x NB, zero or identity
-    // beyond the block; rows of acc beyond it are zero)
-    const T* inv = invs + p * 2 * NB * NB + (LOWER ? 0 : NB * NB) + (int64_t)r * NB + hf * H;
-    auto load_inv = [&]() {
+    // the diagonal block's inverse row and the link row M_p (always NB x NB,
+    // zero or identity beyond the block; rows of acc beyond it are zero)
+    const T* blk = invs + p * 4 * NB * NB + (int64_t)r * NB + hf * H;
+    const T* inv = blk + (LOWER ? 0 : NB * NB);
+    const T* lnk = blk + (LOWER ? 2 : 3) * NB * NB;
+    auto load_nb = [&](const T* src) {
 #pragma unroll
         for (int k = 0; k < H; k += V) {
-            const VT u = *reinterpret_cast<const VT*>(inv + k);
+            const VT u = *reinterpret_cast<const VT*>(src + k);
             const T* up = reinterpret_cast<const T*>(&u);
 #pragma unroll
             for (int q = 0; q < V; ++q) trow[k + q] = up[q];
@@ -899,24 +903,31 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
             }
         }
     };
-    if (nsteps > 0) load_row(LOWER ? 0 : nblk - 1);
-    else load_inv();
-#pragma unroll 1
-    for (int64_t s = 0; s < nsteps; ++s) {
-        const int64_t j = LOWER ? s : nblk - 1 - s;
+    // x_j of block j of the sweep into xs (waiting for each entry)
+    auto read_x = [&](int64_t j) {
         const int64_t c0 = j * NB;
         const int nbj = (int)min((int64_t)NB, n - c0);
-        stage_row();
         for (int idx = tid; idx < NB * NR; idx += NT) {
             const int c = idx / NR, q = idx % NR;
             xs[idx] = (c < nbj && q < nrhs) ? chain_read(&x[(c0 + c) * nrhs + q], gave_up) : T(0);
         }
+    };
+    // every block of the sweep before this one but the nearest: acc -= T_pj x_j
+    const int64_t far = nsteps > 0 ? nsteps - 1 : 0;
+    if (far > 0) load_row(LOWER ? 0 : nblk - 1);
+    else load_nb(inv);
+#pragma unroll 1
+    for (int64_t s = 0; s < far; ++s) {
+        stage_row();
+        read_x(LOWER ? s : nblk - 1 - s);
         __syncthreads();
-        if (s + 1 < nsteps) load_row(LOWER ? s + 1 : nblk - 2 - s);
-        else load_inv();
+        if (s + 1 < far) load_row(LOWER ? s + 1 : nblk - 2 - s);
+        else load_nb(inv);
         row_product(part);
         __syncthreads();
     }
+    // c_p = inv(T_pp) (b_p - acc), formed while the nearest block is still
+    // being solved by the workgroup before this one
 #pragma unroll
     for (int q = 0; q < NR; ++q) part[q] = row_sum<TPR>(part[q]);  // the TPR parts of row r
     if (hf == 0)
@@ -925,12 +936,26 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
             xs[r * NR + q] = (r < nbp && q < nrhs) ? bq[q] - part[q] : T(0);
     stage_row();
     __syncthreads();
+    if (nsteps > 0) load_nb(lnk);
     T y[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) y[q] = T(0);
     row_product(y);
 #pragma unroll
     for (int q = 0; q < NR; ++q) y[q] = row_sum<TPR>(y[q]);
+    if (nsteps > 0) {
+        // x_p = c_p - M_p x_nearest: the only work after the nearest block arrives
+        __syncthreads();
+        stage_row();
+        read_x(LOWER ? p - 1 : p + 1);
+        __syncthreads();
+        T z[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) z[q] = T(0);
+        row_product(z);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) y[q] -= row_sum<TPR>(z[q]);
+    }
     const int bad = __syncthreads_or(gave_up);
     if (tid == 0 && bad) atomicOr(&sync[1], 1u);
     if (hf == 0 && r < nbp)
@@ -1186,16 +1211,83 @@ int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 template <typename T>
 struct FactorAux {
-    // aux layout: perm (n int32, padded to 256 B), then per panel p: invL (NB x NB), invU (NB x NB)
+    // aux layout: perm (n int32, padded to 256 B), then per panel p four
+    // NB x NB blocks: invL, invU, and the chained solves' links
+    // ML = inv(L_pp) L_p,p-1 and MU = inv(U_pp) U_p,p+1 (link_kernel)
+    static constexpr int BLOCKS = 4;
     static size_t perm_bytes(int64_t n) { return align_up((size_t)n * sizeof(int32_t), 256); }
-    static size_t bytes(int64_t n) { return perm_bytes(n) + (size_t)div_up(n, NB) * 2 * NB * NB * sizeof(T); }
+    static size_t bytes(int64_t n)
+    {
+        return perm_bytes(n) + (size_t)div_up(n, NB) * BLOCKS * NB * NB * sizeof(T);
+    }
     static int32_t* perm(void* aux) { return static_cast<int32_t*>(aux); }
     static T* invL(void* aux, int64_t n, int64_t p)
     {
-        return reinterpret_cast<T*>(static_cast<char*>(aux) + perm_bytes(n)) + (size_t)p * 2 * NB * NB;
+        return reinterpret_cast<T*>(static_cast<char*>(aux) + perm_bytes(n)) + (size_t)p * BLOCKS * NB * NB;
     }
     static T* invU(void* aux, int64_t n, int64_t p) { return invL(aux, n, p) + NB * NB; }
+    static T* linkL(void* aux, int64_t n, int64_t p) { return invL(aux, n, p) + 2 * NB * NB; }
+    static T* linkU(void* aux, int64_t n, int64_t p) { return invL(aux, n, p) + 3 * NB * NB; }
 };
+
+// The links of the chained triangular solves: for block row p,
+//   ML_p = inv(L_pp) L_p,p-1  (p >= 1),   MU_p = inv(U_pp) U_p,p+1  (p <= nblk - 2),
+// NB x NB (ld NB), zero beyond the matrix and where the neighbour block does
+// not exist.  With them a sweep's block step depends on its neighbour only
+// through x_p = c_p - M_p x_(p-/+1), c_p being formed before the neighbour
+// arrives.  grid (nblk, 2): y = 0 forms ML, y = 1 MU; 256 lanes, each an
+// 8 x 8 tile of the product, K staged through LDS in slices of 32.  blocks =
+// FactorAux::invL(aux, n, 0) (the four blocks of panel p start at p 4 NB^2).
+template <typename T>
+__global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int64_t n, T* blocks)
+{
+    constexpr int KC = 32;
+    __shared__ T Is[NB * (KC + 1)];
+    __shared__ T Bs[KC * NB];
+    const int64_t p = blockIdx.x;
+    const bool upper = blockIdx.y == 1;
+    const int64_t nblk = (n + NB - 1) / NB;
+    T* blk = blocks + p * FactorAux<T>::BLOCKS * NB * NB;
+    const T* inv = blk + (upper ? NB * NB : 0);
+    T* out = blk + (upper ? 3 : 2) * NB * NB;
+    const bool exists = upper ? p + 1 < nblk : p >= 1;
+    const int tid = threadIdx.x, ti = tid / 16, tj = tid % 16;
+    T acc[8][8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) acc[u][v] = T(0);
+    if (exists) {
+        const int64_t r0 = p * NB, c0 = (upper ? p + 1 : p - 1) * NB;
+        const int64_t nbr = min((int64_t)NB, n - r0), nbc = min((int64_t)NB, n - c0);
+        for (int k0 = 0; k0 < NB; k0 += KC) {
+            for (int idx = tid; idx < NB * KC; idx += 256) {
+                const int i = idx / KC, k = idx % KC;
+                Is[i * (KC + 1) + k] = inv[i * NB + k0 + k];
+                const int kk = idx / NB, j = idx % NB;
+                Bs[kk * NB + j] = (k0 + kk < nbr && j < nbc) ? LU[(r0 + k0 + kk) * lda + c0 + j] : T(0);
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int k = 0; k < KC; ++k) {
+                T a[8], b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = Is[(ti * 8 + u) * (KC + 1) + k];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) b[v] = Bs[k * NB + tj * 8 + v];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) out[(ti * 8 + u) * NB + tj * 8 + v] = acc[u][v];
+}
 
 // ---------------------------------------------------------------- trailing update
 // A22 -= L21 U12 is a plain GEMM: large ones go to rocBLAS (one handle per
@@ -1510,6 +1602,10 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                 return s;
         }
     }
+    // the chained solves' links, from the finished factors
+    hipLaunchKernelGGL(link_kernel<T>, dim3((unsigned)div_up(n, NB), 2), dim3(256), 0, st, (const T*)A, lda, n,
+                       FactorAux<T>::invL(aux, n, 0));
+    FL_LAUNCHED();
     if (spec) {
         hipLaunchKernelGGL(spec_finish_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, (const unsigned*)flag,
                            status, FactorAux<T>::perm(aux), n);
