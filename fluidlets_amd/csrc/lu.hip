@@ -1044,6 +1044,10 @@ __global__ __launch_bounds__(256) void copy_rows_kernel(const TS* src, TD* dst, 
 // n = 5,400 that is 675 workgroups, enough to fill the chip.  x and r are
 // n x nrhs (ld nrhs), b has ld ldb.
 constexpr int RES_ROWS = 8;
+// PAIR: two adjacent columns per lane and step, read as one 16-byte load per
+// row (A 16-byte aligned, lda even), which doubles the bytes each load keeps
+// in flight.  Either variant sums in a fixed order for a given shape.
+template <bool PAIR>
 __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t lda, int64_t n, const double* b,
                                                        int64_t ldb, const double* x, double* r, int nrhs)
 {
@@ -1055,7 +1059,25 @@ __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t 
     for (int i = 0; i < RES_ROWS; ++i)
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[i][q] = 0.0;
-    for (int64_t c = threadIdx.x; c < n; c += 256) {
+    int64_t c = PAIR ? 2 * (int64_t)threadIdx.x : (int64_t)threadIdx.x;
+    if (PAIR) {
+        for (; c + 1 < n; c += 512) {
+            double x0[8], x1[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                x0[q] = q < nrhs ? x[c * nrhs + q] : 0.0;
+                x1[q] = q < nrhs ? x[(c + 1) * nrhs + q] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < RES_ROWS; ++i) {
+                const double2 a = row0 + i < n ? *reinterpret_cast<const double2*>(A + (row0 + i) * lda + c)
+                                               : make_double2(0.0, 0.0);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[i][q] += a.x * x0[q] + a.y * x1[q];
+            }
+        }
+    }
+    for (; c < n; c += PAIR ? 512 : 256) {   // PAIR: at most the odd last column
         double xv[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) xv[q] = q < nrhs ? x[c * nrhs + q] : 0.0;
@@ -1749,6 +1771,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     unsigned* sync = reinterpret_cast<unsigned*>(w + 3 * d8 + 2 * d4 + 1024);
     const unsigned g = (unsigned)div_up(n * nrhs, 256);
     const int q = (int)nrhs;
+    const bool pair = reinterpret_cast<uintptr_t>(A64) % 16 == 0 && lda64 % 2 == 0;
 
     // r = b (contiguous, ld nrhs) and ||b|| per column
     hipLaunchKernelGGL((gather_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, b, ldb, r, n, q,
@@ -1811,8 +1834,12 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
                                n, q, (const double*)scale);
         FL_LAUNCHED();
         ++it;
-        hipLaunchKernelGGL(residual_kernel, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, A64, lda64, n, b, ldb,
-                           (const double*)xc, r, q);
+        if (pair)
+            hipLaunchKernelGGL(residual_kernel<true>, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, A64, lda64,
+                               n, b, ldb, (const double*)xc, r, q);
+        else
+            hipLaunchKernelGGL(residual_kernel<false>, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, A64,
+                               lda64, n, b, ldb, (const double*)xc, r, q);
         FL_LAUNCHED();
     }
     hipLaunchKernelGGL((copy_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, (const double*)xc, x, ldb, n, q);
