@@ -23,23 +23,26 @@ class FactorCache:
         self.hits = 0
         self.misses = 0
         self.inflight: dict = {}   # key (or (key, "upgrade")) -> Event of the thread making it
-        self.key_hits: dict = {}   # hits per stored key since it was stored
+        self.key_uses: dict = {}   # counted lookups (hits and makes) per stored key
 
-    def get_or_make(self, key, make, made: list | None = None):
+    def get_or_make(self, key, make, made: list | None = None, count: bool = True):
         """The item for `key`, made once: on a miss ``make()`` returns
         ``(item, nbytes)`` and is stored; a caller that finds another thread
         making the same key waits for it instead of making it again (and
         makes it itself if that attempt failed or the item was not kept).
         ``make`` must leave the item complete (its device work finished)
         before returning, since callers use it on their own streams.  With
-        ``made`` (a list), True is appended when this call made the item."""
+        ``made`` (a list), True is appended when this call made the item.
+        count=False leaves the hit statistics alone (a prefetching caller);
+        an item it makes still counts as a miss."""
         while True:
             with self.lock:
                 item = self.items.get(key)
                 if item is not None:
                     self.items.move_to_end(key)
-                    self.hits += 1
-                    self.key_hits[key] = self.key_hits.get(key, 0) + 1
+                    if count:
+                        self.hits += 1
+                        self.key_uses[key] = self.key_uses.get(key, 0) + 1
                     if made is not None:
                         made.append(False)
                     return item
@@ -56,6 +59,8 @@ class FactorCache:
             if made is not None:
                 made.append(True)
             with self.lock:
+                if count and key in self.items:
+                    self.key_uses[key] = self.key_uses.get(key, 0) + 1
                 stored = self.items.get(key)   # the stored tuple itself, for upgrade()'s identity check
             same = stored is not None and len(stored) == len(item) + 1 and all(a is b for a, b in zip(stored, item))
             return stored if same else (*item, nbytes)
@@ -111,6 +116,6 @@ class FactorCache:
             while self.bytes + nbytes > self.max_bytes and self.items:
                 gone, ev = self.items.popitem(last=False)
                 self.bytes -= ev[-1]
-                self.key_hits.pop(gone, None)
+                self.key_uses.pop(gone, None)
             self.items[key] = (*item, nbytes)
             self.bytes += nbytes

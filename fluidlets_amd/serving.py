@@ -64,8 +64,9 @@ def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) ->
     """HBM bytes a request holds while it runs: the FP64 matrix (8 n^2), the
     FP32 copy for the mixed path (4 n^2), plus O(n) vectors and workspaces;
     a mixed-precision swimmer also holds the FP32 inverse it makes next to
-    its factors (4 n^2) and, without the server's shared cache, its own cache
-    of body-frame inverses."""
+    its factors and the factors its helpers make ahead (4 n^2 each) and,
+    without the server's shared cache, its own cache of body-frame
+    factors."""
     if req.kind == "resistance":
         n = 3 * len(req.payload["points"])
     else:
@@ -73,7 +74,8 @@ def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) ->
     per = 8 + (4 if precision == "mixed" else 8)
     extra = 0
     if req.kind == "swimmer" and precision == "mixed":
-        extra += 4 * n * n
+        from .swimming import PREFETCH_WORKERS
+        extra += 4 * (1 + PREFETCH_WORKERS) * n * n   # its inverse next to its factors; its helpers' factors
     if req.kind == "swimmer" and precision == "mixed" and not shared_cache:
         from .swimming import reuse_cache_bytes
         extra += reuse_cache_bytes(req.payload["n_points"])   # the swimmer's own cache

@@ -26,14 +26,17 @@ comes back a second time (its third visit), its factors are traded for the
 explicit FP32 inverse (fl_sgetri, the same memory), so every later
 correction is one pass over it (fl_refine_inv) instead of two chained
 triangular sweeps.  Up to then the factors serve, since the inverse costs
-40 % of a factorization more.  Concurrent swimmers sharing a cache make each shape once
-(FactorCache.get_or_make).  A step's result therefore depends on the cache's
-history only within the refinement tolerance.
+40 % of a factorization more.  Concurrent swimmers sharing a cache make each
+shape once (FactorCache.get_or_make), and swim_rk2 makes the factors of the
+coming stage shapes on helper threads and streams ahead of the integrator,
+since the body-frame shapes depend only on time.  A step's result depends
+on the cache's history only within the refinement tolerance.
 """
 from __future__ import annotations
 
 import ctypes
 import hashlib
+import threading
 from dataclasses import dataclass
 from typing import Callable
 
@@ -108,6 +111,35 @@ class SwimState:
     nearest: np.ndarray      # (Q,) nearest force point of each quadrature point
 
 
+def _eps_points(eps, Q: int) -> np.ndarray:
+    """Blob widths of the Q quadrature points: one value when all are equal."""
+    eps_arr = np.broadcast_to(np.asarray(eps, float), (Q,))
+    return eps_arr if not (Q > 1 and np.all(eps_arr == eps_arr[0])) else eps_arr[:1]
+
+
+def _make_body_factors(kernel, shape: SwimmerShape, nearest, e, mu: float, stream, lu_hint):
+    """FP32 factors of the body-frame matrix of `shape` (the factor cache's
+    item for its key): assembled and factored on `stream` (torch's current
+    stream carries the uploads), complete when this returns."""
+    t = D.torch()
+    dev = D.device()
+    N = len(shape.x)
+    xb, Xb = D.soa(np.asarray(shape.x, float), dev), D.soa(np.asarray(shape.X, float), dev)
+    order, start = nearest_groups(nearest, N)
+    order_t, start_t = t.from_numpy(order).to(dev), t.from_numpy(start).to(dev)
+    eps_t, stride = D.eps_arg(e, len(shape.X), dev)
+
+    def body32():
+        return assemble_nearest_device(kernel, xb, Xb, eps_t, stride, order_t, start_t, N, mu=mu, fp32=True,
+                                       fp64=False, stream=stream)[1]
+
+    F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()), hint=lu_hint,
+                     products="bf16x3")        # returns with the factors complete
+    if F.info > 0:
+        raise np.linalg.LinAlgError("Singular matrix")
+    return (F, None), F.nbytes()
+
+
 def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, nearest=None,
                      precision: str = "fp64", stream=None, factor_cache: FactorCache | None = None,
                      lu_hint: dict | None = None) -> SwimState:
@@ -138,8 +170,7 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
         nearest = nearest_device(X_soa, x_soa, stream=stream).cpu().numpy()
     nearest = np.ascontiguousarray(nearest, dtype=np.int64)
     order, start = nearest_groups(nearest, N)
-    eps_arr = np.broadcast_to(np.asarray(eps, float), (Q,))
-    e = eps_arr if not (Q > 1 and np.all(eps_arr == eps_arr[0])) else eps_arr[:1]
+    e = _eps_points(eps, Q)
     eps_t, stride = D.eps_arg(e, Q, dev)
     order_t, start_t = t.from_numpy(order).to(dev), t.from_numpy(start).to(dev)
     body = precision == "mixed" and factor_cache is not None and _rotation_invariant(kernel)
@@ -162,23 +193,9 @@ def swimming_problem(shape: SwimmerShape, x0, B, eps, mu=1.0, kernel=None, neare
         Y = linalg.getrs(F, R, stream=stream)
     elif precision == "mixed" and body:
         key = _body_key(kernel, shape, nearest, e, mu)
-
-        def make_factors():
-            xb, Xb = D.soa(np.asarray(shape.x, float), dev), D.soa(np.asarray(shape.X, float), dev)
-
-            def body32():
-                return assemble_nearest_device(kernel, xb, Xb, eps_t, stride, order_t, start_t, N, mu=mu, fp32=True,
-                                               fp64=False, stream=stream)[1]
-
-            F = linalg.getrf(body32(), stream=stream, refill=lambda A: A.copy_(body32()), hint=lu_hint,
-                             products="bf16x3")        # returns with the factors complete
-            if F.info > 0:
-                raise np.linalg.LinAlgError("Singular matrix")
-            return (F, None), F.nbytes()
-
-        made: list = []
-        item = factor_cache.get_or_make(key, make_factors, made)
-        if item[1] is None and not made[0] and factor_cache.key_hits.get(key, 0) >= 2:
+        item = factor_cache.get_or_make(
+            key, lambda: _make_body_factors(kernel, shape, nearest, e, mu, stream, lu_hint))
+        if item[1] is None and factor_cache.key_uses.get(key, 0) >= 3:
             # the shape keeps coming back (third visit): trade its factors for
             # the explicit inverse, so this and later visits correct with one
             # pass over it instead of two chained triangular sweeps
@@ -255,6 +272,47 @@ def swim(shape_at: Callable[[float], SwimmerShape], t_span, eps, mu=1.0, x0=(0, 
     return Trajectory(sol.t, sol.y[:3].T, Bt)
 
 
+PREFETCH_DEPTH = 4      # stage shapes made ahead of the integrator
+PREFETCH_WORKERS = 2    # helpers, each on its own stream, taking every other stage
+
+
+def _prefetch_body_factors(shape_at, stage_times, kernel, nearest, e, mu, cache: FactorCache, progress, cond,
+                           first: int = 0, step: int = 1):
+    """swim_rk2's helper: make the body-frame factors of the stage shapes in
+    order, at most PREFETCH_DEPTH ahead of the integrator, on a stream of its
+    own.  The cache makes each shape once (FactorCache.get_or_make), so the
+    integrator either finds a shape ready or waits for it; a failure here
+    only ends the helper (the integrator then makes what it needs).  After
+    2 PREFETCH_DEPTH shapes in a row that were already made (a beat already
+    in the cache) the helper stops, leaving the interpreter to the
+    integrator."""
+    t = D.torch()
+    stream = t.cuda.Stream(device=D.device())
+    hint: dict = {}
+    idle = 0
+    try:
+        for k in range(first, len(stage_times), step):
+            tk = stage_times[k]
+            if idle >= 2 * PREFETCH_DEPTH // step:
+                return
+            with cond:
+                while not progress["stop"] and k >= progress["done"] + PREFETCH_DEPTH:
+                    cond.wait()
+                if progress["stop"]:
+                    return
+            shape = shape_at(tk)
+            key = _body_key(kernel, shape, nearest, e, mu)
+            made: list = []
+            with t.cuda.stream(stream):
+                cache.get_or_make(key, lambda: _make_body_factors(kernel, shape, nearest, e, mu, stream, hint),
+                                  made, count=False)
+            idle = 0 if made[0] else idle + 1
+    except Exception:
+        return
+    finally:
+        stream.synchronize()
+
+
 def reuse_cache_bytes(n_points: int) -> int:
     """Bound of a swimmer's own factor cache: 128 body shapes' FP32 inverses
     (two solves per step, so a beat of 64 steps or fewer fits), at most 16 GiB."""
@@ -264,7 +322,8 @@ def reuse_cache_bytes(n_points: int) -> int:
 
 def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, steps: int, eps, mu=1.0,
              x0=(0, 0, 0), B0=None, kernel=None, fixed_nearest=True, precision: str = "fp64",
-             stream=None, factor_cache: FactorCache | None = None, reuse_factors: bool = True) -> Trajectory:
+             stream=None, factor_cache: FactorCache | None = None, reuse_factors: bool = True,
+             prefetch: bool = True) -> Trajectory:
     """Fixed-step explicit midpoint (RK2) integration of the frame:
     z_{n+1} = z_n + dt F(t_n + dt/2, z_n + dt/2 F(t_n, z_n)),  two device
     swimming_problem solves per step.
@@ -272,7 +331,12 @@ def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, st
     With precision "mixed" and a free-space kernel the body-frame FP32
     factors are reused whenever the beat revisits a shape (``factor_cache``,
     or a cache of this call bounded by reuse_cache_bytes); reuse_factors=False
-    factors every step."""
+    factors every step.  The body-frame shapes depend only on time, so
+    PREFETCH_WORKERS helper threads make the factors of the next
+    PREFETCH_DEPTH stage shapes, each on a stream of its own, while the
+    integrator solves (the factorizations of one swimmer are latency-bound
+    and leave most of the GPU idle); they are joined before this returns.  prefetch=False leaves all making to the
+    integrator."""
     B0 = np.eye(3) if B0 is None else np.asarray(B0, float)
     nearest = None
     s0 = shape_at(t0)
@@ -284,23 +348,51 @@ def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, st
     elif factor_cache is None:
         factor_cache = FactorCache(reuse_cache_bytes(len(s0.x)))
     lu_hint: dict = {}
+    kern = Stokeslet3D() if kernel is None else kernel
+    progress = {"done": 0, "stop": False}
+    cond = threading.Condition()
 
     def F(t, z):
         b1, b2 = z[3:6], z[6:9]
         B = np.c_[b1, b2, np.cross(b1, b2)]
         st = swimming_problem(shape_at(t), z[:3], B, eps, mu, kernel, nearest, precision=precision, stream=stream,
                               factor_cache=factor_cache, lu_hint=lu_hint)
+        with cond:
+            progress["done"] += 1
+            cond.notify_all()
         return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
 
+    helpers: list = []
+    if (prefetch and factor_cache is not None and nearest is not None and _rotation_invariant(kern)
+            and not np.iscomplexobj(mu)):
+        stage_times = []
+        t = t0
+        for _ in range(steps):       # the integrator's own time arithmetic, so the shapes match
+            stage_times += [t, t + 0.5 * dt]
+            t += dt
+        helpers = [threading.Thread(target=_prefetch_body_factors,
+                                    args=(shape_at, stage_times, kern, nearest, _eps_points(eps, len(s0.X)),
+                                          float(mu), factor_cache, progress, cond, i, PREFETCH_WORKERS))
+                   for i in range(PREFETCH_WORKERS)]
+        for h in helpers:
+            h.start()
     z = np.concatenate([np.asarray(x0, float), B0[:, 0], B0[:, 1]])
     ts, zs = [t0], [z.copy()]
     t = t0
-    for _ in range(steps):
-        k1 = F(t, z)
-        z = z + dt * F(t + 0.5 * dt, z + 0.5 * dt * k1)
-        t += dt
-        ts.append(t)
-        zs.append(z.copy())
+    try:
+        for _ in range(steps):
+            k1 = F(t, z)
+            z = z + dt * F(t + 0.5 * dt, z + 0.5 * dt * k1)
+            t += dt
+            ts.append(t)
+            zs.append(z.copy())
+    finally:
+        if helpers:
+            with cond:
+                progress["stop"] = True
+                cond.notify_all()
+            for h in helpers:
+                h.join()
     zs = np.array(zs)
     b1, b2 = zs[:, 3:6], zs[:, 6:9]
     return Trajectory(np.array(ts), zs[:, :3], np.stack([b1, b2, np.cross(b1, b2)], axis=2))
