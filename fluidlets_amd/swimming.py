@@ -273,7 +273,7 @@ def swim(shape_at: Callable[[float], SwimmerShape], t_span, eps, mu=1.0, x0=(0, 
 
 
 PREFETCH_DEPTH = 4      # stage shapes made ahead of the integrator
-PREFETCH_WORKERS = 2    # helpers, each on its own stream, taking every other stage
+PREFETCH_WORKERS = 3    # helpers, each on its own stream, taking every third stage
 
 
 def _prefetch_body_factors(shape_at, stage_times, kernel, nearest, e, mu, cache: FactorCache, progress, cond,
