@@ -58,3 +58,19 @@ def test_admission_and_failures_do_not_stop_the_server():
     by = {r.rid: r for r in out}
     assert by[10].error.startswith("rejected") and by[11].error is not None
     assert all(by[r.rid].error is None for r in reqs)
+
+
+def test_concurrent_swimmers_share_each_shape_once():
+    """Two identical swimmers served together over the server's cache: every
+    body shape is made once (the other swimmer waits for it), and the two
+    trajectories agree to the refinement tolerance."""
+    from fluidlets_amd.swimming import biflagellate
+    print("case: serving two swimmers, shared cache", flush=True)
+    shape_at = biflagellate(n_body=150, n_flag=40)
+    pay = {"shape_at": shape_at, "dt": 1.0 / 8, "steps": 12, "eps": 0.02, "n_points": 230}
+    srv = SolveServer(slots=2, precision="mixed", cache_bytes=2 << 30)
+    out = srv.serve([Request("swimmer", pay, 0.0, 0), Request("swimmer", pay, 0.0, 1)], realtime=False)
+    assert all(r.error is None for r in out), [r.error for r in out]
+    assert srv.cache.misses == 16                # 16 stage shapes per beat (dt = T/8)
+    a, b = out[0].value, out[1].value
+    assert np.allclose(a, b, rtol=0, atol=1e-11 * np.abs(a).max())

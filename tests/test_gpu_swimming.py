@@ -208,3 +208,19 @@ def test_factor_reuse_over_a_periodic_beat():
     plain = swimming_problem(shape_at(0.25), x0, B, 0.02, precision="fp64")
     assert rel_err(warm.U, plain.U) < 1e-10 and rel_err(warm.Omega, plain.Omega) < 1e-10
     assert rel_err(warm.forces, plain.forces) < 1e-10
+
+
+def test_prefetching_helpers_do_not_change_the_trajectory():
+    """swim_rk2 with the factor-making helpers (three threads and streams
+    ahead of the integrator) gives bit for bit the trajectory it gives
+    without them: a shape's factors are the same whoever makes them, and the
+    switch to the explicit inverse counts the integrator's uses only."""
+    from fluidlets_amd.cache import FactorCache
+    print("case: swimmer prefetch vs none", flush=True)
+    shape_at = biflagellate(n_body=200, n_flag=60)
+    c1, c2 = FactorCache(1 << 30), FactorCache(1 << 30)
+    tr1 = swim_rk2(shape_at, 0.0, 0.125, 24, 0.02, precision="mixed", factor_cache=c1, prefetch=True)
+    tr2 = swim_rk2(shape_at, 0.0, 0.125, 24, 0.02, precision="mixed", factor_cache=c2, prefetch=False)
+    assert np.array_equal(tr1.x0, tr2.x0) and np.array_equal(tr1.B, tr2.B)
+    assert c1.misses == c2.misses == 16          # 16 stage shapes per beat, each made once
+    assert any(v[0] is None for v in c1.items.values())   # third uses refine with inverses
