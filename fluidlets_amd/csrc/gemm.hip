@@ -54,10 +54,9 @@ struct GemmArgs {
 // resident on one XCD share A row blocks and B column blocks in its L2.
 constexpr int NXCD = 8, GROUP_M = 8;
 template <typename T>
-__device__ __forceinline__ void tile_of(const GemmArgs<T>& g, int& tm, int& tn)
+__device__ __forceinline__ void tile_of(const GemmArgs<T>& g, int id, int& tm, int& tn)
 {
     const int nt = g.tiles_m * g.tiles_n;
-    int id = blockIdx.x;
     if (!g.swizzle) {
         tm = id / g.tiles_n;
         tn = id % g.tiles_n;
@@ -73,8 +72,9 @@ __device__ __forceinline__ void tile_of(const GemmArgs<T>& g, int& tm, int& tn)
     tn = in / gm;
 }
 
+// One 128 x 128 tile of C for workgroup-local tile index `id` of problem g.
 template <typename T>
-__global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
+__device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
 {
     using P = Mma<T>;
     constexpr int KT = P::KT;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
     const int lane = tid % 64, wave = tid / 64;
     const int wm = wave / 2, wn = wave % 2;
     int tm, tn;
-    tile_of(g, tm, tn);
+    tile_of(g, id, tm, tn);
     const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
     const bool full_mn = g.aligned && m0 + TM <= g.M && n0 + TN <= g.N;
 
@@ -247,6 +247,24 @@ __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
+{
+    gemm_tile(g, (int)blockIdx.x);
+}
+
+// Two independent products in one launch (a grouped launch): workgroups
+// [0, tiles1) take problem g1, the rest problem g2.  The LU pairs its
+// panel's L21 = A21 inv(U11) with the in-panel U12 = inv(L11) A12, whose
+// outputs are disjoint, to save a launch per sub-panel.
+template <typename T>
+__global__ __launch_bounds__(WG, 2) void gemm_pair_kernel(GemmArgs<T> g1, GemmArgs<T> g2, int tiles1)
+{
+    const int id = (int)blockIdx.x;
+    if (id < tiles1) gemm_tile(g1, id);
+    else gemm_tile(g2, id - tiles1);
+}
+
 // C = beta C (the alpha == 0 case)
 template <typename T>
 __global__ __launch_bounds__(WG) void scale_kernel(T* C, int64_t ldc, int64_t M, int64_t N, T beta)
@@ -257,7 +275,48 @@ __global__ __launch_bounds__(WG) void scale_kernel(T* C, int64_t ldc, int64_t M,
     *p = beta == T(0) ? T(0) : beta * *p;
 }
 
+template <typename T>
+GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+                      T beta, T* C, int64_t ldc, unsigned int* absmax_bits)
+{
+    const int vw = 16 / (int)sizeof(T);
+    const int aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
+                        (lda % vw == 0) && (ldb % vw == 0);
+    static const char* sw_env = std::getenv("FL_GEMM_SWIZZLE");
+    const int swizzle = sw_env && *sw_env ? std::atoi(sw_env) : 1;
+    const int tiles_m = (int)((M + TM - 1) / TM), tiles_n = (int)((N + TN - 1) / TN);
+    return GemmArgs<T>{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits, aligned, tiles_m, tiles_n, swizzle};
+}
+
 }  // namespace
+
+// Two products C_i = alpha_i A_i B_i + beta_i C_i in one launch; the
+// outputs must not overlap each other's inputs or outputs (each may be in
+// place on its own operand as gemm_launch allows).  alpha_i != 0.
+template <typename T>
+int gemm_pair_launch(int64_t M1, int64_t N1, int64_t K1, T alpha1, const T* A1, int64_t lda1, const T* B1,
+                     int64_t ldb1, T beta1, T* C1, int64_t ldc1, unsigned int* absmax1, int64_t M2, int64_t N2,
+                     int64_t K2, T alpha2, const T* A2, int64_t lda2, const T* B2, int64_t ldb2, T beta2, T* C2,
+                     int64_t ldc2, hipStream_t stream)
+{
+    FL_CHECK(M1 > 0 && N1 > 0 && K1 >= 0 && M2 > 0 && N2 > 0 && K2 >= 0, "grouped GEMM needs non-empty products");
+    FL_CHECK(alpha1 != T(0) && alpha2 != T(0), "grouped GEMM needs alpha != 0");
+    const GemmArgs<T> g1 = gemm_args<T>(M1, N1, K1, alpha1, A1, lda1, B1, ldb1, beta1, C1, ldc1, absmax1);
+    const GemmArgs<T> g2 = gemm_args<T>(M2, N2, K2, alpha2, A2, lda2, B2, ldb2, beta2, C2, ldc2, nullptr);
+    const int t1 = g1.tiles_m * g1.tiles_n, t2 = g2.tiles_m * g2.tiles_n;
+    hipLaunchKernelGGL(gemm_pair_kernel<T>, dim3((unsigned)(t1 + t2)), dim3(WG), 0, stream, g1, g2, t1);
+    FL_LAUNCHED();
+    return FL_OK;
+}
+
+template int gemm_pair_launch<float>(int64_t, int64_t, int64_t, float, const float*, int64_t, const float*, int64_t,
+                                     float, float*, int64_t, unsigned int*, int64_t, int64_t, int64_t, float,
+                                     const float*, int64_t, const float*, int64_t, float, float*, int64_t,
+                                     hipStream_t);
+template int gemm_pair_launch<double>(int64_t, int64_t, int64_t, double, const double*, int64_t, const double*,
+                                      int64_t, double, double*, int64_t, unsigned int*, int64_t, int64_t, int64_t,
+                                      double, const double*, int64_t, const double*, int64_t, double, double*,
+                                      int64_t, hipStream_t);
 
 template <typename T>
 int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
@@ -271,14 +330,8 @@ int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t ld
         FL_LAUNCHED();
         return FL_OK;
     }
-    const int vw = 16 / (int)sizeof(T);
-    const int aligned = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
-                        (lda % vw == 0) && (ldb % vw == 0);
-    static const char* sw_env = std::getenv("FL_GEMM_SWIZZLE");
-    const int swizzle = sw_env && *sw_env ? std::atoi(sw_env) : 1;
-    const int tiles_m = (int)((M + TM - 1) / TM), tiles_n = (int)((N + TN - 1) / TN);
-    GemmArgs<T> g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits, aligned, tiles_m, tiles_n, swizzle};
-    hipLaunchKernelGGL(gemm_kernel<T>, dim3((unsigned)(tiles_m * tiles_n)), dim3(WG), 0, stream, g);
+    const GemmArgs<T> g = gemm_args<T>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, absmax_bits);
+    hipLaunchKernelGGL(gemm_kernel<T>, dim3((unsigned)(g.tiles_m * g.tiles_n)), dim3(WG), 0, stream, g);
     FL_LAUNCHED();
     return FL_OK;
 }

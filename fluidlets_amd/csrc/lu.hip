@@ -39,6 +39,11 @@ namespace fl {
 template <typename T>
 int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb, T beta,
                 T* C, int64_t ldc, unsigned int* absmax_bits, hipStream_t stream);
+template <typename T>
+int gemm_pair_launch(int64_t M1, int64_t N1, int64_t K1, T alpha1, const T* A1, int64_t lda1, const T* B1,
+                     int64_t ldb1, T beta1, T* C1, int64_t ldc1, unsigned int* absmax1, int64_t M2, int64_t N2,
+                     int64_t K2, T alpha2, const T* A2, int64_t lda2, const T* B2, int64_t ldb2, T beta2, T* C2,
+                     int64_t ldc2, hipStream_t stream);
 
 namespace {
 
@@ -1521,14 +1526,24 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             T* Akk = A + ks * lda + ks;
             T* invL = FactorAux<T>::invL(aux, n, p);
             T* invU = FactorAux<T>::invU(aux, n, p);
+            bool u12_done = false;   // the in-panel U12 already formed (grouped with L21)
             if (spec) {
                 hipLaunchKernelGGL(diag_nopiv_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag, invL,
                                    invU);
                 FL_LAUNCHED();
-                if (m > nb)
+                const int64_t rest0 = pe - ks - nb;
+                if (m > nb && rest0 > 0) {
+                    // L21 = A21 inv(U11) and U12 = inv(L11) A12 (in-panel), disjoint, in one launch
+                    if (int s = gemm_pair_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0),
+                                                    Akk + nb * lda, lda, flag, nb, rest0, nb, T(1), invL, NB, Akk + nb,
+                                                    lda, T(0), Akk + nb, lda, st))
+                        return s;
+                    u12_done = true;
+                } else if (m > nb) {
                     if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0),
                                                Akk + nb * lda, lda, flag, st))
                         return s;
+                }
             }
             // Tier 0: no interchanges; tier 1: interchanges inside the diagonal
             // block; tier 2: pivot search over every row below the diagonal.
@@ -1591,8 +1606,9 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             const int64_t rest = pe - ks - nb;
             if (rest > 0) {
                 T* A12 = Akk + nb;
-                if (int s = gemm_launch<T>(nb, rest, nb, T(1), invL, NB, A12, lda, T(0), A12, lda, nullptr, st))
-                    return s;
+                if (!u12_done)
+                    if (int s = gemm_launch<T>(nb, rest, nb, T(1), invL, NB, A12, lda, T(0), A12, lda, nullptr, st))
+                        return s;
                 if (m > nb)
                     if (int s = gemm_launch<T>(m - nb, rest, nb, T(-1), Akk + nb * lda, lda, A12, lda, T(1),
                                                Akk + nb * lda + nb, lda, nullptr, st))
