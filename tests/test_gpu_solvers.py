@@ -206,6 +206,34 @@ def test_bf16x3_products_lu_refines_to_the_fp64_solution(n_points):
     assert 1e-8 < d < 1e-4
 
 
+def test_bf16x3_products_with_row_interchanges():
+    """Split-bf16 trailing products on the verified path (a random matrix
+    needs interchanges in every panel, tiers 1 and 2): LAPACK's pivots,
+    a backward-stable factorization to the split's accuracy, and FP64
+    refinement of the FP32 factors to the FP64 solution."""
+    from fluidlets_amd import linalg
+    import torch
+    print("case: sgetrf bf16x3 random n=3000", flush=True)
+    n = 3000
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal((n, n))
+    A32 = A.astype(np.float32)
+    F = linalg.getrf(dev(A32.copy()), products="bf16x3")
+    F0 = linalg.getrf(dev(A32.copy()))
+    assert F.extra["products"] == "bf16x3" and F.info == 0
+    LU = F.lu.cpu().numpy().astype(np.float64)
+    L, U = np.tril(LU, -1) + np.eye(n), np.triu(LU)
+    perm = F.aux.cpu().numpy()[:4 * n].view(np.int32)
+    A64 = A32.astype(np.float64)
+    berr = np.linalg.norm(L @ U - A64[perm]) / (np.linalg.norm(np.abs(L) @ np.abs(U)))
+    assert berr < 2e-5
+    assert not np.array_equal(LU, F0.lu.cpu().numpy().astype(np.float64))   # the split path ran
+    b = rng.standard_normal((n, 3))
+    x, steps, res = linalg.refine(dev(A64), F, dev(b), tol=1e-12, max_iter=40)
+    assert res.max() <= 1e-12
+    assert rel_err(x.cpu().numpy(), np.linalg.solve(A64, b)) < 1e-8 * np.linalg.cond(A64) * 1e-2
+
+
 def test_bf16x3_products_are_ignored_for_fp64_and_small_matrices():
     from fluidlets_amd import linalg
     rng = np.random.default_rng(2)
