@@ -80,7 +80,7 @@ int getri_impl(const float* LU, int64_t n, int64_t lda, const void* aux, float* 
 int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux, int64_t n,
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
                 double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st,
-                const float* inv32 = nullptr, int64_t ldi = 0);
+                const float* inv32 = nullptr, int64_t ldi = 0, double* resid_dev = nullptr);
 
 size_t gmres_workspace_bytes(int64_t N, int restart, int inner_precision, int block_points);
 int gmres_impl(int kind, double alpha, int64_t N, const double* X, const double* eps, int64_t eps_stride,
@@ -268,6 +268,17 @@ int fl_refine_frame(const double* A64, int64_t lda64, const float* LU32, int64_t
     if (!frame) return set_error(FL_EINVAL, "fl_refine_frame needs a frame");
     return refine_impl(A64, lda64, LU32, lda32, aux32, n, b, x, nrhs, ldb, tol, max_iter, iters_out, resid_out,
                        frame, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_refine_steps(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32,
+                    int64_t n, const double* b, double* x, int64_t nrhs, int64_t ldb, int steps, double* resid_dev,
+                    void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (!resid_dev) return set_error(FL_EINVAL, "fl_refine_steps needs resid_dev");
+    if (steps < 0) return set_error(FL_EINVAL, "fl_refine_steps: negative step count");
+    return refine_impl(A64, lda64, LU32, lda32, aux32, n, b, x, nrhs, ldb, 0.0, steps, nullptr, nullptr, nullptr,
+                       workspace, workspace_bytes, as_stream(stream), nullptr, 0, resid_dev);
 }
 
 int fl_sgetri(const float* LU, int64_t n, int64_t lda, const void* aux, float* Ainv, int64_t ldi, void* stream)

@@ -1744,6 +1744,28 @@ int getrs_impl(const T* LU, int64_t n, int64_t lda, const void* aux_c, T* B, int
     return FL_OK;
 }
 
+// Zeroes n doubles (the fixed-step refinement clears x with a kernel, not a
+// memset, so that it can be captured in a graph).
+__global__ void zero_f64_kernel(double* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Final relative residuals of the fixed-step refinement, on the device:
+// ||r||_2 / ||b||_2 per column (||r||_2 when b = 0), from the squared norms;
+// NaN when a chained solve's block wait timed out (e0, e1: its flags).
+__global__ void finish_resid_kernel(const double* norms, const double* bnorms, const unsigned* e0, const unsigned* e1,
+                                    double* out, int q)
+{
+    const int c = threadIdx.x;
+    if (c >= q) return;
+    const double bn = sqrt(bnorms[c]);
+    double rel = bn > 0 ? sqrt(norms[c]) / bn : sqrt(norms[c]);
+    if ((e0 && *e0) || (e1 && *e1)) rel = __builtin_nan("");
+    out[c] = rel;
+}
+
 size_t refine_ws_bytes(int64_t n, int64_t nrhs)
 {
     return 3 * align_up((size_t)n * nrhs * sizeof(double), 256) + 2 * align_up((size_t)n * nrhs * sizeof(float), 256) +
@@ -1755,9 +1777,14 @@ size_t refine_ws_bytes(int64_t n, int64_t nrhs)
 int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux_c, int64_t n,
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
                 double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st,
-                const float* inv32 = nullptr, int64_t ldi = 0)
+                const float* inv32 = nullptr, int64_t ldi = 0, double* resid_dev = nullptr)
 {
+    // With resid_dev the refinement runs exactly max_iter corrections with no
+    // host synchronization and no memset (capturable in a HIP graph) and
+    // writes the final relative residuals to resid_dev on the device.
+    const bool fixed = resid_dev != nullptr;
     FL_CHECK(!inv32 || ldi >= n, "ldi < n");
+    FL_CHECK(!fixed || max_iter >= 0, "negative step count");
     FL_CHECK(n >= 0 && nrhs >= 1 && nrhs <= 8, "need 1 <= nrhs <= 8");
     FL_CHECK(ldb >= nrhs, "ldb < nrhs");
     FL_CHECK(!frame || n % 3 == 0, "a frame needs n = 3 x points");
@@ -1795,7 +1822,12 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     FL_LAUNCHED();
     hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, bnorms);
     FL_LAUNCHED();
-    FL_HIP(hipMemsetAsync(xc, 0, (size_t)n * nrhs * sizeof(double), st));
+    if (fixed) {
+        hipLaunchKernelGGL(zero_f64_kernel, dim3(g), dim3(256), 0, st, xc, n * nrhs);
+        FL_LAUNCHED();
+    } else {
+        FL_HIP(hipMemsetAsync(xc, 0, (size_t)n * nrhs * sizeof(double), st));
+    }
     // small = norms[16], scale[8], inv_scale[8], bnorms[16]: one read-back
     // brings the residual's and the right-hand side's norms to the host
     std::vector<double> hs(48);
@@ -1810,16 +1842,20 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
         if (it > 0 || max_iter <= 0) {
             hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, norms);
             FL_LAUNCHED();
-            FL_HIP(hipMemcpyAsync(hs.data(), small, 48 * sizeof(double), hipMemcpyDeviceToHost, st));
-            FL_HIP(hipStreamSynchronize(st));
-            worst = 0.0;
-            for (int c = 0; c < q; ++c) {
-                const double bn = std::sqrt(hb[c]);
-                const double rel = bn > 0 ? std::sqrt(hr[c]) / bn : std::sqrt(hr[c]);
-                if (resid_out) resid_out[c] = rel;
-                worst = std::max(worst, rel);
+            if (fixed) {
+                if (it >= max_iter) break;   // the norms stay on the device
+            } else {
+                FL_HIP(hipMemcpyAsync(hs.data(), small, 48 * sizeof(double), hipMemcpyDeviceToHost, st));
+                FL_HIP(hipStreamSynchronize(st));
+                worst = 0.0;
+                for (int c = 0; c < q; ++c) {
+                    const double bn = std::sqrt(hb[c]);
+                    const double rel = bn > 0 ? std::sqrt(hr[c]) / bn : std::sqrt(hr[c]);
+                    if (resid_out) resid_out[c] = rel;
+                    worst = std::max(worst, rel);
+                }
+                if ((it > 0 && worst <= tol) || it >= max_iter) break;
             }
-            if ((it > 0 && worst <= tol) || it >= max_iter) break;
         }
         // d = LU32^{-1} (r / s) s, with s = max |r| per column so FP32 sees O(1) values
         hipLaunchKernelGGL(scale_from_norms_kernel, dim3(1), dim3(8), 0, st, (const double*)(it == 0 ? bnorms : norms),
@@ -1860,6 +1896,16 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     }
     hipLaunchKernelGGL((copy_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, (const double*)xc, x, ldb, n, q);
     FL_LAUNCHED();
+    if (fixed) {
+        const int64_t half = div_up(n, NB) + 2;
+        const bool waits = !inv32 && it > 0;
+        hipLaunchKernelGGL(finish_resid_kernel, dim3(1), dim3(64), 0, st, (const double*)norms, (const double*)bnorms,
+                           (const unsigned*)(waits ? sync + 1 : nullptr),
+                           (const unsigned*)(waits ? sync + half + 1 : nullptr), resid_dev, q);
+        FL_LAUNCHED();
+        if (iters_out) *iters_out = it;
+        return FL_OK;
+    }
     unsigned sync_err[2] = {0u, 0u};
     if (!inv32 && it > 0) {   // the chained solves' wait flags (an explicit inverse has none)
         const int64_t half = div_up(n, NB) + 2;

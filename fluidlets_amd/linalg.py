@@ -136,6 +136,30 @@ def getri(F: LUFactor, stream=None):
     return X
 
 
+def refine_steps(A64, F32: LUFactor, b, x, resid, steps: int, ws, stream=None):
+    """Exactly ``steps`` refinement corrections of A64 x = b from the FP32
+    factors (fl_refine_steps), launched on ``stream`` with no host
+    synchronization and no allocation, so that the call can be captured in a
+    HIP graph.  b, x: (n, nrhs <= 8) float64 device tensors; ``resid``
+    (float64, at least nrhs) receives the final relative residuals on the
+    device; ``ws`` is a uint8 device tensor of fl_refine_workspace_size bytes.
+    For the same factors x is bit for bit ``refine``'s result when ``refine``
+    stops after ``steps`` corrections."""
+    L = lib()
+    n, nrhs = int(b.shape[0]), int(b.shape[1])
+    if not (b.is_contiguous() and x.is_contiguous()) or tuple(x.shape) != (n, nrhs) or n != F32.n:
+        raise ValueError("refine_steps needs contiguous (n, nrhs) b and x matching the factors")
+    if F32.precision != FL_FP32C or int(A64.shape[0]) != n or resid.numel() < nrhs:
+        raise ValueError("refine_steps needs FP32 factors of A64 and room for nrhs residuals")
+    nbytes = int(L.fl_refine_workspace_size(n, nrhs))
+    if ws.numel() < nbytes:
+        raise ValueError(f"refine_steps needs {nbytes} workspace bytes")
+    check(L.fl_refine_steps(D.ptr(A64), int(A64.stride(0)), D.ptr(F32.lu), int(F32.lu.stride(0)), D.ptr(F32.aux), n,
+                            D.ptr(b), D.ptr(x), nrhs, nrhs, int(steps), D.ptr(resid), D.ptr(ws), nbytes,
+                            D.stream_handle(stream)), "refine_steps")
+    return x
+
+
 def refine(A64, F32: LUFactor | None, b, tol: float = 1e-10, max_iter: int = 30, stream=None, frame=None,
            inverse=None):
     """x solving A64 x = b (FP64) from the FP32 factors by iterative

@@ -90,9 +90,23 @@ def geometry_key(kernel, points, eps) -> str:
     return h.hexdigest()
 
 
-def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, stream=None):
+def _grand(X, center, x, mu) -> np.ndarray:
+    """6 x 6 grand resistance from the (3N, 6) forces of the six modes."""
+    f = x.reshape(len(X), 3, 6).transpose(2, 0, 1) * float(np.real(mu))
+    r = X - np.asarray(center, float)
+    R = np.zeros((6, 6))
+    for m in range(6):
+        R[:3, m] = f[m].sum(axis=0)
+        R[3:, m] = np.cross(r, f[m]).sum(axis=0)
+    return R
+
+
+def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, stream=None, graphs=None):
     """Six-mode grand resistance of one rigid body: factor (or reuse the
-    resident factors) and solve the six rigid-body right-hand sides."""
+    resident factors) and solve the six rigid-body right-hand sides.  With a
+    ``graphs`` pool (graphs.GraphPool), mixed-precision solves of at most
+    graphs.GRAPH_MAX_POINTS points run as a captured graph on a padded
+    system (resident factors then stay padded)."""
     from . import linalg
     from .core.assemble import assemble_device
     from .core.solve import rigid_modes
@@ -100,6 +114,21 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     t = D.torch()
     kernel, X, center, eps = payload["kernel"], payload["points"], payload["center"], payload["eps"]
     dev = D.device()
+    if graphs is not None and precision == "mixed":
+        from . import graphs as G
+        if len(X) <= G.GRAPH_MAX_POINTS:
+            key = geometry_key(kernel, X, eps) + "graph" if cache is not None else None
+            item = cache.get(key) if cache is not None else None
+            modes = rigid_modes(X, center)
+            if item is None:
+                g = graphs.get(len(X), stream)
+                x, _, F = g.solve(kernel, X, eps, modes)
+                if cache is not None:
+                    K64p, Fp = g.resident(F)
+                    cache.put(key, (K64p, Fp), K64p.numel() * 8 + Fp.nbytes())
+            else:
+                x, _ = G.solve_resident(item[0], item[1], modes, steps=graphs.steps, stream=stream)
+            return _grand(X, center, x, payload.get("mu", 1.0))
     key = geometry_key(kernel, X, eps) + precision if cache is not None else None
     item = cache.get(key) if cache is not None else None
     rhs = t.from_numpy(rigid_modes(X, center)).to(dev)
@@ -123,13 +152,7 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     else:
         x = rhs.clone()
         linalg.getrs(F, x, stream=stream)
-    f = x.cpu().numpy().reshape(len(X), 3, 6).transpose(2, 0, 1) * float(np.real(payload.get("mu", 1.0)))
-    r = X - np.asarray(center, float)
-    R = np.zeros((6, 6))
-    for m in range(6):
-        R[:3, m] = f[m].sum(axis=0)
-        R[3:, m] = np.cross(r, f[m]).sum(axis=0)
-    return R
+    return _grand(X, center, x.cpu().numpy(), payload.get("mu", 1.0))
 
 
 def run_swimmer(payload: dict, precision: str, cache: FactorCache | None = None, stream=None):
@@ -141,10 +164,13 @@ def run_swimmer(payload: dict, precision: str, cache: FactorCache | None = None,
 
 class SolveServer:
     """Continuous-batching server: `slots` concurrent requests, each on its
-    own stream, admitted by HBM footprint against `budget_bytes`."""
+    own stream, admitted by HBM footprint against `budget_bytes`.  With
+    ``graphs`` (mixed precision) small resistance solves replay captured
+    graphs, whose buffers (at most graphs.pool_bytes(slots)) are set aside
+    from the budget."""
 
     def __init__(self, slots: int = 4, precision: str = "mixed", budget_bytes: int | None = None,
-                 cache_bytes: int = 0):
+                 cache_bytes: int = 0, graphs: bool = True):
         t = D.torch()
         dev = D.device()
         if budget_bytes is None:
@@ -155,12 +181,17 @@ class SolveServer:
         self.budget = budget_bytes
         self.cache = FactorCache(cache_bytes) if cache_bytes > 0 else None
         self.streams = [t.cuda.Stream(device=dev) for _ in range(slots)]
+        self.graphs = None
+        if graphs and precision == "mixed":
+            from .graphs import GraphPool, pool_bytes
+            self.graphs = GraphPool()
+            self.budget -= pool_bytes(slots)
 
     def _execute(self, req: Request, stream):
         t = D.torch()
         with t.cuda.stream(stream):
             if req.kind == "resistance":
-                out = solve_resistance(req.payload, self.precision, self.cache, stream=stream)
+                out = solve_resistance(req.payload, self.precision, self.cache, stream=stream, graphs=self.graphs)
             elif req.kind == "swimmer":
                 out = run_swimmer(req.payload, self.precision, self.cache, stream=stream)
             else:

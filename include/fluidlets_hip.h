@@ -268,6 +268,19 @@ int fl_refine(const double* A64, int64_t lda64, const float* LU32, int64_t lda32
               const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
               double* resid_out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* fl_refine with a fixed number of corrections, for solves captured in a HIP
+ * graph: exactly `steps` corrections, no host synchronization and no memset
+ * (only kernel launches on stream), and the final relative residual of each
+ * column written to resid_dev (device, nrhs doubles; NaN if a triangular
+ * solve's block wait timed out).  For the same factors and steps, x is bit
+ * for bit fl_refine's when fl_refine stops after that many corrections.
+ * Same workspace as fl_refine.  No counterpart in the reference (its solves
+ * are scipy.linalg.solve at src/fluidlets/core/solve.py:66); used by the
+ * served small solves. */
+int fl_refine_steps(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32,
+                    int64_t n, const double* b, double* x, int64_t nrhs, int64_t ldb, int steps, double* resid_dev,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
 /* fl_refine with factors of the same operator written in another frame: A64
  * acts on n = 3P point-major rows (row 3p + a is component a of point p), and
  * LU32 factors the FP32 copy of A' with A64 ~ R A' R^T, R = blockdiag of the

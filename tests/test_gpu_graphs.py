@@ -1,0 +1,132 @@
+"""Small resistance solves captured as HIP graphs (fluidlets_amd.graphs) and
+the fixed-step refinement they replay (fl_refine_steps).
+
+Bars: every replay bit for bit equal to the same kernels launched directly,
+with other work between replays; the padded solve within the refinement
+tolerance of the oracle's dense FP64 solve of the unpadded matrix; resident
+padded factors bit for bit the graph's result; fl_refine_steps bit for bit
+fl_refine when fl_refine stops after the same number of corrections."""
+import numpy as np
+import pytest
+
+import oracle as O
+import fluidlets_amd as fl
+from fluidlets_amd import graphs as G
+from fluidlets_amd import linalg
+from fluidlets_amd._lib import lib
+from fluidlets_amd.core.solve import rigid_modes
+from fluidlets_amd.geometry import ellipsoid, ellipsoid_area
+
+pytestmark = pytest.mark.gpu
+
+
+def body(N, seed, wall=True):
+    rng = np.random.default_rng(seed)
+    axes = rng.uniform(0.5, 2.0, 3)
+    center = (0.0, 0.0, 2.5 + axes[2]) if wall else (0.0, 0.0, 0.0)
+    eps = 0.5 * np.sqrt(ellipsoid_area(axes) / N)
+    kernel = fl.WallStokeslet3D() if wall else fl.Brinkmanlet3D(float(rng.uniform(0, 10)))
+    return kernel, ellipsoid(N, axes, center), center, eps
+
+
+def test_refine_steps_is_refine_stopped_after_the_same_steps():
+    import torch
+    print("case: fl_refine_steps vs fl_refine n=1500", flush=True)
+    kernel, X, c, eps = body(500, 3)
+    K = O.assemble(kernel, X, X, eps)
+    A64 = torch.from_numpy(K).cuda()
+    F = linalg.getrf(A64.float(), refill=lambda A: A.copy_(A64))
+    b = torch.from_numpy(rigid_modes(X, c)).cuda()
+    x_ref, steps, res_ref = linalg.refine(A64, F, b, tol=1e-10)
+    assert 1 <= steps <= 6
+    x = torch.empty_like(b)
+    resid = torch.zeros(8, dtype=torch.float64, device="cuda")
+    ws = torch.empty(int(lib().fl_refine_workspace_size(len(K), 6)), dtype=torch.uint8, device="cuda")
+    linalg.refine_steps(A64, F, b.contiguous(), x, resid, steps, ws)
+    torch.cuda.synchronize()
+    assert torch.equal(x, x_ref)
+    assert np.array_equal(resid.cpu().numpy()[:6], res_ref)
+
+
+def test_graph_replays_equal_direct_launches():
+    """Three geometries of one bucket (different N, wall and Brinkman
+    kernels), each replayed and launched directly with other device work in
+    between, then replayed again: all equal bit for bit."""
+    import torch
+    print("case: graph replays vs direct launches, bucket 320 points", flush=True)
+    s = torch.cuda.Stream()
+    g = G.SolveGraph(G.bucket_rows(300), s)
+    assert g.n_pad == 960 and g.graph is not None
+    junk = torch.randn(1500, 1500, device="cuda", dtype=torch.float64)
+    for N, seed, wall in [(300, 1, True), (257, 2, False), (320, 3, True)]:
+        kernel, X, c, eps = body(N, seed, wall)
+        modes = rigid_modes(X, c)
+        x1, r1, _ = g.solve(kernel, X, eps, modes)
+        (junk @ junk).sum().item()
+        x2, r2, _ = g.solve(kernel, X, eps, modes, replay=False)
+        linalg.getrf(junk.clone()).info
+        x3, r3, _ = g.solve(kernel, X, eps, modes)
+        assert np.array_equal(x1, x2) and np.array_equal(x1, x3), (N, seed)
+        assert np.array_equal(r1, r2) and np.array_equal(r1, r3)
+        assert np.all(r1 <= 1e-10)
+    assert g.replays == 6
+
+
+@pytest.mark.parametrize("N,wall", [(200, True), (451, False), (640, True)])
+def test_padded_graph_solve_matches_the_dense_fp64_solve(N, wall):
+    """The padded system's first 3N rows: residual of the unpadded FP64
+    matrix within the tolerance and forward error within cond x tolerance of
+    LAPACK's solve."""
+    import torch
+    print(f"case: padded graph solve N={N} wall={wall}", flush=True)
+    kernel, X, c, eps = body(N, 10 + N, wall)
+    K = O.assemble(kernel, X, X, eps)
+    modes = rigid_modes(X, c)
+    g = G.SolveGraph(G.bucket_rows(N), torch.cuda.Stream())
+    x, res, _ = g.solve(kernel, X, eps, modes)
+    ref = np.linalg.solve(K, modes)
+    rel = np.linalg.norm(K @ x - modes, axis=0) / np.linalg.norm(modes, axis=0)
+    assert np.all(rel <= 1.5e-10) and np.all(res <= 1e-10)
+    cond = np.linalg.cond(K)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 2e-10 * cond
+
+
+def test_resident_padded_factors_reproduce_the_graph():
+    import torch
+    print("case: resident padded factors", flush=True)
+    kernel, X, c, eps = body(410, 21, False)
+    modes = rigid_modes(X, c)
+    g = G.SolveGraph(G.bucket_rows(410), torch.cuda.Stream())
+    x, res, F = g.solve(kernel, X, eps, modes)
+    K64p, Fp = g.resident(F)
+    other = body(400, 22, True)                    # the graph's buffers now hold another solve
+    g.solve(other[0], other[1], other[3], rigid_modes(other[1], other[2]))
+    y, res2 = G.solve_resident(K64p, Fp, modes)
+    assert np.array_equal(x, y) and np.array_equal(res, res2)
+
+
+def test_too_few_steps_fall_back_to_adaptive_refinement():
+    """One correction in the graph leaves the residual above 1e-10: the
+    solve continues with fl_refine from the start on the graph's factors."""
+    import torch
+    print("case: graph with one correction", flush=True)
+    kernel, X, c, eps = body(300, 5, True)
+    modes = rigid_modes(X, c)
+    g = G.SolveGraph(G.bucket_rows(300), torch.cuda.Stream(), steps=1)
+    x, res, F = g.solve(kernel, X, eps, modes)
+    assert np.all(res <= 1e-10)
+    xa, steps, _ = linalg.refine(g.K64, F, g.b, tol=1e-10)
+    assert steps > 1
+    assert np.array_equal(x, xa[:900].cpu().numpy())
+
+
+def test_pool_rejects_large_solves_and_shares_graphs_per_stream():
+    import torch
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    pool = G.GraphPool()
+    with pytest.raises(ValueError):
+        pool.get(G.GRAPH_MAX_POINTS + 1, s1)
+    a = pool.get(200, s1)
+    assert pool.get(250, s1) is a and pool.get(150, s1) is not a
+    assert pool.get(200, s2) is not a and pool.captures == 3
+    assert pool.nbytes() == 2 * G.graph_bytes(G.bucket_rows(200)) + G.graph_bytes(G.bucket_rows(150))

@@ -20,8 +20,12 @@ def small_mix():
 def test_results_alone_equal_results_in_the_mix(precision):
     print(f"case: serving alone vs mix {precision}", flush=True)
     reqs = small_mix()
-    mixed = SolveServer(slots=4, precision=precision).serve(reqs, realtime=True)
+    srv = SolveServer(slots=4, precision=precision)
+    mixed = srv.serve(reqs, realtime=True)
     assert all(r.error is None for r in mixed), [r.error for r in mixed]
+    if precision == "mixed":       # the small solves replayed captured graphs
+        small = [r for r in reqs if r.kind == "resistance" and len(r.payload["points"]) <= 640]
+        assert small and srv.graphs.replays == len(small)
     alone = SolveServer(slots=1, precision=precision)
     for r in reqs:
         ref = alone.serve([Request(r.kind, r.payload, 0.0, r.rid)], realtime=False)[0]
@@ -29,6 +33,20 @@ def test_results_alone_equal_results_in_the_mix(precision):
         assert np.array_equal(ref.value, mixed[r.rid].value), r.rid
     s = summarize(mixed)
     assert s["completed"] == len(reqs) and s["latency_p99_s"] >= s["latency_p50_s"] > 0
+
+
+def test_graph_and_direct_small_solves_agree():
+    """Small solves through the captured graphs (padded, fixed corrections)
+    and through the unpadded adaptive path: the same grand resistance to
+    the refinement tolerance."""
+    reqs = [r for r in small_mix() if r.kind == "resistance"]
+    reqs = [r for r in reqs if len(r.payload["points"]) <= 640][:3]
+    assert reqs
+    a = SolveServer(slots=2, precision="mixed").serve(reqs, realtime=False)
+    b = SolveServer(slots=2, precision="mixed", graphs=False).serve(reqs, realtime=False)
+    for x, y in zip(a, b):
+        assert x.error is None and y.error is None
+        assert np.allclose(x.value, y.value, rtol=0, atol=1e-9 * np.abs(y.value).max())
 
 
 def test_grand_resistance_matches_the_library_solve():

@@ -1,0 +1,36 @@
+"""The padding behind the captured small solves, checked with LAPACK on the
+CPU: partial pivoting on [[K, 0], [0, I]] never takes a padding row in the
+first n columns, the factors are K's next to I, and the padded solution is
+K's solution over zeros.  Random Gaussian K, so that every column pivots."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from fluidlets_amd import graphs as G
+
+
+def test_bucket_rows():
+    assert [G.bucket_rows(n) for n in (1, 64, 65, 128, 300, 640)] == [192, 192, 384, 384, 960, 1920]
+    assert G.bucket_rows(G.GRAPH_MAX_POINTS) == 3 * G.GRAPH_MAX_POINTS
+
+
+@pytest.mark.parametrize("n,n_pad", [(7, 192), (300, 384), (1157, 1920)])
+def test_identity_padding_keeps_the_factors_and_the_solution(n, n_pad):
+    rng = np.random.default_rng(n)
+    K = rng.standard_normal((n, n))
+    P = np.eye(n_pad)
+    P[:n, :n] = K
+    P[:n, n:] = 0.0
+    lu, piv = sl.lu_factor(K)
+    lu_p, piv_p = sl.lu_factor(P)
+    assert np.array_equal(piv_p[:n], piv)                  # the same interchanges, none with padding rows
+    assert np.array_equal(piv_p[n:], np.arange(n, n_pad))  # the padding is not interchanged
+    assert np.allclose(lu_p[:n, :n], lu, rtol=0, atol=1e-12 * np.abs(lu).max())
+    assert np.array_equal(lu_p[n:, :], P[n:, :]) and np.array_equal(lu_p[:n, n:], np.zeros((n, n_pad - n)))
+    b = rng.standard_normal((n, 6))
+    bp = np.zeros((n_pad, 6))
+    bp[:n] = b
+    x = sl.lu_solve((lu_p, piv_p), bp)
+    assert np.array_equal(x[n:], np.zeros((n_pad - n, 6)))
+    ref = np.linalg.solve(K, b)
+    assert np.linalg.norm(x[:n] - ref) <= 1e-13 * np.linalg.cond(K) * np.linalg.norm(ref)
