@@ -77,6 +77,8 @@ int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, i
 int dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb, void* ws,
            size_t wsb, hipStream_t st);
 int getri_impl(const float* LU, int64_t n, int64_t lda, const void* aux, float* X, int64_t ldx, hipStream_t st);
+int pad_solve_impl(double* K64, float* K32, int64_t n_pad, const double* n_dev, const double* rhs, double* b,
+                   int64_t nrhs, hipStream_t st);
 int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux, int64_t n,
                 const double* b, double* x, int64_t nrhs, int64_t ldb, double tol, int max_iter, int* iters_out,
                 double* resid_out, const double* frame, void* ws, size_t ws_bytes, hipStream_t st,
@@ -279,6 +281,13 @@ int fl_refine_steps(const double* A64, int64_t lda64, const float* LU32, int64_t
     if (steps < 0) return set_error(FL_EINVAL, "fl_refine_steps: negative step count");
     return refine_impl(A64, lda64, LU32, lda32, aux32, n, b, x, nrhs, ldb, 0.0, steps, nullptr, nullptr, nullptr,
                        workspace, workspace_bytes, as_stream(stream), nullptr, 0, resid_dev);
+}
+
+int fl_pad_solve(double* K64, float* K32, int64_t n_pad, const double* n_dev, const double* rhs, double* b,
+                 int64_t nrhs, void* stream)
+{
+    clear_error();
+    return pad_solve_impl(K64, K32, n_pad, n_dev, rhs, b, nrhs, as_stream(stream));
 }
 
 int fl_sgetri(const float* LU, int64_t n, int64_t lda, const void* aux, float* Ainv, int64_t ldi, void* stream)

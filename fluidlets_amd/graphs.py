@@ -46,7 +46,7 @@ from ._lib import FL_FP32C, FL_LU_FP32, check, kernel_struct, lib
 
 GRAPH_MAX_POINTS = 640     # n_pad <= 1920
 BUCKET_POINTS = 64
-GRAPH_STEPS = 3            # corrections inside the graph
+GRAPH_STEPS = 2            # corrections inside the graph (what 1e-10 takes on resistance matrices of these sizes)
 NRHS = 6                   # the six rigid-body modes
 
 
@@ -59,7 +59,7 @@ def graph_bytes(n_pad: int, nrhs: int = NRHS) -> int:
     """Device bytes one graph's buffers hold."""
     L = lib()
     return int(12 * n_pad * n_pad + L.fl_getrf_aux_size(n_pad, FL_FP32C) + L.fl_getrf_workspace_size(n_pad, FL_FP32C)
-               + L.fl_refine_workspace_size(n_pad, nrhs) + 16 * n_pad * nrhs + 4 * n_pad + 1024)
+               + L.fl_refine_workspace_size(n_pad, nrhs) + 24 * n_pad * nrhs + 4 * n_pad + 32 * (n_pad // 3) + 1024)
 
 
 def pool_bytes(streams: int) -> int:
@@ -69,12 +69,11 @@ def pool_bytes(streams: int) -> int:
                          for k in range(1, GRAPH_MAX_POINTS // BUCKET_POINTS + 1))
 
 
-def _finish(K64, F, b, x, res_host, x_host, n, tol, stream):
-    """After the fixed corrections: adaptive refinement from the start if any
-    column's residual exceeds tol (or is NaN).  Returns the first n rows of
-    x (host) and the residuals."""
-    nrhs = int(b.shape[1])
-    res = res_host[:nrhs].numpy().copy()
+def _finish(K64, F, b, x, res, x_host, n, tol, stream):
+    """After the fixed corrections (residuals res, x_host[:n] on the host):
+    adaptive refinement from the start if any column's residual exceeds tol
+    (or is NaN).  Returns x[:n] (host) and the residuals."""
+    res = np.array(res, dtype=np.float64)
     if not np.all(res <= tol):
         xa, _, res = linalg.refine(K64, F, b, tol=tol, stream=stream)
         x.copy_(xa)
@@ -84,8 +83,13 @@ def _finish(K64, F, b, x, res_host, x_host, n, tol, stream):
 
 
 class SolveGraph:
-    """The buffers of one padded order on one stream and the graph that
-    factors and refines in them."""
+    """The buffers of one padded order on one stream and the graph that pads,
+    factors and refines in them.
+
+    Inputs travel in one pinned staging buffer and one copy: points (3 x N),
+    blob widths (N, or one value broadcast), N as a double, right-hand sides
+    (N x nrhs).  Outputs come back in one copy: the residuals, the LU's
+    status word and x."""
 
     def __init__(self, n_pad: int, stream, steps: int = GRAPH_STEPS, nrhs: int = NRHS, capture: bool = True):
         t = D.torch()
@@ -96,20 +100,25 @@ class SolveGraph:
         if not 1 <= nrhs <= 8 or steps < 1:
             raise ValueError("need 1 <= nrhs <= 8 and at least one correction")
         self.n_pad, self.steps, self.nrhs, self.stream = n_pad, int(steps), int(nrhs), stream
+        P = n_pad // 3
+        self.off_eps, self.off_n, self.off_rhs = 3 * P, 4 * P, 4 * P + 8
+        self.stage_host = t.empty((self.off_rhs + n_pad * nrhs,), dtype=t.float64, pin_memory=True)
+        self.stage = t.zeros((self.off_rhs + n_pad * nrhs,), dtype=t.float64, device=dev)
         self.K64 = t.empty((n_pad, n_pad), dtype=t.float64, device=dev)
         self.K32 = t.empty((n_pad, n_pad), dtype=t.float32, device=dev)
         self.F = linalg.LUFactor(self.K32, t.empty((n_pad,), dtype=t.int32, device=dev),
                                  t.empty((int(L.fl_getrf_aux_size(n_pad, FL_FP32C)),), dtype=t.uint8, device=dev),
                                  n_pad, extra={"async": True, "products": "fp32"})
-        self.status = t.zeros((1,), dtype=t.int32, device=dev)
         self.ws_lu = t.empty((int(L.fl_getrf_workspace_size(n_pad, FL_FP32C)),), dtype=t.uint8, device=dev)
         self.b = t.zeros((n_pad, nrhs), dtype=t.float64, device=dev)
-        self.x = t.zeros((n_pad, nrhs), dtype=t.float64, device=dev)
-        self.resid = t.zeros((8,), dtype=t.float64, device=dev)
+        # out = [residuals (8) | status word (int32 in slot 8) | unused | x (n_pad x nrhs)]
+        self.out = t.zeros((16 + n_pad * nrhs,), dtype=t.float64, device=dev)
+        self.resid = self.out[:8]
+        self.status = self.out.view(t.int32)[16:17]
+        self.x = self.out[16:].view(n_pad, nrhs)
+        self.out_host = t.empty((16 + n_pad * nrhs,), dtype=t.float64, pin_memory=True)
+        self.x_host = self.out_host[16:].view(n_pad, nrhs)
         self.ws_ref = t.empty((int(L.fl_refine_workspace_size(n_pad, nrhs)),), dtype=t.uint8, device=dev)
-        self.x_host = t.empty((n_pad, nrhs), dtype=t.float64, pin_memory=True)
-        self.res_host = t.empty((8,), dtype=t.float64, pin_memory=True)
-        self.status_host = t.empty((1,), dtype=t.int32, pin_memory=True)
         self.replays = 0
         self.graph = None
         if capture:
@@ -123,12 +132,16 @@ class SolveGraph:
             self.graph = g
 
     def _launch(self):
-        """The graph's work: asynchronous LU of K32, fixed-step refinement of
-        K64 x = b from it."""
+        """The graph's work: identity padding and right-hand sides, the
+        asynchronous LU of K32, fixed-step refinement of K64 x = b."""
+        L = lib()
         n = self.n_pad
-        check(lib().fl_sgetrf_async_ex(D.ptr(self.K32), n, n, D.ptr(self.F.ipiv), D.ptr(self.F.aux),
-                                       D.ptr(self.status), FL_LU_FP32, D.ptr(self.ws_lu), self.ws_lu.numel(),
-                                       D.stream_handle(self.stream)), "getrf_async")
+        sh = D.stream_handle(self.stream)
+        base = D.ptr(self.stage)
+        check(L.fl_pad_solve(D.ptr(self.K64), D.ptr(self.K32), n, base + 8 * self.off_n, base + 8 * self.off_rhs,
+                             D.ptr(self.b), self.nrhs, sh), "pad_solve")
+        check(L.fl_sgetrf_async_ex(D.ptr(self.K32), n, n, D.ptr(self.F.ipiv), D.ptr(self.F.aux), D.ptr(self.status),
+                                   FL_LU_FP32, D.ptr(self.ws_lu), self.ws_lu.numel(), sh), "getrf_async")
         linalg.refine_steps(self.K64, self.F, self.b, self.x, self.resid, self.steps, self.ws_ref, stream=self.stream)
 
     def solve(self, kernel, points, eps, rhs, tol: float = 1e-10, replay: bool = True):
@@ -145,40 +158,43 @@ class SolveGraph:
         if N == 0 or n > self.n_pad or rhs.shape != (n, self.nrhs):
             raise ValueError(f"{N} points and rhs {rhs.shape} do not fit padded order {self.n_pad}, {self.nrhs} columns")
         st = self.stream
+        q = self.nrhs
+        h = self.stage_host.numpy()        # free: the previous solve on this stream ended with a synchronization
+        h[:n] = points.T.ravel()
+        e = np.asarray(eps, dtype=np.float64)
+        if e.size == 1:
+            h[self.off_eps] = float(e.reshape(-1)[0])
+            stride = 0
+        else:
+            h[self.off_eps:self.off_eps + N] = np.broadcast_to(e, (N,))
+            stride = 1
+        h[self.off_n] = float(n)
+        used = self.off_rhs + n * q
+        h[self.off_rhs:used] = rhs.ravel()
+        back = 16 + n * q
         with t.cuda.stream(st):
-            dev = self.K64.device
-            x_soa = D.soa(points, dev)
-            eps_t, stride = D.eps_arg(eps, N, dev)
-            for K in (self.K64, self.K32):       # the identity padding around the leading n x n block
-                K[n:].zero_()
-                K[:n, n:].zero_()
-                K.diagonal()[n:].fill_(1.0)
-            check(lib().fl_assemble(kernel_struct(kernel), N, N, D.ptr(x_soa), D.ptr(x_soa), D.ptr(eps_t), stride,
-                                    1.0, 0.0, D.ptr(self.K64), self.n_pad, D.ptr(self.K32), self.n_pad, None,
+            self.stage[:used].copy_(self.stage_host[:used], non_blocking=True)
+            base = D.ptr(self.stage)
+            check(lib().fl_assemble(kernel_struct(kernel), N, N, base, base, base + 8 * self.off_eps, stride, 1.0,
+                                    0.0, D.ptr(self.K64), self.n_pad, D.ptr(self.K32), self.n_pad, None,
                                     D.stream_handle(st)), "fl_assemble")
-            self.b[:n].copy_(t.from_numpy(rhs))
-            self.b[n:].zero_()
             if replay and self.graph is not None:
                 self.graph.replay()
                 self.replays += 1
             else:
                 self._launch()
-            self.status_host.copy_(self.status, non_blocking=True)
-            self.res_host.copy_(self.resid, non_blocking=True)
-            self.x_host[:n].copy_(self.x[:n], non_blocking=True)
+            self.out_host[:back].copy_(self.out[:back], non_blocking=True)
             st.synchronize()
             F = self.F
-            if int(self.status_host[0]) != 0:   # interchanges needed: the verified factorization
+            if int(self.out_host.view(t.int32)[16]) != 0:   # interchanges needed: the verified factorization
                 self.K32.copy_(self.K64)
                 F = linalg.getrf(self.K32, stream=st)
                 if F.info > 0:
                     raise np.linalg.LinAlgError("Matrix is singular.")
                 linalg.refine_steps(self.K64, F, self.b, self.x, self.resid, self.steps, self.ws_ref, stream=st)
-                self.res_host.copy_(self.resid, non_blocking=True)
-                self.x_host[:n].copy_(self.x[:n], non_blocking=True)
+                self.out_host[:back].copy_(self.out[:back], non_blocking=True)
                 st.synchronize()
-            x, res = _finish(self.K64, F, self.b, self.x, self.res_host, self.x_host, n, tol, st)
-        return x, res, F
+            return (*_finish(self.K64, F, self.b, self.x, self.out_host[:q].numpy(), self.x_host, n, tol, st), F)
 
     def resident(self, F):
         """Copies of the padded FP64 matrix and the factors F (this graph's or
@@ -208,10 +224,10 @@ def solve_resident(K64, F, rhs, steps: int = GRAPH_STEPS, tol: float = 1e-10, st
         resid = t.zeros((8,), dtype=t.float64, device=dev)
         ws = t.empty((int(lib().fl_refine_workspace_size(n_pad, nrhs)),), dtype=t.uint8, device=dev)
         linalg.refine_steps(K64, F, b, x, resid, steps, ws, stream=st)
-        res_host = resid.cpu()
+        res = resid[:nrhs].cpu().numpy()
         x_host = t.empty((n_pad, nrhs), dtype=t.float64)
         x_host[:n].copy_(x[:n])
-        return _finish(K64, F, b, x, res_host, x_host, n, tol, st)
+        return _finish(K64, F, b, x, res, x_host, n, tol, st)
 
 
 class GraphPool:
