@@ -234,6 +234,23 @@ def test_bf16x3_products_with_row_interchanges():
     assert rel_err(x.cpu().numpy(), np.linalg.solve(A64, b)) < 1e-8 * np.linalg.cond(A64) * 1e-2
 
 
+@pytest.mark.parametrize("n", [2700, 4100])
+def test_bf16x3_factorization_repeats_bit_for_bit(n):
+    """The split-bf16 trailing updates (library GEMMs chosen without
+    workspace, so without split-K partials) give the same factors on every
+    run, on the verified and on the asynchronous path."""
+    from fluidlets_amd import linalg
+    print(f"case: sgetrf bf16x3 repeatability n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    D = (rng.standard_normal((n, n)) + n * np.eye(n)).astype(np.float32)   # no interchanges: the async path
+    for M, refill in ((A, None), (D, lambda T: T.copy_(dev(D)))):
+        F1 = linalg.getrf(dev(M.copy()), products="bf16x3", refill=refill)
+        F2 = linalg.getrf(dev(M.copy()), products="bf16x3", refill=refill)
+        assert F1.info == 0 and F1.extra.get("async", False) == (refill is not None)
+        assert np.array_equal(F1.lu.cpu().numpy(), F2.lu.cpu().numpy())
+
+
 def test_bf16x3_products_are_ignored_for_fp64_and_small_matrices():
     from fluidlets_amd import linalg
     rng = np.random.default_rng(2)
