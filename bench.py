@@ -364,20 +364,20 @@ def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4,
     return out
 
 
-def small_solves_section(slots=4, seed=5, n_requests=192, passes=3):
+def small_solves_section(slots=4, seed=5, n_requests=192, passes=3, modes=("graphs", "direct")):
     """Small six-mode resistance solves (N uniform in [200, 640] points, the
     serving mix's ellipsoids and kernels), all queued at t = 0, served with
     the captured graphs (padded buckets, fixed corrections) and without them
     (unpadded, adaptive refinement), on one slot and on `slots` slots.  Each
     server serves the trace once to warm up (graph captures, allocations),
     then `passes` times timed; the pass with the median throughput is
-    reported."""
+    reported.  ``modes`` selects the paths ("graphs", "direct")."""
     from fluidlets_amd.serving import SolveServer, serving_mix, summarize
 
     reqs = serving_mix(n_requests, seed=seed + 2, n_min=200, n_max=640, swimmer_frac=0.0, rate=1.0)
     out = {"requests": n_requests, "n_points": [200, 640], "passes": passes}
     for k in sorted({1, slots}):
-        for graphs in (True, False):
+        for graphs in [m == "graphs" for m in modes]:
             srv = SolveServer(slots=k, precision="mixed", graphs=graphs)
             srv.serve(reqs, realtime=False)
             runs = sorted((summarize(srv.serve(reqs, realtime=False)) for _ in range(passes)),
@@ -390,7 +390,9 @@ def small_solves_section(slots=4, seed=5, n_requests=192, passes=3):
                 out[tag]["graph_captures"] = srv.graphs.captures
             progress(f"serving: small solves {tag} {s['solves_per_s']:.1f} requests/s")
     for k in sorted({1, slots}):
-        out[f"graph_speedup_{k}_slots"] = out[f"graphs_{k}_slots"]["solves_per_s"] / out[f"direct_{k}_slots"]["solves_per_s"]
+        if f"graphs_{k}_slots" in out and f"direct_{k}_slots" in out:
+            out[f"graph_speedup_{k}_slots"] = (out[f"graphs_{k}_slots"]["solves_per_s"]
+                                               / out[f"direct_{k}_slots"]["solves_per_s"])
     return out
 
 

@@ -1499,7 +1499,13 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     unsigned int* flag = reinterpret_cast<unsigned int*>(tail);      // largest |multiplier| of a panel
     int32_t* info_blk = reinterpret_cast<int32_t*>(tail + 4);        // zero pivot found by tier 1
     int32_t* info_d = reinterpret_cast<int32_t*>(tail + 64);
-    uint16_t* split = split_products && std::is_same_v<T, float> ? reinterpret_cast<uint16_t*>(tail + 256) : nullptr;
+    // The split products run with NBO_SPLIT-column panels, and pay only where a
+    // trailing update then reaches the library GEMM (n - NBO_SPLIT >= 2048).
+    // Below that the factorization is the FP32-product one (256-column
+    // panels, faster there).
+    uint16_t* split = split_products && std::is_same_v<T, float> && n - NBO_SPLIT >= 2048
+                          ? reinterpret_cast<uint16_t*>(tail + 256)
+                          : nullptr;
     if (spec) {
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, flag, (int64_t)1);
         FL_LAUNCHED();

@@ -251,10 +251,13 @@ def test_bf16x3_factorization_repeats_bit_for_bit(n):
         assert np.array_equal(F1.lu.cpu().numpy(), F2.lu.cpu().numpy())
 
 
-def test_bf16x3_products_are_ignored_for_fp64_and_small_matrices():
+@pytest.mark.parametrize("n", [700, 2559])
+def test_bf16x3_products_are_ignored_for_fp64_and_small_matrices(n):
+    """Below n = 2560 no trailing update is split, and the factors are those
+    of the FP32 products bit for bit (and FP64 ignores the setting)."""
     from fluidlets_amd import linalg
     rng = np.random.default_rng(2)
-    A = rng.standard_normal((700, 700))
+    A = rng.standard_normal((n, n))
     F64a = linalg.getrf(dev(A.copy()))
     F64b = linalg.getrf(dev(A.copy()), products="bf16x3")
     assert np.array_equal(F64a.lu.cpu().numpy(), F64b.lu.cpu().numpy())
