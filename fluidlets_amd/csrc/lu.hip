@@ -1976,21 +1976,21 @@ namespace {
 
 // The identity padding of an order-n solve held in buffers of order n_pad
 // (row-major, leading dimension n_pad): rows and columns n..n_pad-1 of K64
-// and K32 become the identity's, and b (n_pad x nrhs) gets rhs (n x nrhs)
-// over zeros.  n is read on the device (a double at n_dev), so that one
-// captured graph serves every n <= n_pad.  One workgroup per row; the
-// leading n x n block is left alone.
+// and K32 (either may be null) become the identity's, and b (n_pad x nrhs)
+// gets rhs (n x nrhs) over zeros.  n is read on the device (a double at
+// n_dev), so that one captured graph serves every n <= n_pad.  One
+// workgroup per row; the leading n x n block is left alone.
 __global__ __launch_bounds__(256) void pad_solve_kernel(double* K64, float* K32, int64_t n_pad, const double* n_dev,
                                                         const double* rhs, double* b, int nrhs)
 {
     const int64_t n = std::min<int64_t>(std::max<int64_t>((int64_t)n_dev[0], 0), n_pad);
     const int64_t i = blockIdx.x;
-    double* r64 = K64 + i * n_pad;
-    float* r32 = K32 + i * n_pad;
+    double* r64 = K64 ? K64 + i * n_pad : nullptr;
+    float* r32 = K32 ? K32 + i * n_pad : nullptr;
     for (int64_t c = (i < n ? n : 0) + threadIdx.x; c < n_pad; c += blockDim.x) {
         const double v = c == i ? 1.0 : 0.0;
-        r64[c] = v;
-        r32[c] = (float)v;
+        if (r64) r64[c] = v;
+        if (r32) r32[c] = (float)v;
     }
     if ((int)threadIdx.x < nrhs) b[i * nrhs + threadIdx.x] = i < n ? rhs[i * nrhs + threadIdx.x] : 0.0;
 }
@@ -2003,7 +2003,7 @@ int pad_solve_impl(double* K64, float* K32, int64_t n_pad, const double* n_dev, 
     FL_CHECK(n_pad >= 0, "negative order");
     FL_CHECK(nrhs >= 1 && nrhs <= 8, "need 1 <= nrhs <= 8");
     if (n_pad == 0) return FL_OK;
-    FL_CHECK(K64 && K32 && n_dev && rhs && b, "null pointer argument");
+    FL_CHECK(n_dev && rhs && b, "null pointer argument");
     hipLaunchKernelGGL(pad_solve_kernel, dim3((unsigned)n_pad), dim3(256), 0, st, K64, K32, n_pad, n_dev, rhs, b,
                        (int)nrhs);
     FL_LAUNCHED();

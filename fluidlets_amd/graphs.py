@@ -48,6 +48,45 @@ GRAPH_MAX_POINTS = 640     # n_pad <= 1920
 BUCKET_POINTS = 64
 GRAPH_STEPS = 2            # corrections inside the graph (what 1e-10 takes on resistance matrices of these sizes)
 NRHS = 6                   # the six rigid-body modes
+PAD_MULTIPLE = 128         # larger served solves: order rounded up to this
+
+
+def padded_order(n: int, multiple: int = PAD_MULTIPLE) -> int:
+    return -(-int(n) // multiple) * multiple
+
+
+def assemble_padded(kernel, points, eps, rhs, fp32: bool = True, stream=None, multiple: int = PAD_MULTIPLE):
+    """The resistance matrix of ``kernel`` on the (N, 3) ``points`` (mu = 1)
+    in the leading block of identity-padded buffers of order n_pad = 3N
+    rounded up to ``multiple``, and the right-hand sides (3N, q) host over
+    zeros.  The factorization of a ragged order is up to a quarter slower
+    than at the next multiple of 128 (its GEMM tiles and rows are
+    misaligned); the padding leaves the solution unchanged.  Returns (K64,
+    K32 or None, b (n_pad, q), n)."""
+    t = D.torch()
+    dev = D.device()
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    N = len(points)
+    n = 3 * N
+    q = int(rhs.shape[1])
+    if N == 0 or rhs.shape[0] != n or not 1 <= q <= 8:
+        raise ValueError(f"{N} points do not match right-hand sides of shape {rhs.shape}")
+    n_pad = padded_order(n, multiple)
+    st = stream if stream is not None else t.cuda.current_stream()
+    L = lib()
+    with t.cuda.stream(st):
+        x_soa = D.soa(points, dev)
+        eps_t, stride = D.eps_arg(eps, N, dev)
+        K64 = t.empty((n_pad, n_pad), dtype=t.float64, device=dev)
+        K32 = t.empty((n_pad, n_pad), dtype=t.float32, device=dev) if fp32 else None
+        check(L.fl_assemble(kernel_struct(kernel), N, N, D.ptr(x_soa), D.ptr(x_soa), D.ptr(eps_t), stride, 1.0, 0.0,
+                            D.ptr(K64), n_pad, D.ptr(K32), n_pad, None, D.stream_handle(st)), "fl_assemble")
+        stage = t.from_numpy(np.concatenate([[float(n)], rhs.ravel()])).to(dev)   # n, then the right-hand sides
+        b = t.empty((n_pad, q), dtype=t.float64, device=dev)
+        check(L.fl_pad_solve(D.ptr(K64), D.ptr(K32), n_pad, D.ptr(stage), D.ptr(stage) + 8, D.ptr(b), q,
+                             D.stream_handle(st)), "pad_solve")
+    return K64, K32, b, n
 
 
 def bucket_rows(n_points: int) -> int:
@@ -231,15 +270,30 @@ def solve_resident(K64, F, rhs, steps: int = GRAPH_STEPS, tol: float = 1e-10, st
 
 
 class GraphPool:
-    """Captured small-solve graphs, one per (padded order, stream), made on
-    first use.  A stream serves one request at a time, so a graph is never
-    replayed by two threads at once."""
+    """Captured small-solve graphs, one per (padded order, stream).
 
-    def __init__(self, steps: int = GRAPH_STEPS):
+    With ``streams`` every bucket's graph is captured for every stream at
+    construction, by the constructing thread, and ``get`` never captures:
+    a server builds its pool before its worker threads start, so no capture
+    overlaps another thread's runtime calls or another capture (a capture
+    made in a worker thread while other workers factored and captured ended
+    the process in the HIP runtime).  Without ``streams`` graphs are
+    captured on first use, for single-threaded callers.  A stream serves one
+    request at a time, so a graph is never replayed by two threads at once."""
+
+    def __init__(self, steps: int = GRAPH_STEPS, streams=None):
         self.steps = steps
         self._graphs: dict = {}
         self._lock = threading.Lock()
         self.captures = 0
+        self.lazy = streams is None
+        for s in streams or ():
+            for k in range(1, GRAPH_MAX_POINTS // BUCKET_POINTS + 1):
+                rows = 3 * BUCKET_POINTS * k
+                self._graphs[(rows, D.stream_handle(s))] = SolveGraph(rows, s, self.steps)
+                self.captures += 1
+        if streams is not None:
+            D.torch().cuda.synchronize()
 
     def get(self, n_points: int, stream) -> SolveGraph:
         if not 0 < n_points <= GRAPH_MAX_POINTS:
@@ -248,6 +302,8 @@ class GraphPool:
         with self._lock:
             g = self._graphs.get(key)
         if g is None:
+            if not self.lazy:
+                raise KeyError(f"no graph captured for {key[0]} rows on this stream")
             g = SolveGraph(key[0], stream, self.steps)
             with self._lock:
                 self._graphs[key] = g

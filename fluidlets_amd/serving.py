@@ -28,6 +28,7 @@ import hashlib
 import heapq
 import threading
 import time
+from contextlib import nullcontext
 from dataclasses import dataclass
 from typing import Any
 
@@ -68,7 +69,8 @@ def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) ->
     without the server's shared cache, its own cache of body-frame
     factors."""
     if req.kind == "resistance":
-        n = 3 * len(req.payload["points"])
+        from .graphs import padded_order
+        n = padded_order(3 * len(req.payload["points"]))
     else:
         n = 3 * req.payload["n_points"]
     per = 8 + (4 if precision == "mixed" else 8)
@@ -108,7 +110,6 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     graphs.GRAPH_MAX_POINTS points run as a captured graph on a padded
     system (resident factors then stay padded)."""
     from . import linalg
-    from .core.assemble import assemble_device
     from .core.solve import rigid_modes
 
     t = D.torch()
@@ -129,14 +130,14 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
             else:
                 x, _ = G.solve_resident(item[0], item[1], modes, steps=graphs.steps, stream=stream)
             return _grand(X, center, x, payload.get("mu", 1.0))
+    # larger solves: identity-padded to a multiple of 128 rows (graphs.assemble_padded)
+    from .graphs import assemble_padded
     key = geometry_key(kernel, X, eps) + precision if cache is not None else None
     item = cache.get(key) if cache is not None else None
-    rhs = t.from_numpy(rigid_modes(X, center)).to(dev)
+    modes = rigid_modes(X, center)
+    n = 3 * len(X)
     if item is None:
-        x_soa = D.soa(X, dev)
-        eps_t, stride = D.eps_arg(eps, len(X), dev)
-        K64, K32 = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=(precision == "mixed"),
-                                   stream=stream)
+        K64, K32, b, _ = assemble_padded(kernel, X, eps, modes, fp32=(precision == "mixed"), stream=stream)
         if precision == "mixed":
             F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3")
         else:
@@ -146,13 +147,17 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
         item = (K64, F)
         if cache is not None:
             cache.put(key, item, K64.numel() * 8 + F.nbytes())
+    else:
+        with t.cuda.stream(stream) if stream is not None else nullcontext():
+            b = t.zeros((int(item[0].shape[0]), 6), dtype=t.float64, device=dev)
+            b[:n].copy_(t.from_numpy(modes))
     K64, F = item[0], item[1]
     if precision == "mixed":
-        x, steps, _ = linalg.refine(K64, F, rhs, tol=1e-10, stream=stream)
+        x, steps, _ = linalg.refine(K64, F, b, tol=1e-10, stream=stream)
     else:
-        x = rhs.clone()
+        x = b
         linalg.getrs(F, x, stream=stream)
-    return _grand(X, center, x.cpu().numpy(), payload.get("mu", 1.0))
+    return _grand(X, center, x[:n].cpu().numpy(), payload.get("mu", 1.0))
 
 
 def run_swimmer(payload: dict, precision: str, cache: FactorCache | None = None, stream=None):
@@ -184,7 +189,7 @@ class SolveServer:
         self.graphs = None
         if graphs and precision == "mixed":
             from .graphs import GraphPool, pool_bytes
-            self.graphs = GraphPool()
+            self.graphs = GraphPool(streams=self.streams)   # every capture here, before any worker thread
             self.budget -= pool_bytes(slots)
 
     def _execute(self, req: Request, stream):

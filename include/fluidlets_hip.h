@@ -283,12 +283,15 @@ int fl_refine_steps(const double* A64, int64_t lda64, const float* LU32, int64_t
 
 /* Identity padding for a solve of order n <= n_pad captured in a HIP graph:
  * rows and columns n..n_pad-1 of K64 and K32 (row-major, leading dimension
- * n_pad) are set to the identity's, and b (n_pad x nrhs, nrhs <= 8) receives
+ * n_pad; either may be null) are set to the identity's, and b (n_pad x nrhs,
+ * nrhs <= 8) receives
  * rhs (n x nrhs, device) over zeros; the leading n x n block is not touched.
  * n is read on the device from n_dev (one double), so one graph serves every
  * order up to n_pad.  [[K, 0], [0, I]] has K's factors next to I and the
  * padded solution is K's over zeros.  No counterpart in the reference; used
- * by the served small solves. */
+ * by the served solves (captured small ones, and larger ones padded to a
+ * multiple of 128, whose factorization is up to a quarter faster than at a
+ * ragged order). */
 int fl_pad_solve(double* K64, float* K32, int64_t n_pad, const double* n_dev, const double* rhs, double* b,
                  int64_t nrhs, void* stream);
 

@@ -120,6 +120,19 @@ def test_too_few_steps_fall_back_to_adaptive_refinement():
     assert np.array_equal(x, xa[:900].cpu().numpy())
 
 
+def test_pool_captures_every_bucket_up_front_for_given_streams():
+    import torch
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    pool = G.GraphPool(streams=[s1, s2])
+    nb = G.GRAPH_MAX_POINTS // G.BUCKET_POINTS
+    assert pool.captures == 2 * nb and not pool.lazy
+    assert pool.nbytes() <= G.pool_bytes(2)
+    g = pool.get(300, s2)
+    assert g.n_pad == G.bucket_rows(300) and pool.get(300, s2) is g and pool.captures == 2 * nb
+    with pytest.raises(KeyError):
+        pool.get(300, torch.cuda.Stream())
+
+
 def test_pool_rejects_large_solves_and_shares_graphs_per_stream():
     import torch
     s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
@@ -130,3 +143,23 @@ def test_pool_rejects_large_solves_and_shares_graphs_per_stream():
     assert pool.get(250, s1) is a and pool.get(150, s1) is not a
     assert pool.get(200, s2) is not a and pool.captures == 3
     assert pool.nbytes() == 2 * G.graph_bytes(G.bucket_rows(200)) + G.graph_bytes(G.bucket_rows(150))
+
+
+@pytest.mark.parametrize("N,multiple", [(331, 128), (700, 128), (64, 192)])
+def test_assemble_padded_is_the_matrix_over_the_identity(N, multiple):
+    """fl_assemble into the leading block and fl_pad_solve around it: the
+    oracle's matrix, the identity below and to the right, the right-hand
+    sides over zeros; the FP32 copy its rounding."""
+    kernel, X, c, eps = body(N, 40 + N, N % 2 == 1)
+    modes = rigid_modes(X, c)
+    K64, K32, b, n = G.assemble_padded(kernel, X, eps, modes, multiple=multiple)
+    n_pad = G.padded_order(3 * N, multiple)
+    assert n == 3 * N and tuple(K64.shape) == (n_pad, n_pad) and n_pad % multiple == 0
+    P = np.eye(n_pad)
+    P[:n, :n] = O.assemble(kernel, X, X, eps)
+    got = K64.cpu().numpy()
+    assert np.array_equal(got[n:], P[n:]) and np.array_equal(got[:n, n:], P[:n, n:])
+    assert np.abs(got[:n, :n] - P[:n, :n]).max() <= 1e-13 * np.abs(P).max()
+    assert np.array_equal(K32.cpu().numpy(), got.astype(np.float32))
+    bb = b.cpu().numpy()
+    assert np.array_equal(bb[:n], modes) and not bb[n:].any()

@@ -49,6 +49,21 @@ def test_graph_and_direct_small_solves_agree():
         assert np.allclose(x.value, y.value, rtol=0, atol=1e-9 * np.abs(y.value).max())
 
 
+def test_padded_direct_solves_match_the_library():
+    """Requests above the graphs' range run identity-padded to a multiple of
+    128 rows: the same grand resistance as the library's unpadded FP64
+    solve, to the refinement tolerance (mixed) and to rounding (FP64)."""
+    reqs = [r for r in small_mix() if r.kind == "resistance" and len(r.payload["points"]) > 640][:2]
+    assert len(reqs) == 2 and all(3 * len(r.payload["points"]) % 128 for r in reqs)
+    for precision, tol in (("mixed", 1e-9), ("fp64", 1e-12)):
+        out = SolveServer(slots=2, precision=precision).serve(reqs, realtime=False)
+        for r, o in zip(reqs, out):
+            assert o.error is None, o.error
+            p = r.payload
+            R, _, _ = fl.grand_resistance(p["kernel"], p["points"], p["center"], p["eps"])
+            assert np.allclose(o.value, R, rtol=0, atol=tol * np.abs(R).max()), precision
+
+
 def test_grand_resistance_matches_the_library_solve():
     reqs = [r for r in small_mix() if r.kind == "resistance"][:2]
     out = SolveServer(slots=2, precision="fp64").serve(reqs, realtime=False)
