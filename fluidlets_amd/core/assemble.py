@@ -8,6 +8,8 @@ target i, column q*j + b is component b of source j, so
 """
 from __future__ import annotations
 
+from contextlib import nullcontext
+
 import numpy as np
 
 from .. import _device as D
@@ -40,6 +42,59 @@ def assemble_device(kernel, tgt, src, eps_t, eps_stride, mu=1.0, fp64=True, fp32
                        D.ptr(K64), 3 * N, D.ptr(K32), 3 * N, None, D.stream_handle(stream))
     check(st, "fl_assemble")
     return K64, K32
+
+
+PAD_MULTIPLE = 128   # dense solves run at orders rounded up to this
+
+
+def padded_order(n: int, multiple: int = PAD_MULTIPLE) -> int:
+    return -(-int(n) // multiple) * multiple
+
+
+def assemble_padded_device(kernel, x_soa, eps_t, eps_stride, rhs, fp32: bool = True, stream=None,
+                           multiple: int = PAD_MULTIPLE):
+    """The resistance matrix of ``kernel`` on the (3, N) device points (mu =
+    1) in the leading block of identity-padded buffers of order n_pad = 3N
+    rounded up to ``multiple``, and the right-hand sides rhs (3N, q, float64,
+    device) over zeros: [[K, 0], [0, I]] [x; 0] = [rhs; 0].  Partial pivoting
+    never takes a padding row in the first 3N columns, so the factors are
+    K's next to I and the solution is K's over zeros; the factorization of a
+    ragged order is up to a quarter slower than at the next multiple of 128
+    (misaligned rows and partial GEMM tiles).  Returns (K64, K32 or None,
+    b (n_pad, q), n)."""
+    t = D.torch()
+    N = int(x_soa.shape[1])
+    n = 3 * N
+    q = int(rhs.shape[1])
+    if N == 0 or int(rhs.shape[0]) != n or not 1 <= q <= 8:
+        raise ValueError(f"{N} points do not match right-hand sides of shape {tuple(rhs.shape)}")
+    n_pad = padded_order(n, multiple)
+    dev = x_soa.device
+    L = lib()
+    with t.cuda.stream(stream) if stream is not None else nullcontext():
+        K64 = t.empty((n_pad, n_pad), dtype=t.float64, device=dev)
+        K32 = t.empty((n_pad, n_pad), dtype=t.float32, device=dev) if fp32 else None
+        check(L.fl_assemble(kernel_struct(kernel), N, N, D.ptr(x_soa), D.ptr(x_soa), D.ptr(eps_t), eps_stride, 1.0,
+                            0.0, D.ptr(K64), n_pad, D.ptr(K32), n_pad, None, D.stream_handle(stream)), "fl_assemble")
+        n_dev = t.full((1,), float(n), dtype=t.float64, device=dev)
+        src = rhs.contiguous()
+        b = t.empty((n_pad, q), dtype=t.float64, device=dev)
+        check(L.fl_pad_solve(D.ptr(K64), D.ptr(K32), n_pad, D.ptr(n_dev), D.ptr(src), D.ptr(b), q,
+                             D.stream_handle(stream)), "fl_pad_solve")
+    return K64, K32, b, n
+
+
+def assemble_padded(kernel, points, eps, rhs, fp32: bool = True, stream=None, multiple: int = PAD_MULTIPLE):
+    """``assemble_padded_device`` from host points (N, 3), blob widths and
+    right-hand sides (3N, q)."""
+    t = D.torch()
+    dev = D.device()
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    with t.cuda.stream(stream) if stream is not None else nullcontext():
+        x_soa = D.soa(points, dev)
+        eps_t, stride = D.eps_arg(eps, len(points), dev)
+        rhs_t = t.from_numpy(np.ascontiguousarray(rhs, dtype=np.float64)).to(dev)
+    return assemble_padded_device(kernel, x_soa, eps_t, stride, rhs_t, fp32=fp32, stream=stream, multiple=multiple)
 
 
 def assemble(kernel, targets, sources, eps, mu=1.0) -> np.ndarray:

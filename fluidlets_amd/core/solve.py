@@ -16,7 +16,7 @@ import numpy as np
 from .. import _device as D
 from .. import linalg
 from ._common import blob_widths, field_dtype, points as as_points
-from .assemble import assemble_device
+from .assemble import assemble_padded_device
 from .matvec import evaluate_device
 
 #: above this many unknowns, method="auto" switches from dense LU to GMRES
@@ -51,26 +51,29 @@ def _from_columns(x: np.ndarray, dtype, mu):
 
 def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-10, max_iter=30, stream=None):
     """Dense solve of K x = rhs (K assembled for mu = 1) on the device.
-    rhs: (3N, q) float64 device tensor.  Returns (x, refinement_steps, factor)."""
+    rhs: (3N, q) float64 device tensor.  The system is factored at its order
+    rounded up to a multiple of 128, identity-padded (assemble_padded_device:
+    the same solution, a faster factorization).  Returns (x (3N, q),
+    refinement_steps, factor of the padded system)."""
+    n = int(rhs.shape[0])
     if precision == "fp64":
         def assemble64():
-            return assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=False, stream=stream)[0]
+            return assemble_padded_device(kernel, x_soa, eps_t, stride, rhs, fp32=False, stream=stream)
 
-        K64 = assemble64()
-        F = linalg.getrf(K64, stream=stream, refill=lambda A: A.copy_(assemble64()))
+        K64, _, b, _ = assemble64()
+        F = linalg.getrf(K64, stream=stream, refill=lambda A: A.copy_(assemble64()[0]))
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
-        x = rhs.clone()
-        linalg.getrs(F, x, stream=stream)
-        return x, 0, F
+        linalg.getrs(F, b, stream=stream)
+        return b[:n], 0, F
     if precision == "mixed":
-        K64, K32 = assemble_device(kernel, x_soa, x_soa, eps_t, stride, 1.0, fp64=True, fp32=True, stream=stream)
+        K64, K32, b, _ = assemble_padded_device(kernel, x_soa, eps_t, stride, rhs, fp32=True, stream=stream)
         F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3")
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
-        x, steps, _ = linalg.refine(K64, F, rhs, tol=tol, max_iter=max_iter, stream=stream)
+        x, steps, _ = linalg.refine(K64, F, b, tol=tol, max_iter=max_iter, stream=stream)
         F.extra["A64"] = K64
-        return x, steps, F
+        return x[:n], steps, F
     raise ValueError(f"unknown precision {precision!r}")
 
 

@@ -48,45 +48,6 @@ GRAPH_MAX_POINTS = 640     # n_pad <= 1920
 BUCKET_POINTS = 64
 GRAPH_STEPS = 2            # corrections inside the graph (what 1e-10 takes on resistance matrices of these sizes)
 NRHS = 6                   # the six rigid-body modes
-PAD_MULTIPLE = 128         # larger served solves: order rounded up to this
-
-
-def padded_order(n: int, multiple: int = PAD_MULTIPLE) -> int:
-    return -(-int(n) // multiple) * multiple
-
-
-def assemble_padded(kernel, points, eps, rhs, fp32: bool = True, stream=None, multiple: int = PAD_MULTIPLE):
-    """The resistance matrix of ``kernel`` on the (N, 3) ``points`` (mu = 1)
-    in the leading block of identity-padded buffers of order n_pad = 3N
-    rounded up to ``multiple``, and the right-hand sides (3N, q) host over
-    zeros.  The factorization of a ragged order is up to a quarter slower
-    than at the next multiple of 128 (its GEMM tiles and rows are
-    misaligned); the padding leaves the solution unchanged.  Returns (K64,
-    K32 or None, b (n_pad, q), n)."""
-    t = D.torch()
-    dev = D.device()
-    points = np.ascontiguousarray(points, dtype=np.float64)
-    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
-    N = len(points)
-    n = 3 * N
-    q = int(rhs.shape[1])
-    if N == 0 or rhs.shape[0] != n or not 1 <= q <= 8:
-        raise ValueError(f"{N} points do not match right-hand sides of shape {rhs.shape}")
-    n_pad = padded_order(n, multiple)
-    st = stream if stream is not None else t.cuda.current_stream()
-    L = lib()
-    with t.cuda.stream(st):
-        x_soa = D.soa(points, dev)
-        eps_t, stride = D.eps_arg(eps, N, dev)
-        K64 = t.empty((n_pad, n_pad), dtype=t.float64, device=dev)
-        K32 = t.empty((n_pad, n_pad), dtype=t.float32, device=dev) if fp32 else None
-        check(L.fl_assemble(kernel_struct(kernel), N, N, D.ptr(x_soa), D.ptr(x_soa), D.ptr(eps_t), stride, 1.0, 0.0,
-                            D.ptr(K64), n_pad, D.ptr(K32), n_pad, None, D.stream_handle(st)), "fl_assemble")
-        stage = t.from_numpy(np.concatenate([[float(n)], rhs.ravel()])).to(dev)   # n, then the right-hand sides
-        b = t.empty((n_pad, q), dtype=t.float64, device=dev)
-        check(L.fl_pad_solve(D.ptr(K64), D.ptr(K32), n_pad, D.ptr(stage), D.ptr(stage) + 8, D.ptr(b), q,
-                             D.stream_handle(st)), "pad_solve")
-    return K64, K32, b, n
 
 
 def bucket_rows(n_points: int) -> int:

@@ -69,7 +69,7 @@ def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) ->
     without the server's shared cache, its own cache of body-frame
     factors."""
     if req.kind == "resistance":
-        from .graphs import padded_order
+        from .core.assemble import padded_order
         n = padded_order(3 * len(req.payload["points"]))
     else:
         n = 3 * req.payload["n_points"]
@@ -130,8 +130,8 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
             else:
                 x, _ = G.solve_resident(item[0], item[1], modes, steps=graphs.steps, stream=stream)
             return _grand(X, center, x, payload.get("mu", 1.0))
-    # larger solves: identity-padded to a multiple of 128 rows (graphs.assemble_padded)
-    from .graphs import assemble_padded
+    # larger solves: identity-padded to a multiple of 128 rows
+    from .core.assemble import assemble_padded
     key = geometry_key(kernel, X, eps) + precision if cache is not None else None
     item = cache.get(key) if cache is not None else None
     modes = rigid_modes(X, center)

@@ -14,6 +14,7 @@ import fluidlets_amd as fl
 from fluidlets_amd import graphs as G
 from fluidlets_amd import linalg
 from fluidlets_amd._lib import lib
+from fluidlets_amd.core.assemble import assemble_padded, padded_order
 from fluidlets_amd.core.solve import rigid_modes
 from fluidlets_amd.geometry import ellipsoid, ellipsoid_area
 
@@ -152,8 +153,8 @@ def test_assemble_padded_is_the_matrix_over_the_identity(N, multiple):
     sides over zeros; the FP32 copy its rounding."""
     kernel, X, c, eps = body(N, 40 + N, N % 2 == 1)
     modes = rigid_modes(X, c)
-    K64, K32, b, n = G.assemble_padded(kernel, X, eps, modes, multiple=multiple)
-    n_pad = G.padded_order(3 * N, multiple)
+    K64, K32, b, n = assemble_padded(kernel, X, eps, modes, multiple=multiple)
+    n_pad = padded_order(3 * N, multiple)
     assert n == 3 * N and tuple(K64.shape) == (n_pad, n_pad) and n_pad % multiple == 0
     P = np.eye(n_pad)
     P[:n, :n] = O.assemble(kernel, X, X, eps)
