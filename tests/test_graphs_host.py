@@ -9,6 +9,13 @@ import scipy.linalg as sl
 from fluidlets_amd import graphs as G
 
 
+def test_padded_order():
+    from fluidlets_amd.core.assemble import PAD_MULTIPLE, padded_order
+    assert PAD_MULTIPLE == 128
+    assert [padded_order(n) for n in (1, 128, 129, 1926, 5403, 24576)] == [128, 128, 256, 2048, 5504, 24576]
+    assert padded_order(300, 192) == 384
+
+
 def test_bucket_rows():
     assert [G.bucket_rows(n) for n in (1, 64, 65, 128, 300, 640)] == [192, 192, 384, 384, 960, 1920]
     assert G.bucket_rows(G.GRAPH_MAX_POINTS) == 3 * G.GRAPH_MAX_POINTS
