@@ -92,6 +92,23 @@ def test_padded_graph_solve_matches_the_dense_fp64_solve(N, wall):
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) <= 2e-10 * cond
 
 
+def test_graph_solve_with_per_point_blob_widths():
+    """Blob widths given per point (staged and read with stride 1): the
+    padded graph solve against the oracle's dense FP64 solve."""
+    import torch
+    print("case: graph solve with per-point eps", flush=True)
+    kernel, X, c, eps = body(333, 77, True)
+    eps_i = eps * np.random.default_rng(5).uniform(0.8, 1.2, len(X))
+    K = O.assemble(kernel, X, X, eps_i)
+    modes = rigid_modes(X, c)
+    g = G.SolveGraph(G.bucket_rows(333), torch.cuda.Stream())
+    x, res, _ = g.solve(kernel, X, eps_i, modes)
+    rel = np.linalg.norm(K @ x - modes, axis=0) / np.linalg.norm(modes, axis=0)
+    assert np.all(rel <= 1.5e-10) and np.all(res <= 1e-10)
+    x_scalar, _, _ = g.solve(kernel, X, eps, modes)      # the same graph, one width again
+    assert not np.allclose(x, x_scalar, rtol=1e-6)
+
+
 def test_resident_padded_factors_reproduce_the_graph():
     import torch
     print("case: resident padded factors", flush=True)
@@ -146,12 +163,14 @@ def test_pool_rejects_large_solves_and_shares_graphs_per_stream():
     assert pool.nbytes() == 2 * G.graph_bytes(G.bucket_rows(200)) + G.graph_bytes(G.bucket_rows(150))
 
 
-@pytest.mark.parametrize("N,multiple", [(331, 128), (700, 128), (64, 192)])
-def test_assemble_padded_is_the_matrix_over_the_identity(N, multiple):
+@pytest.mark.parametrize("N,multiple,per_point", [(331, 128, False), (700, 128, True), (64, 192, False)])
+def test_assemble_padded_is_the_matrix_over_the_identity(N, multiple, per_point):
     """fl_assemble into the leading block and fl_pad_solve around it: the
     oracle's matrix, the identity below and to the right, the right-hand
     sides over zeros; the FP32 copy its rounding."""
     kernel, X, c, eps = body(N, 40 + N, N % 2 == 1)
+    if per_point:
+        eps = eps * np.random.default_rng(N).uniform(0.8, 1.2, N)
     modes = rigid_modes(X, c)
     K64, K32, b, n = assemble_padded(kernel, X, eps, modes, multiple=multiple)
     n_pad = padded_order(3 * N, multiple)
