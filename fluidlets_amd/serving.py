@@ -172,7 +172,10 @@ class SolveServer:
     own stream, admitted by HBM footprint against `budget_bytes`.  With
     ``graphs`` (mixed precision) small resistance solves replay captured
     graphs, whose buffers (at most graphs.pool_bytes(slots)) are set aside
-    from the budget."""
+    from the budget.  The graphs are captured here, in the constructing
+    thread: build the server while no other thread of the process is
+    using the GPU (a capture that overlaps other threads' runtime calls
+    can end the process)."""
 
     def __init__(self, slots: int = 4, precision: str = "mixed", budget_bytes: int | None = None,
                  cache_bytes: int = 0, graphs: bool = True):
