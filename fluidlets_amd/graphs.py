@@ -2,8 +2,9 @@
 
 A served resistance solve of at most GRAPH_MAX_POINTS points runs on a
 padded system.  Its n x n matrix K (n = 3N) is the leading block of an
-n_pad x n_pad buffer, n_pad = 3 x (N rounded up to BUCKET_POINTS points),
-padded with the identity: [[K, 0], [0, I]].  Partial pivoting never takes a
+n_pad x n_pad buffer, n_pad = n rounded up to a multiple of BUCKET_ROWS (the
+factorization runs fastest at multiples of 128), padded with the identity:
+[[K, 0], [0, I]].  Partial pivoting never takes a
 padding row in the first n columns (they are zero there) and the padding's
 L21 and U12 blocks stay zero, so the factors are K's next to I.  The padded
 right-hand sides are zero below row n, and so are the residuals, the
@@ -45,14 +46,14 @@ from . import linalg
 from ._lib import FL_FP32C, FL_LU_FP32, check, kernel_struct, lib
 
 GRAPH_MAX_POINTS = 640     # n_pad <= 1920
-BUCKET_POINTS = 64
+BUCKET_ROWS = 128
 GRAPH_STEPS = 2            # corrections inside the graph (what 1e-10 takes on resistance matrices of these sizes)
 NRHS = 6                   # the six rigid-body modes
 
 
 def bucket_rows(n_points: int) -> int:
     """Padded order of an N-point solve."""
-    return 3 * (-(-int(n_points) // BUCKET_POINTS) * BUCKET_POINTS)
+    return -(-3 * int(n_points) // BUCKET_ROWS) * BUCKET_ROWS
 
 
 def graph_bytes(n_pad: int, nrhs: int = NRHS) -> int:
@@ -65,8 +66,7 @@ def graph_bytes(n_pad: int, nrhs: int = NRHS) -> int:
 def pool_bytes(streams: int) -> int:
     """Largest footprint of a pool serving `streams` streams: one graph per
     bucket and stream."""
-    return streams * sum(graph_bytes(3 * BUCKET_POINTS * k)
-                         for k in range(1, GRAPH_MAX_POINTS // BUCKET_POINTS + 1))
+    return streams * sum(graph_bytes(BUCKET_ROWS * k) for k in range(1, bucket_rows(GRAPH_MAX_POINTS) // BUCKET_ROWS + 1))
 
 
 def _finish(K64, F, b, x, res, x_host, n, tol, stream):
@@ -95,8 +95,8 @@ class SolveGraph:
         t = D.torch()
         dev = D.device()
         L = lib()
-        if n_pad <= 0 or n_pad % 3 or n_pad > 3 * GRAPH_MAX_POINTS:
-            raise ValueError(f"padded order {n_pad} outside (0, {3 * GRAPH_MAX_POINTS}] or not 3 x points")
+        if not 0 < n_pad <= bucket_rows(GRAPH_MAX_POINTS):
+            raise ValueError(f"padded order {n_pad} outside (0, {bucket_rows(GRAPH_MAX_POINTS)}]")
         if not 1 <= nrhs <= 8 or steps < 1:
             raise ValueError("need 1 <= nrhs <= 8 and at least one correction")
         self.n_pad, self.steps, self.nrhs, self.stream = n_pad, int(steps), int(nrhs), stream
@@ -249,8 +249,8 @@ class GraphPool:
         self.captures = 0
         self.lazy = streams is None
         for s in streams or ():
-            for k in range(1, GRAPH_MAX_POINTS // BUCKET_POINTS + 1):
-                rows = 3 * BUCKET_POINTS * k
+            for k in range(1, bucket_rows(GRAPH_MAX_POINTS) // BUCKET_ROWS + 1):
+                rows = BUCKET_ROWS * k
                 self._graphs[(rows, D.stream_handle(s))] = SolveGraph(rows, s, self.steps)
                 self.captures += 1
         if streams is not None:

@@ -54,12 +54,12 @@ def test_graph_replays_equal_direct_launches():
     kernels), each replayed and launched directly with other device work in
     between, then replayed again: all equal bit for bit."""
     import torch
-    print("case: graph replays vs direct launches, bucket 320 points", flush=True)
+    print("case: graph replays vs direct launches, bucket of 1024 rows", flush=True)
     s = torch.cuda.Stream()
     g = G.SolveGraph(G.bucket_rows(300), s)
-    assert g.n_pad == 960 and g.graph is not None
+    assert g.n_pad == 1024 and g.graph is not None
     junk = torch.randn(1500, 1500, device="cuda", dtype=torch.float64)
-    for N, seed, wall in [(300, 1, True), (257, 2, False), (320, 3, True)]:
+    for N, seed, wall in [(300, 1, True), (331, 2, False), (341, 3, True)]:
         kernel, X, c, eps = body(N, seed, wall)
         modes = rigid_modes(X, c)
         x1, r1, _ = g.solve(kernel, X, eps, modes)
@@ -142,7 +142,7 @@ def test_pool_captures_every_bucket_up_front_for_given_streams():
     import torch
     s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
     pool = G.GraphPool(streams=[s1, s2])
-    nb = G.GRAPH_MAX_POINTS // G.BUCKET_POINTS
+    nb = G.bucket_rows(G.GRAPH_MAX_POINTS) // G.BUCKET_ROWS
     assert pool.captures == 2 * nb and not pool.lazy
     assert pool.nbytes() <= G.pool_bytes(2)
     g = pool.get(300, s2)
@@ -158,7 +158,7 @@ def test_pool_rejects_large_solves_and_shares_graphs_per_stream():
     with pytest.raises(ValueError):
         pool.get(G.GRAPH_MAX_POINTS + 1, s1)
     a = pool.get(200, s1)
-    assert pool.get(250, s1) is a and pool.get(150, s1) is not a
+    assert pool.get(210, s1) is a and pool.get(150, s1) is not a
     assert pool.get(200, s2) is not a and pool.captures == 3
     assert pool.nbytes() == 2 * G.graph_bytes(G.bucket_rows(200)) + G.graph_bytes(G.bucket_rows(150))
 

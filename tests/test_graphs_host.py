@@ -17,7 +17,7 @@ def test_padded_order():
 
 
 def test_bucket_rows():
-    assert [G.bucket_rows(n) for n in (1, 64, 65, 128, 300, 640)] == [192, 192, 384, 384, 960, 1920]
+    assert [G.bucket_rows(n) for n in (1, 64, 65, 128, 300, 640)] == [128, 256, 256, 384, 1024, 1920]
     assert G.bucket_rows(G.GRAPH_MAX_POINTS) == 3 * G.GRAPH_MAX_POINTS
 
 
