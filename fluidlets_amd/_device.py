@@ -56,16 +56,79 @@ def eps_arg(eps, n: int, dev=None):
     return t.from_numpy(e).to(dev or device()), 1
 
 
+_hip = None
+_stream_lock = threading.Lock()
+_free_streams: dict = {}       # device -> [ExternalStream] owned here, not leased
+_owned_streams: dict = {}      # handle -> ExternalStream (every stream created here)
+
+
+def _hip_runtime():
+    """The HIP runtime that torch has loaded (opened by its soname, so
+    dlopen returns the copy already in the process)."""
+    global _hip
+    if _hip is None:
+        import ctypes
+        torch()
+        _hip = ctypes.CDLL("libamdhip64.so.7")
+        _hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        _hip.hipStreamCreateWithFlags.restype = ctypes.c_int
+    return _hip
+
+
+def lease_stream():
+    """A HIP stream for the exclusive use of the caller until release_stream.
+
+    The streams are created here (hipStreamCreateWithFlags, non-blocking,
+    default priority) and reused after release; torch's own stream pool is
+    not used for work that holds state keyed by its stream, because that pool
+    hands out the same 32 streams per device in turn, so two live users could
+    end up on one stream."""
+    import ctypes
+    t = torch()
+    dev = t.cuda.current_device()
+    with _stream_lock:
+        free = _free_streams.setdefault(dev, [])
+        if free:
+            return free.pop()
+    h = ctypes.c_void_p()
+    rc = _hip_runtime().hipStreamCreateWithFlags(ctypes.byref(h), 1)   # hipStreamNonBlocking
+    if rc != 0:
+        raise RuntimeError(f"hipStreamCreateWithFlags failed with HIP error {rc}")
+    s = t.cuda.ExternalStream(int(h.value), device=t.device("cuda", dev))
+    with _stream_lock:
+        _owned_streams[int(h.value)] = s
+    return s
+
+
+def release_stream(s) -> None:
+    """Return a stream from lease_stream; work enqueued on it may still run."""
+    with _stream_lock:
+        if int(s.cuda_stream) not in _owned_streams:
+            raise ValueError("release_stream: not a stream from lease_stream")
+        _free_streams.setdefault(s.device.index, []).append(s)
+
+
 def workspace(nbytes: int, stream=None):
-    """A device byte buffer of at least nbytes, cached per (device, stream).
-    Work that shares a stream is ordered, so reuse on one stream is safe."""
+    """A device byte buffer of at least nbytes, cached per (device, stream,
+    host thread).  The work of one thread on one stream is ordered, so reuse
+    there is safe; two threads that share a stream interleave their launches,
+    so each gets its own buffer."""
     if nbytes <= 0:
         return None
     t = torch()
-    key = (t.cuda.current_device(), stream_handle(stream))
+    key = (t.cuda.current_device(), stream_handle(stream), threading.get_ident())
+    s = stream if stream is not None else t.cuda.current_stream()
     with _ws_lock:
         buf = _workspaces.get(key)
         if buf is None or buf.numel() < nbytes:
-            buf = t.empty(int(nbytes), dtype=t.uint8, device=device())
+            if len(_workspaces) >= 64:
+                # drop the buffers of threads that have ended; each was
+                # allocated on its own stream, so the allocator reuses it only
+                # after the work already queued there
+                alive = {th.ident for th in threading.enumerate()}
+                for k in [k for k in _workspaces if k[2] not in alive]:
+                    del _workspaces[k]
+            with t.cuda.stream(s):
+                buf = t.empty(int(nbytes), dtype=t.uint8, device=device())
             _workspaces[key] = buf
         return buf

@@ -9,6 +9,8 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
+import numpy as np
+
 LIB_PATH = Path(__file__).resolve().parent / "libfluidlets_hip.so"
 
 FL_OK, FL_EINVAL, FL_EHIP, FL_ENOCONV, FL_ESINGULAR, FL_EWORKSPACE = 0, -1, -2, -3, -4, -5
@@ -96,6 +98,10 @@ def lib():
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH.name} is not built; run `python -m fluidlets_amd.build` "
                               "(fluidlets_amd has no CPU fallback)")
+        # torch first: the library then binds to the HIP runtime torch has
+        # loaded instead of bringing a second copy into the process (two
+        # runtimes cannot share the GPU)
+        import torch  # noqa: F401
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(L, name)
@@ -126,4 +132,29 @@ def check(status: int, what: str) -> None:
 
 
 def kernel_struct(kernel) -> fl_kernel:
-    return fl_kernel(int(kernel.kind), 0, float(getattr(kernel, "alpha", 0.0)))
+    """The device descriptor of a kernel object.
+
+    In the reference a kernel is its Numba ``block`` function and the core
+    loops call whatever ``block`` a subclass supplies
+    (src/fluidlets/kernels/base.py:1-7, 14-19).  Here the pair math of each
+    kernel is compiled into the HIP library, so only the three kernels it
+    implements are accepted: Stokeslet3D, WallStokeslet3D and
+    Brinkmanlet3D(alpha), or subclasses of them that keep their ``block``
+    and ``kind``.  Any other kernel (a new ``Kernel`` subclass, or a
+    subclass that supplies its own ``block``) raises NotImplementedError
+    instead of silently computing a built-in kernel."""
+    from .kernels import Brinkmanlet3D, Stokeslet3D, WallStokeslet3D
+
+    for base in (WallStokeslet3D, Brinkmanlet3D, Stokeslet3D):
+        if isinstance(kernel, base):
+            cls = type(kernel)
+            if cls.block is base.block and cls.kind == base.kind:
+                alpha = float(getattr(kernel, "alpha", 0.0)) if base is Brinkmanlet3D else 0.0
+                if not (alpha >= 0.0 and np.isfinite(alpha)):
+                    raise ValueError("alpha must be real and non-negative")
+                return fl_kernel(int(base.kind), 0, alpha)
+            break
+    raise NotImplementedError(
+        f"{type(kernel).__name__} has no device implementation: fluidlets_amd runs the kernels compiled into "
+        "libfluidlets_hip.so (Stokeslet3D, WallStokeslet3D, Brinkmanlet3D(alpha)); a kernel defined by its own "
+        "`block` function cannot be dispatched to the GPU")

@@ -100,6 +100,7 @@ def assemble_padded(kernel, points, eps, rhs, fp32: bool = True, stream=None, mu
 def assemble(kernel, targets, sources, eps, mu=1.0) -> np.ndarray:
     """Dense (p M) x (q N) matrix of ``kernel`` from N sources to M targets
     (reference signature and result; a complex mu gives a complex matrix)."""
+    kernel_struct(kernel)   # only the kernels the device implements (NotImplementedError otherwise)
     targets = points(targets, kernel.dim)
     sources = points(sources, kernel.dim)
     M, N = targets.shape[0], sources.shape[0]
@@ -147,6 +148,7 @@ def assemble_nearest(kernel, targets, quad_points, nearest, n_force, eps, mu=1.0
     """Kernel matrix with forces on a coarse set and quadrature on a fine set
     (Gallagher & Smith nearest-neighbour discretization):
     K[i, j] = sum over k with nearest[k] = j of G(x_i, X_k) / mu."""
+    kernel_struct(kernel)   # only the kernels the device implements (NotImplementedError otherwise)
     targets = points(targets, kernel.dim)
     quad = points(quad_points, kernel.dim)
     Q = quad.shape[0]

@@ -59,6 +59,7 @@ def evaluate(kernel, targets, sources, strengths, eps, mu=1.0, precision="fp64")
 
     Returns (M, p) array, complex if mu or strengths are complex.
     """
+    kernel_struct(kernel)   # only the kernels the device implements (NotImplementedError otherwise)
     targets = points(targets, kernel.dim)
     sources = points(sources, kernel.dim)
     N = sources.shape[0]

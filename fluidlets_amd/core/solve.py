@@ -15,6 +15,7 @@ import numpy as np
 
 from .. import _device as D
 from .. import linalg
+from .._lib import kernel_struct
 from ._common import blob_widths, field_dtype, points as as_points
 from .assemble import assemble_padded_device
 from .matvec import evaluate_device
@@ -93,6 +94,7 @@ def resistance(kernel, points, velocities, eps, mu=1.0, method="auto", rtol=1e-1
     forces : (N, dim) array
     info : SolveInfo
     """
+    kernel_struct(kernel)   # only the kernels the device implements (NotImplementedError otherwise)
     X = as_points(points, kernel.dim)
     n = X.shape[0]
     eps = blob_widths(eps, n)
@@ -156,6 +158,7 @@ def grand_resistance(kernel, points, center, eps, mu=1.0, precision="fp64", tol=
     column m is [sum f_m ; sum (x - c) x f_m] (the pattern of
     examples/translating_sphere/run.py:45-48 with all six modes at once).
     Returns (R, forces (6, N, 3), info dict)."""
+    kernel_struct(kernel)
     X = as_points(points, 3)
     n = X.shape[0]
     eps = blob_widths(eps, n)

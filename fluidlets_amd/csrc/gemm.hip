@@ -8,10 +8,12 @@
 // operands), with the next tile prefetched into registers while the current
 // one is multiplied.
 //
-// The accumulators start from (beta / alpha) C, loaded in the MFMA output
-// layout before the K loop, so the epilogue is stores only:
-// C = alpha * (A B + (beta / alpha) C).  (For the LU's alpha = -1, beta = 1
-// the scaling is exact.)  alpha == 0 is handled as C = beta C.
+// For alpha = +-1 (the LU's updates) the accumulators start from
+// (beta / alpha) C = alpha beta C, loaded in the MFMA output layout before the
+// K loop, so the epilogue is stores only: C = alpha (A B + alpha beta C), with
+// both scalings exact.  For any other alpha the accumulators start from zero
+// and the epilogue forms alpha (A B) + beta C, each term rounded once as in a
+// reference GEMM.  alpha == 0 is handled as C = beta C.
 //
 // A workgroup reads all of K for its tile before it writes C, so the call is
 // safe in place when the C tile is the A tile (C = A B with N <= 128 and
@@ -99,7 +101,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
 
     typename P::acc_t acc[P::NB][P::NB];
     const bool use_c = g.beta != T(0) && g.alpha != T(0);
-    const T cscale = use_c ? g.beta / g.alpha : T(0);
+    const bool pre_c = use_c && (g.alpha == T(1) || g.alpha == T(-1));   // (beta / alpha) = alpha beta exactly
+    const bool post_c = use_c && !pre_c;
+    const T cscale = pre_c ? g.alpha * g.beta : T(0);
     // interior tiles take unpredicated accesses from one lane base pointer
     const bool full_c = m0 + TM <= g.M && n0 + TN <= g.N;
     T* cbase = g.C + (m0 + wm * 64 + P::row_lane(lane)) * g.ldc + n0 + wn * 64 + P::d_col(lane);
@@ -108,7 +112,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
     for (int i = 0; i < P::NB; ++i)
 #pragma unroll
         for (int j = 0; j < P::NB; ++j) acc[i][j] = P::zero();
-    if (use_c) {
+    if (pre_c) {
         if (full_c) {
 #pragma unroll
             for (int i = 0; i < P::NB; ++i)
@@ -214,15 +218,18 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
 #pragma unroll
             for (int j = 0; j < P::NB; ++j)
 #pragma unroll
-                for (int v = 0; v < P::NACC; ++v) cbase[coff(i, j, v)] = outscale * acc[i][j][v];
+                for (int v = 0; v < P::NACC; ++v) {
+                    const int64_t o = coff(i, j, v);
+                    acc[i][j][v] = post_c ? outscale * acc[i][j][v] + g.beta * cbase[o] : outscale * acc[i][j][v];
+                    cbase[o] = acc[i][j][v];
+                }
         if (g.absmax_bits) {
 #pragma unroll
             for (int i = 0; i < P::NB; ++i)
 #pragma unroll
                 for (int j = 0; j < P::NB; ++j)
 #pragma unroll
-                    for (int v = 0; v < P::NACC; ++v)
-                        local_max = fmaxf(local_max, (float)fabs((double)(outscale * acc[i][j][v])));
+                    for (int v = 0; v < P::NACC; ++v) local_max = fmaxf(local_max, (float)fabs((double)acc[i][j][v]));
         }
     } else {
 #pragma unroll
@@ -235,6 +242,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
                     int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
                     if (r < g.M && c < g.N) {
                         T val = outscale * acc[i][j][v];
+                        if (post_c) val += g.beta * g.C[r * g.ldc + c];
                         g.C[r * g.ldc + c] = val;
                         local_max = fmaxf(local_max, (float)fabs((double)val));
                     }

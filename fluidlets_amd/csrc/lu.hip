@@ -1322,9 +1322,21 @@ __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int
 // 15-20 % faster than gemm_kernel on the K = 256 trailing shapes), the rest,
 // and every product with a fused epilogue, to gemm_kernel.  FL_LU_ROCBLAS=0
 // keeps everything on gemm_kernel.
+// The handles of one host thread, destroyed when the thread ends (so a
+// process that runs many short-lived threads does not accumulate them).
+struct ThreadHandles {
+    std::vector<rocblas_handle> per_dev;
+    ~ThreadHandles()
+    {
+        for (rocblas_handle h : per_dev)
+            if (h) rocblas_destroy_handle(h);
+    }
+};
+
 int rocblas_handle_for_device(rocblas_handle& out)
 {
-    static thread_local std::vector<rocblas_handle> per_dev;
+    static thread_local ThreadHandles handles;
+    std::vector<rocblas_handle>& per_dev = handles.per_dev;
     int dev = 0;
     FL_HIP(hipGetDevice(&dev));
     if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);

@@ -799,10 +799,20 @@ int pick_split(int64_t M, int64_t N, int tpl)
     return (int)std::max<int64_t>(1, s);
 }
 
+__global__ __launch_bounds__(WG) void zero_kernel(double* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * WG + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
 // Symmetric plan: T tiles of SB points, c tiles per workgroup along a row,
 // Q = ceil(T / c) chunks (grid.y).  Used only when there are enough tile pairs
-// to give every CU several workgroups (T (T + 1) / 2 >= 4 CUs) and not
-// switched off by FL_EVAL_SYMMETRIC=0 (measurement switch).
+// to give every CU several workgroups (T (T + 1) / 2 >= 4 CUs), when its
+// workspace (one column partial per tile pair, about 0.023 N^2 bytes) stays
+// within SYM_MAX_BYTES (N up to about 428,000; larger sums take the general
+// kernel, whose workspace is O(N)), and not switched off by
+// FL_EVAL_SYMMETRIC=0 (measurement switch).
+constexpr size_t SYM_MAX_BYTES = size_t(4) << 30;
 struct SymPlan {
     int T = 0, c = 0, Q = 0;
     size_t bytes = 0;
@@ -824,7 +834,7 @@ SymPlan sym_plan(int64_t N)
     const size_t rowb = align_up((size_t)T * p.Q * 3 * SB * sizeof(double), 256);
     const size_t colb = (size_t)(T * (T - 1) / 2) * 3 * SB * sizeof(double);
     p.bytes = rowb + colb;
-    p.ok = true;
+    p.ok = p.bytes <= SYM_MAX_BYTES;
     return p;
 }
 
@@ -882,8 +892,13 @@ int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt,
     FL_CHECK(!(nf == 2 || w_im != 0.0) || u_im != nullptr, "complex result needs u_im");
     if (M == 0) return FL_OK;
     if (N == 0) {
-        FL_HIP(hipMemsetAsync(u_re, 0, 3 * M * sizeof(double), stream));
-        if (u_im) FL_HIP(hipMemsetAsync(u_im, 0, 3 * M * sizeof(double), stream));
+        // a fill kernel rather than hipMemsetAsync, so the call can be captured in a graph
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((3 * M + WG - 1) / WG)), dim3(WG), 0, stream, u_re, 3 * M);
+        FL_LAUNCHED();
+        if (u_im) {
+            hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((3 * M + WG - 1) / WG)), dim3(WG), 0, stream, u_im, 3 * M);
+            FL_LAUNCHED();
+        }
         return FL_OK;
     }
     FL_CHECK(tgt && src && eps && f && u_re, "null pointer argument");

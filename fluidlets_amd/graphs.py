@@ -10,10 +10,12 @@ L21 and U12 blocks stay zero, so the factors are K's next to I.  The padded
 right-hand sides are zero below row n, and so are the residuals, the
 corrections and the solution, whose first n rows solve K x = b.
 
-One graph per (bucket, stream) is captured on first use (torch.cuda.CUDAGraph
-around the library's launches, in thread-local capture mode so that other
-streams keep working) and replayed for every later solve of that bucket on
-that stream.  It holds the asynchronous FP32 LU without interchanges
+One graph per (bucket, stream) is captured (torch.cuda.CUDAGraph around the
+library's launches, in thread-local capture mode so that other streams keep
+working) and replayed for every later solve of that bucket on that stream.
+A server captures every bucket for every slot stream when it is built, in
+the constructing thread; a pool without streams captures on first use, and
+only in the thread that created it (GraphPool).  It holds the asynchronous FP32 LU without interchanges
 (fl_sgetrf_async_ex) and GRAPH_STEPS FP64 refinement corrections
 (fl_refine_steps): some 40 to 90 kernel launches become one graph launch,
 with one synchronization after it instead of one per correction.  The
@@ -248,6 +250,7 @@ class GraphPool:
         self._lock = threading.Lock()
         self.captures = 0
         self.lazy = streams is None
+        self._owner = threading.get_ident()
         for s in streams or ():
             for k in range(1, bucket_rows(GRAPH_MAX_POINTS) // BUCKET_ROWS + 1):
                 rows = BUCKET_ROWS * k
@@ -265,6 +268,11 @@ class GraphPool:
         if g is None:
             if not self.lazy:
                 raise KeyError(f"no graph captured for {key[0]} rows on this stream")
+            if threading.get_ident() != self._owner:
+                # a capture overlapping other threads' runtime calls can end the
+                # process, so a lazy pool captures only in its creating thread
+                raise RuntimeError("a GraphPool without streams captures only in the thread that created it; "
+                                   "build it with streams= to capture every graph up front for other threads")
             g = SolveGraph(key[0], stream, self.steps)
             with self._lock:
                 self._graphs[key] = g

@@ -2,8 +2,9 @@
 
 Requests (rigid-body resistance solves of any size, free-swimmer time
 stepping) are admitted as they arrive whenever a slot is free and their HBM
-footprint fits the budget; each slot is a worker thread driving its own HIP
-stream (default priority), so admitted requests overlap on the device.  The
+footprint fits the budget; each slot is a long-lived worker thread driving
+its own HIP stream (created for the server, default priority), so admitted
+requests overlap on the device.  The
 HIP library calls release the GIL, and every kernel is deterministic (fixed
 summation orders, no floating-point atomics), so a request's result does not
 depend on what else is in flight: tests check it bit for bit, alone and in a
@@ -26,6 +27,7 @@ from __future__ import annotations
 
 import hashlib
 import heapq
+import queue
 import threading
 import time
 from contextlib import nullcontext
@@ -188,12 +190,52 @@ class SolveServer:
         self.precision = precision
         self.budget = budget_bytes
         self.cache = FactorCache(cache_bytes) if cache_bytes > 0 else None
-        self.streams = [t.cuda.Stream(device=dev) for _ in range(slots)]
+        self.budget -= max(0, int(cache_bytes))            # resident factors live inside the budget
+        # one stream per slot, created for this server (never shared with
+        # another user of the process), and one long-lived worker thread per
+        # slot that takes its requests from a queue
+        self.streams = [D.lease_stream() for _ in range(slots)]
         self.graphs = None
         if graphs and precision == "mixed":
             from .graphs import GraphPool, pool_bytes
             self.graphs = GraphPool(streams=self.streams)   # every capture here, before any worker thread
             self.budget -= pool_bytes(slots)
+        self._jobs = [queue.SimpleQueue() for _ in range(slots)]
+        self._workers = [threading.Thread(target=self._worker, args=(sid,), daemon=True,
+                                          name=f"fl-slot-{sid}") for sid in range(slots)]
+        for w in self._workers:
+            w.start()
+
+    def _worker(self, sid: int):
+        dev = D.device()
+        t = D.torch()
+        t.cuda.set_device(dev)
+        while True:
+            job = self._jobs[sid].get()
+            if job is None:
+                return
+            job()
+
+    def close(self) -> None:
+        """Stop the slot workers (after the requests they hold) and hand the
+        slots' streams back."""
+        if not self._workers:
+            return
+        for q in self._jobs:
+            q.put(None)
+        for w in self._workers:
+            w.join()
+        self._workers = []
+        for s in self.streams:
+            s.synchronize()
+            D.release_stream(s)
+        self.streams = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def _execute(self, req: Request, stream):
         t = D.torch()
@@ -239,9 +281,10 @@ class SolveServer:
             if on_result is not None:
                 on_result(res)
 
+        if not self._workers:
+            raise RuntimeError("the server is closed")
         state["t0"] = time.perf_counter()
         waiting: list = []
-        threads = []
         i = 0
         with lock:
             while i < len(reqs) or waiting or state["running"]:
@@ -264,9 +307,7 @@ class SolveServer:
                     sid = free_streams.pop()
                     state["in_use"] += fp
                     state["running"] += 1
-                    th = threading.Thread(target=worker_body, args=(req, sid, fp), daemon=True)
-                    threads.append(th)
-                    th.start()
+                    self._jobs[sid].put(lambda req=req, sid=sid, fp=fp: worker_body(req, sid, fp))
                     admitted = True
                 if admitted:
                     continue
@@ -274,8 +315,6 @@ class SolveServer:
                 if realtime and i < len(reqs):
                     timeout = max(0.0, reqs[i].arrival - now())
                 lock.wait(timeout=timeout if timeout is not None else 1.0)
-        for th in threads:
-            th.join()
         return sorted(results, key=lambda r: r.rid)
 
 

@@ -288,7 +288,7 @@ def _prefetch_body_factors(shape_at, stage_times, kernel, nearest, e, mu, cache:
     in the cache) the helper stops, leaving the interpreter to the
     integrator."""
     t = D.torch()
-    stream = t.cuda.Stream(device=D.device())
+    stream = D.lease_stream()
     hint: dict = {}
     idle = 0
     try:
@@ -312,6 +312,7 @@ def _prefetch_body_factors(shape_at, stage_times, kernel, nearest, e, mu, cache:
         return
     finally:
         stream.synchronize()
+        D.release_stream(stream)
 
 
 def reuse_cache_bytes(n_points: int) -> int:

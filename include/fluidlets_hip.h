@@ -97,7 +97,8 @@ int fl_device_cus(void);
  * kernels are reciprocal (G(y, x) = G(x, y)^T) and each unordered pair's block
  * is evaluated once and applied both ways.  fl_eval takes that path when the
  * workspace holds at least fl_eval_sym_workspace_size(N, 0, FL_FP64) bytes
- * (0 means the path is not used at this size); the partial sums are again
+ * (0 means the path is not used at this size: too few tile pairs to fill
+ * the device, or a workspace above 4 GiB, about N > 428,000); the partial sums are again
  * combined in a fixed order.  This is the matvec of resistance problems
  * (core/solve.py:71-73 passes the points as both targets and sources).
  */
@@ -175,8 +176,13 @@ int fl_dgetrf(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* 
  * never waits on the stream.  Every diagonal block is factored without
  * interchanges and the largest multiplier of the whole matrix is checked on
  * the device.  When the stream has run the call, *status (one device word)
- * is 0 if no multiplier exceeded 1 + 2^-20: the pivots are then those partial
- * pivoting chooses, and A, ipiv and aux are exactly what fl_?getrf writes.
+ * is 0 if no multiplier exceeded 1 + 2^-20 in magnitude.  Every pivot is then
+ * within a factor 1 + 2^-20 of the largest entry left in its column: the
+ * factorization is partial pivoting's except at near-ties (an entry below
+ * the diagonal within 2^-20 relative of the pivot), where LAPACK could have
+ * interchanged rows and this path does not; growth stays bounded as with
+ * partial pivoting.  When no multiplier exceeds 1, A, ipiv and aux are
+ * exactly what fl_?getrf writes.
  * Otherwise *status is 1 and A holds unusable values: refill it from its
  * source and call fl_?getrf.  Same workspace as fl_?getrf.  Replaces, like
  * fl_?getrf, the LU inside scipy.linalg.solve (src/fluidlets/core/solve.py:66)
@@ -236,6 +242,8 @@ int fl_sgetrf_async_ex(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* au
 
 /* C = alpha A B + beta C, row-major (M x K, K x N, M x N), on MFMA with FP32
  * (fl_sgemm) or FP64 (fl_dgemm) accumulation; C is not read when beta == 0.
+ * alpha (A B) and beta C are each formed once and added, as in a reference
+ * GEMM (for alpha = +-1 the sum starts from alpha beta C, exact as well).
  * The LU's update kernel.  It has no counterpart call in the reference (there
  * it is the gemm inside LAPACK getrf); exported for tests and for callers
  * composing their own blocked algorithms. */

@@ -162,15 +162,6 @@ def nearest_neighbors(quad_points, force_points) -> np.ndarray:
     return cKDTree(np.asarray(force_points, float)).query(np.asarray(quad_points, float))[1]
 
 
-def block(kernel, x, x0, eps) -> np.ndarray:
-    kind, alpha = kind_of(kernel)
-    x = np.ascontiguousarray(x, dtype=float)
-    x0 = np.ascontiguousarray(x0, dtype=float)
-    G = np.empty(9)
-    lib().flo_block(kind, alpha, _p(x), _p(x0), float(eps), _p(G))
-    return G.reshape(3, 3)
-
-
 def image_block(x, x0, eps) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=float)
     x0 = np.ascontiguousarray(x0, dtype=float)

@@ -77,3 +77,40 @@ def test_compute_path_fails_loudly_without_gpu():
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError, match="GPU"):
         fl.evaluate(fl.Stokeslet3D(), np.ones((2, 3)), np.zeros((2, 3)), np.ones((2, 3)), 0.1)
+
+
+def test_kernel_plugin_contract_is_explicit():
+    """The reference dispatches any Kernel through its `block` function
+    (src/fluidlets/kernels/base.py:1-7); the device library implements three
+    kernels, so anything else must fail with a clear error rather than an
+    AttributeError or a silently substituted built-in kernel."""
+    import fluidlets_amd as fl
+    from fluidlets_amd import _lib
+    from fluidlets_amd.kernels import Kernel, VectorKernel
+
+    assert _lib.kernel_struct(fl.Stokeslet3D()).kind == 0
+    assert _lib.kernel_struct(fl.WallStokeslet3D()).kind == 1
+    k = _lib.kernel_struct(fl.Brinkmanlet3D(2.5))
+    assert (k.kind, k.alpha) == (2, 2.5)
+
+    class Renamed(fl.WallStokeslet3D):          # keeps the built-in block: accepted
+        pass
+    assert _lib.kernel_struct(Renamed()).kind == 1
+
+    class Custom(VectorKernel):                 # a new kernel defined by its own block
+        dim = 3
+
+        @staticmethod
+        def block(x, x0, eps, out):
+            out[:] = 0.0
+
+    class Overridden(fl.Stokeslet3D):           # a built-in whose block is replaced
+        @staticmethod
+        def block(x, x0, eps, out):
+            out[:] = 1.0
+
+    for bad in (Custom(), Overridden(), Kernel()):
+        with pytest.raises(NotImplementedError, match="Stokeslet3D, WallStokeslet3D, Brinkmanlet3D"):
+            _lib.kernel_struct(bad)
+    with pytest.raises(NotImplementedError):
+        fl.evaluate(Custom(), np.ones((2, 3)), np.zeros((2, 3)), np.ones((2, 3)), 0.1)

@@ -41,3 +41,28 @@ def test_identity_padding_keeps_the_factors_and_the_solution(n, n_pad):
     assert np.array_equal(x[n:], np.zeros((n_pad - n, 6)))
     ref = np.linalg.solve(K, b)
     assert np.linalg.norm(x[:n] - ref) <= 1e-13 * np.linalg.cond(K) * np.linalg.norm(ref)
+
+
+def test_lazy_pool_captures_only_in_its_creating_thread():
+    """A capture that overlaps other threads' runtime calls can end the
+    process, so a pool without streams refuses to capture from another
+    thread (checked before any GPU call)."""
+    import threading
+
+    class FakeStream:
+        cuda_stream = 12345
+
+    pool = G.GraphPool()
+    seen = []
+
+    def other():
+        try:
+            pool.get(100, FakeStream())
+        except Exception as exc:       # noqa: BLE001 - the exception type is the check
+            seen.append(exc)
+
+    th = threading.Thread(target=other)
+    th.start()
+    th.join()
+    assert len(seen) == 1 and isinstance(seen[0], RuntimeError) and "creat" in str(seen[0])
+    assert pool.captures == 0

@@ -325,3 +325,26 @@ def test_planted_series_mistake_fails_the_high_precision_pin():
     for z, e1 in small:
         assert abs(f1_series(z) / e1 - 1) < 1e-14
         assert abs(f1_series(z, flip=1) / e1 - 1) > 1e-6
+
+
+def test_grand_resistance_of_a_sphere():
+    """oracle.rigid_mode_velocities and oracle.grand_resistance (the six-mode
+    pattern of examples/translating_sphere/run.py:45-48) on a free-space
+    sphere: translation drag 6 pi mu a, rotation torque 8 pi mu a^3 (both to the
+    regularization error), no translation-rotation coupling, and a symmetric
+    R (Lorentz reciprocity).  A swapped cross product or mode order fails
+    the signs or the diagonal structure."""
+    n = 300
+    c = np.array([0.3, -0.2, 0.1])
+    X = fibonacci_sphere(n, center=c)
+    eps = 0.25 * np.sqrt(4 * np.pi / n)
+    modes = O.rigid_mode_velocities(X, c)
+    assert modes.shape == (6, n, 3)
+    assert np.allclose(modes[4], np.cross([0.0, 1.0, 0.0], X - c))
+    forces = [O.resistance("stokeslet", X, modes[m], eps)[0] for m in range(6)]
+    R = O.grand_resistance(forces, X, c)
+    assert np.allclose(np.diag(R)[:3], 6 * np.pi, rtol=0.05)
+    assert np.allclose(np.diag(R)[3:], 8 * np.pi, rtol=0.06)
+    off = R - np.diag(np.diag(R))
+    assert np.abs(off).max() < 2e-2 * np.abs(R).max()
+    assert np.abs(R - R.T).max() < 2e-2 * np.abs(R).max()
