@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libfluidlets_hip.so"
 FL_OK, FL_EINVAL, FL_EHIP, FL_ENOCONV, FL_ESINGULAR, FL_EWORKSPACE = 0, -1, -2, -3, -4, -5
 FL_FP64, FL_FP32C = 0, 1
 FL_LU_FP32, FL_LU_BF16X3 = 0, 1
+FL_EVAL_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
 class fl_kernel(ctypes.Structure):
@@ -30,6 +31,8 @@ _SIGNATURES = {
     "fl_eval_sym_workspace_size": (c_size_t, [c_int64, c_int, c_int]),
     "fl_eval": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                         c_void_p, c_double, c_double, c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
+    "fl_eval_noalloc": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
+                                c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_int, c_void_p]),
     "fl_assemble": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_double,
                             c_double, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p]),
     "fl_assemble_nearest": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p,

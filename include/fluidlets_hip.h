@@ -111,6 +111,23 @@ int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N,
             double* u_re, double* u_im, int precision,
             void* workspace, size_t workspace_bytes, void* stream);
 
+/* fl_eval without a workspace, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It has no counterpart in the reference.  The partial sums live
+ * in a fixed device array of FL_EVAL_NOALLOC_SCRATCH_BYTES per GPU, inside the
+ * library, resolved once per device; the call allocates nothing.  Same
+ * arguments and results as fl_eval given a workspace of exactly
+ * FL_EVAL_NOALLOC_SCRATCH_BYTES bytes (bit for bit).  A sum whose partial
+ * sums need more (fl_eval_workspace_size() above the limit; not the case for
+ * any M and N on a 256-CU MI355X) returns FL_EWORKSPACE.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other host threads) race, and must be ordered by the caller. */
+#define FL_EVAL_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int fl_eval_noalloc(const fl_kernel* kernel, int64_t M, int64_t N,
+                    const double* tgt, const double* src, const double* eps, int64_t eps_stride,
+                    const double* f_re, const double* f_im, double mu_re, double mu_im,
+                    double* u_re, double* u_im, int precision, void* stream);
+
 /* ------------------------------------------------------------------ assembly
  * K[3i+a, 3j+b] = G_ab(x_i, y_j; eps_j) / mu,  i < M, j < N.
  * Replaces _fill called at src/fluidlets/core/assemble.py:45 (loop
