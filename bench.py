@@ -18,7 +18,14 @@ FP32C with their errors against FP64, the roofline of the dominant kernel
 the launch stream) and a CPU baseline: the oracle's C/OpenMP restatement of
 the reference loop timed on this host's cores over a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--solve]
+The default run also carries the other legs of the workloads: the icosphere
+solve, the 8,192-point sphere resistance solves (WallStokeslet3D and
+Brinkmanlet3D at alpha = 1, 5, 10; mixed precision against this library's
+FP64 LU and rocSOLVER's), the 131,072-point rod-carpet GMRES (FP64, flexible
+with FP32C inner cycles, block-Jacobi) and the 2,000-request serving mix.
+`bench_wall_s` is the whole run's wall time.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-solve] [--no-serve] [--no-gmres]
 """
 from __future__ import annotations
 
@@ -295,72 +302,101 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
     return out
 
 
-def serving_section(torch, n_requests=40, swimmer_steps=50, n_max=3000, slots=4, seed=5, cache_gib=48,
-                    rates=(0.9,)):
-    """Serving mix (70 % six-mode resistance solves on ellipsoids, 30 %
-    biflagellate RK2 swimmers), reduced in size so the default run stays
-    short: n_requests requests, N <= n_max, swimmer_steps RK2 steps (50 = one
-    beat).  The same trace is served one request at a time in plain FP64 (the
-    baseline: every step factors its matrix) and by the continuous-batching
-    server (mixed precision, `slots` streams, a resident factor cache of
-    cache_gib GiB that starts empty): offline (all requests queued at t = 0)
-    and online (Poisson arrivals at each fraction in `rates` of the
-    baseline's measured capacity; the first fraction's results are also
-    reported under the top-level online keys)."""
+def serving_section(torch, n_requests=2000, baseline_requests=300, swimmer_steps=200, n_max=6000, slots=4, seed=5,
+                    cache_gib=48, load=0.9, small_solves=True):
+    """Serving mix: n_requests Poisson-arrival requests, 70 % six-mode
+    resistance solves on ellipsoids (N uniform in [200, n_max]) and 30 %
+    biflagellate swimmer time steps (RK2 steps of swimmer sessions of about
+    swimmer_steps steps each, their state kept by the server between steps).
+
+    - The continuous-batching server (mixed precision, `slots` streams, a
+      resident factor cache of cache_gib GiB that starts empty) serves the
+      whole trace offline (all queued at t = 0: its capacity) and online
+      (Poisson arrivals at `load` x that capacity).
+    - The baseline, one request at a time in plain FP64 on the same GPU
+      (every solve factors its matrix), serves the first baseline_requests
+      requests of the trace (the same mix) offline and online at `load` x
+      its own capacity; the server then serves that same subset at the
+      baseline's online arrival rate, for latencies at equal load."""
     from fluidlets_amd.serving import Request, SolveServer, serving_mix, summarize
 
     reqs = serving_mix(n_requests, seed=seed, n_max=n_max, swimmer_steps=swimmer_steps, rate=1.0)
+    sub = sorted(reqs, key=lambda r: r.arrival)[:baseline_requests]
     warm = [r for r in reqs if r.kind == "resistance"][:2]
     base_srv = SolveServer(slots=1, precision="fp64")
     base_srv.serve(warm, realtime=False)               # warm-up: first allocations and library start-up
-    SolveServer(slots=slots, precision="mixed").serve(warm, realtime=False)
+    with SolveServer(slots=slots, precision="mixed") as w:
+        w.serve(warm, realtime=False)
 
     def fresh():
         return SolveServer(slots=slots, precision="mixed", cache_bytes=int(cache_gib) << 30)
 
-    def tick(tag):
+    def tick(tag, total):
         done = [0]
 
         def on_result(res):
             done[0] += 1
-            if res.kind == "swimmer" or done[0] % 10 == 0:
-                progress(f"serving: {tag} {done[0]}/{n_requests} done ({res.kind}, {res.latency:.2f} s)")
+            if done[0] % 100 == 0 or done[0] == total:
+                progress(f"serving: {tag} {done[0]}/{total} done")
         return on_result
 
-    out = {"requests": n_requests, "slots": slots, "swimmer_steps": swimmer_steps, "n_max": n_max,
-           "server_cache_gib": cache_gib}
-    off_base = summarize(base_srv.serve(reqs, realtime=False, on_result=tick("offline baseline")))
-    progress(f"serving: offline baseline {off_base['solves_per_s']:.3f} requests/s")
-    fast_srv = fresh()
-    off_fast = summarize(fast_srv.serve(reqs, realtime=False, on_result=tick("offline server")))
-    progress(f"serving: offline server {off_fast['solves_per_s']:.3f} requests/s")
-    off_fast["cache_hits"], off_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
-    out["offline_baseline_fp64_one_at_a_time"] = off_base
-    out["offline_server_mixed"] = off_fast
-    sweep = []
-    for frac in rates:
-        lam = frac * off_base["solves_per_s"]
-        t = 0.0
-        rng = np.random.default_rng(seed + 1)
-        trace = []
-        for r in reqs:
+    def poisson(trace, lam, rs):
+        rng = np.random.default_rng(rs)
+        t, out = 0.0, []
+        for r in sorted(trace, key=lambda r: r.arrival):
             t += rng.exponential(1.0 / lam)
-            trace.append(Request(r.kind, r.payload, t, r.rid))
-        on_base = summarize(base_srv.serve(trace, realtime=True, on_result=tick(f"online {frac} baseline")))
-        progress(f"serving: online {frac} x capacity, baseline p99 {on_base['latency_p99_s']:.3f} s")
-        fast_srv = fresh()
-        on_fast = summarize(fast_srv.serve(trace, realtime=True, on_result=tick(f"online {frac} server")))
-        progress(f"serving: online {frac} x capacity, server p99 {on_fast['latency_p99_s']:.3f} s")
-        on_fast["cache_hits"], on_fast["cache_misses"] = fast_srv.cache.hits, fast_srv.cache.misses
-        sweep.append({"capacity_fraction": frac, "arrival_rate_per_s": lam,
-                      "baseline_fp64_one_at_a_time": on_base, "server_mixed": on_fast})
-    out["arrival_rate_per_s"] = sweep[0]["arrival_rate_per_s"]
-    out["online_baseline_fp64_one_at_a_time"] = sweep[0]["baseline_fp64_one_at_a_time"]
-    out["online_server_mixed"] = sweep[0]["server_mixed"]
-    if len(sweep) > 1:
-        out["online_sweep"] = sweep
+            out.append(Request(r.kind, r.payload, t, r.rid))
+        return out
+
+    def served(srv, trace, realtime, tag):
+        res = srv.serve(trace, realtime=realtime, on_result=tick(tag, len(trace)))
+        s = summarize(res)
+        errors = [r.error for r in res if r.error]
+        if errors:
+            s["first_errors"] = errors[:3]
+            progress(f"serving: {tag}: {len(errors)} of {len(res)} requests failed, first: {errors[0]}")
+        if srv.cache is not None:
+            s["cache_hits"], s["cache_misses"] = srv.cache.hits, srv.cache.misses
+        torch.cuda.empty_cache()
+        return s
+
+    def fmt(v, f=".4f"):
+        return "n/a" if v is None else format(v, f)
+
+    kinds = {k: sum(r.kind == k for r in reqs) for k in ("resistance", "swim_step")}
+    sessions = sorted({r.payload["session"] for r in reqs if r.kind == "swim_step"})
+    out = {"requests": n_requests, "kinds": kinds, "swimmer_sessions": len(sessions),
+           "steps_per_session": [sum(r.kind == "swim_step" and r.payload["session"] == k for r in reqs)
+                                 for k in sessions],
+           "slots": slots, "n_max": n_max, "server_cache_gib": cache_gib, "load": load,
+           "baseline_requests": len(sub)}
+    t_all = time.perf_counter()
+    with fresh() as srv:
+        off_fast = served(srv, reqs, False, "offline server")
+    progress(f"serving: offline server {off_fast['solves_per_s']:.2f} requests/s")
+    off_base = served(base_srv, sub, False, "offline baseline")
+    progress(f"serving: offline baseline {off_base['solves_per_s']:.3f} requests/s")
+    lam_s = load * off_fast["solves_per_s"]
+    with fresh() as srv:
+        on_fast = served(srv, poisson(reqs, lam_s, seed + 1), True, "online server")
+    progress(f"serving: online server at {lam_s:.1f}/s: p50 {fmt(on_fast['latency_p50_s'])} s, "
+             f"p99 {fmt(on_fast['latency_p99_s'])} s")
+    lam_b = load * off_base["solves_per_s"]
+    sub_trace = poisson(sub, lam_b, seed + 2)
+    on_base = served(base_srv, sub_trace, True, "online baseline")
+    progress(f"serving: online baseline at {lam_b:.2f}/s: p99 {fmt(on_base['latency_p99_s'])} s")
+    with fresh() as srv:
+        on_fast_sub = served(srv, sub_trace, True, "server at the baseline's rate")
+    base_srv.close()
+    out["offline_server_mixed"] = off_fast
+    out["offline_baseline_fp64_one_at_a_time"] = off_base
     out["offline_throughput_speedup"] = off_fast["solves_per_s"] / off_base["solves_per_s"]
-    out["small_solves"] = small_solves_section(slots=slots, seed=seed)
+    out["online_server_mixed"] = dict(on_fast, arrival_rate_per_s=lam_s)
+    out["online_baseline_fp64_one_at_a_time"] = dict(on_base, arrival_rate_per_s=lam_b)
+    out["online_server_mixed_at_baseline_rate"] = dict(on_fast_sub, arrival_rate_per_s=lam_b)
+    out["wall_s"] = time.perf_counter() - t_all
+    if small_solves:
+        out["small_solves"] = small_solves_section(slots=slots, seed=seed)
     return out
 
 
@@ -440,23 +476,26 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-targets", type=int, default=4096)
+    ap.add_argument("--cpu-targets", type=int, default=16384)
     ap.add_argument("--no-table", action="store_true", help="skip the other kernels of the workload")
     ap.add_argument("--no-solve", action="store_true", help="skip the N=8192 resistance-solve section")
-    ap.add_argument("--solve-kernels", default="wall", help="comma list: wall, brinkman:1, brinkman:5, ...")
+    ap.add_argument("--solve-kernels", default="wall,brinkman:1,brinkman:5,brinkman:10",
+                    help="comma list: wall, brinkman:1, brinkman:5, ...")
     ap.add_argument("--solve-points", type=int, default=8192)
     ap.add_argument("--no-vendor-lu", action="store_true", help="skip the rocSOLVER FP64 LU comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix section")
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-swimmer-steps", type=int, default=50)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-baseline-requests", type=int, default=300,
+                    help="requests of the trace (its first, in arrival order) the one-at-a-time baseline serves")
+    ap.add_argument("--serve-swimmer-steps", type=int, default=200)
     ap.add_argument("--serve-cache-gib", type=int, default=48, help="resident factor cache of the mixed server")
-    ap.add_argument("--serve-nmax", type=int, default=3000)
-    ap.add_argument("--serve-rates", default="0.9",
-                    help="comma list of online arrival rates as fractions of the baseline's capacity")
-    ap.add_argument("--gmres", action="store_true", help="run the 131,072-point GMRES rod-carpet section")
+    ap.add_argument("--serve-nmax", type=int, default=6000)
+    ap.add_argument("--serve-load", type=float, default=0.9, help="online arrival rate as a fraction of capacity")
+    ap.add_argument("--no-gmres", action="store_true", help="skip the 131,072-point GMRES rod-carpet section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (the reference point for HBM byte counters)")
     args = ap.parse_args()
+    t_bench = time.perf_counter()
     if args.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
               "(the kernel sum shards by targets with an all-gather of sources; see the design notes)",
@@ -579,17 +618,19 @@ def main():
         progress("wall-sphere solves done")
 
     if not args.no_serve:
-        rates = tuple(float(v) for v in args.serve_rates.split(",") if v.strip())
-        result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_swimmer_steps, args.serve_nmax,
-                                                cache_gib=args.serve_cache_gib, rates=rates or (0.9,))
+        result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_baseline_requests,
+                                                args.serve_swimmer_steps, args.serve_nmax,
+                                                cache_gib=args.serve_cache_gib, load=args.serve_load)
+        progress("serving mix done")
 
-    if args.gmres:
+    if not args.no_gmres:
         result["rod_carpet_gmres"] = gmres_section(torch, fl, D)
         progress("rod-carpet GMRES done")
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(X, F, args.cpu_targets)
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+    result["bench_wall_s"] = time.perf_counter() - t_bench
     print(json.dumps(result))
 
 

@@ -94,6 +94,14 @@ class FactorCache:
                 self.inflight.pop(tag, None)
             ev.set()
 
+    def clear(self) -> None:
+        """Drop every item (their device memory goes back to the allocator);
+        the hit and miss counts stay."""
+        with self.lock:
+            self.items.clear()
+            self.key_uses.clear()
+            self.bytes = 0
+
     def get(self, key):
         with self.lock:
             item = self.items.get(key)

@@ -42,7 +42,7 @@ from .cache import FactorCache
 
 @dataclass
 class Request:
-    kind: str                              # "resistance" | "swimmer"
+    kind: str                              # "resistance" | "swim_step" | "swimmer" (a whole trajectory)
     payload: dict
     arrival: float = 0.0                   # seconds, relative to the trace start
     rid: int = 0
@@ -73,6 +73,13 @@ def footprint_bytes(req: Request, precision: str, shared_cache: bool = False) ->
     if req.kind == "resistance":
         from .core.assemble import padded_order
         n = padded_order(3 * len(req.payload["points"]))
+    elif req.kind == "swim_step":
+        n = 3 * req.payload["spec"]["n_points"]
+        per = 8 + (4 if precision == "mixed" else 8)
+        extra = 4 * n * n if precision == "mixed" else 0     # an inverse made next to its factors
+        if precision == "mixed" and not shared_cache:
+            extra += 4 * n * n                                # no shared cache: the step's own factors
+        return int(per * n * n + 512 * n + (64 << 20) + extra)
     else:
         n = 3 * req.payload["n_points"]
     per = 8 + (4 if precision == "mixed" else 8)
@@ -162,6 +169,14 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     return _grand(X, center, x[:n].cpu().numpy(), payload.get("mu", 1.0))
 
 
+def new_session(spec: dict, precision: str):
+    """The swimmer session a trace's swim_step requests advance (created at
+    its first step)."""
+    from .swimming import SwimmerSession
+    return SwimmerSession(spec["shape_at"], spec["dt"], spec["eps"], spec.get("mu", 1.0), spec.get("x0", (0, 0, 0)),
+                          spec.get("B0"), spec.get("kernel"), spec.get("t0", 0.0), precision=precision)
+
+
 def run_swimmer(payload: dict, precision: str, cache: FactorCache | None = None, stream=None):
     from .swimming import swim_rk2
     tr = swim_rk2(payload["shape_at"], 0.0, payload["dt"], payload["steps"], payload["eps"],
@@ -217,8 +232,10 @@ class SolveServer:
             job()
 
     def close(self) -> None:
-        """Stop the slot workers (after the requests they hold) and hand the
-        slots' streams back."""
+        """Stop the slot workers (after the requests they hold), drop the
+        resident factors and the captured graphs (their device memory goes
+        back to the allocator; the cache's hit and miss counts stay) and hand
+        the slots' streams back."""
         if not self._workers:
             return
         for q in self._jobs:
@@ -230,6 +247,9 @@ class SolveServer:
             s.synchronize()
             D.release_stream(s)
         self.streams = []
+        if self.cache is not None:
+            self.cache.clear()
+        self.graphs = None
 
     def __enter__(self):
         return self
@@ -237,13 +257,15 @@ class SolveServer:
     def __exit__(self, *exc):
         self.close()
 
-    def _execute(self, req: Request, stream):
+    def _execute(self, req: Request, stream, session=None):
         t = D.torch()
         with t.cuda.stream(stream):
             if req.kind == "resistance":
                 out = solve_resistance(req.payload, self.precision, self.cache, stream=stream, graphs=self.graphs)
             elif req.kind == "swimmer":
                 out = run_swimmer(req.payload, self.precision, self.cache, stream=stream)
+            elif req.kind == "swim_step":
+                out = session.step(stream=stream, factor_cache=self.cache)
             else:
                 raise ValueError(f"unknown request kind {req.kind!r}")
             stream.synchronize()
@@ -259,15 +281,34 @@ class SolveServer:
         lock = threading.Condition()
         state = {"in_use": 0, "running": 0, "t0": None, "next": 0}
         free_streams = list(range(self.slots))
+        waiting: list = []
+        # swimmer sessions of this trace: the steps of one session run one at
+        # a time, in arrival order; only a session's next step is a candidate
+        sessions: dict = {}          # key -> SwimmerSession (made at its first step)
+        pending: dict = {}           # key -> arrived steps not yet admitted, in order
+        busy: set = set()            # sessions with a step queued for admission or running
 
         def now():
             return time.perf_counter() - state["t0"]
 
+        def offer_next_step(key):
+            # under lock: the session's next arrived step becomes a candidate
+            if key not in busy and pending.get(key):
+                nxt = pending[key].pop(0)
+                busy.add(key)
+                heapq.heappush(waiting, (nxt.arrival, nxt.rid, nxt))
+
         def worker_body(req: Request, sid: int, fp: int):
             start = now()
             err, value = None, None
+            session = None
             try:
-                value = self._execute(req, self.streams[sid])
+                if req.kind == "swim_step":
+                    key = req.payload["session"]
+                    if key not in sessions:
+                        sessions[key] = new_session(req.payload["spec"], self.precision)
+                    session = sessions[key]
+                value = self._execute(req, self.streams[sid], session)
             except Exception as exc:  # a failing request is reported, the server keeps running
                 err = f"{type(exc).__name__}: {exc}"
             end = now()
@@ -277,6 +318,9 @@ class SolveServer:
                 state["in_use"] -= fp
                 state["running"] -= 1
                 free_streams.append(sid)
+                if req.kind == "swim_step":
+                    busy.discard(req.payload["session"])
+                    offer_next_step(req.payload["session"])
                 lock.notify_all()
             if on_result is not None:
                 on_result(res)
@@ -284,13 +328,17 @@ class SolveServer:
         if not self._workers:
             raise RuntimeError("the server is closed")
         state["t0"] = time.perf_counter()
-        waiting: list = []
         i = 0
         with lock:
             while i < len(reqs) or waiting or state["running"]:
                 # move arrived requests into the waiting list (arrival order)
                 while i < len(reqs) and (not realtime or reqs[i].arrival <= now()):
-                    heapq.heappush(waiting, (reqs[i].arrival, reqs[i].rid, reqs[i]))
+                    r = reqs[i]
+                    if r.kind == "swim_step":
+                        pending.setdefault(r.payload["session"], []).append(r)
+                        offer_next_step(r.payload["session"])
+                    else:
+                        heapq.heappush(waiting, (r.arrival, r.rid, r))
                     i += 1
                 admitted = False
                 while waiting and free_streams:
@@ -300,6 +348,9 @@ class SolveServer:
                         heapq.heappop(waiting)
                         results.append(Result(req.rid, req.kind, None, req.arrival if realtime else 0.0, now(), now(),
                                               f"rejected: footprint {fp} B exceeds the budget {self.budget} B"))
+                        if req.kind == "swim_step":
+                            busy.discard(req.payload["session"])
+                            offer_next_step(req.payload["session"])
                         continue
                     if state["in_use"] + fp > self.budget and state["running"] > 0:
                         break  # wait for memory to free up
@@ -339,28 +390,40 @@ def summarize(results: list[Result]) -> dict:
 
 def serving_mix(n_requests: int, seed: int = 5, n_min: int = 200, n_max: int = 6000, swimmer_frac: float = 0.3,
                 swimmer_steps: int = 200, swimmer_body: int = 1000, swimmer_flag: int = 400,
-                rate: float = 1.0) -> list[Request]:
+                rate: float = 1.0, swimmer_mode: str = "steps") -> list[Request]:
     """The serving workload: Poisson arrivals at `rate` per second; 70 %
     rigid-body resistance solves on seeded ellipsoids (semi-axes U[0.5, 2],
     N uniform in [n_min, n_max], Fibonacci points mapped onto the ellipsoid,
     centre height U[1.2, 4], draws whose vertical semi-axis reaches within
     eps of the wall rejected; WallStokeslet3D or free-space Brinkmanlet3D with
     alpha ~ U[0, 10], 50/50; eps = 0.5 sqrt(area / N); six rigid-body
-    right-hand sides) and 30 % biflagellate swimmers stepped by midpoint RK2
-    (dt = one fiftieth of the beat period)."""
+    right-hand sides) and 30 % biflagellate swimmer time steps (midpoint
+    RK2, dt = one fiftieth of the beat period).
+
+    swimmer_mode "steps": each swimmer request is one RK2 time step of a
+    swimmer session whose state the server keeps between requests; the
+    swimmer requests are dealt round-robin to round(n_swimmer / swimmer_steps)
+    sessions, so each session runs about swimmer_steps steps (the 2,000
+    requests of seed 5: 610 steps of 3 swimmers, 203-204 steps each).  The sessions start from seeded
+    positions and orientations.  swimmer_mode "trajectory": each swimmer
+    request runs a whole trajectory of swimmer_steps steps."""
     from .geometry import ellipsoid, ellipsoid_area
     from .kernels import Brinkmanlet3D, WallStokeslet3D
     from .swimming import biflagellate
 
+    if swimmer_mode not in ("steps", "trajectory"):
+        raise ValueError(f"unknown swimmer_mode {swimmer_mode!r}")
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     shape_at = None
+    swim_rids = []
     for rid in range(n_requests):
         t += rng.exponential(1.0 / rate)
         if rng.random() < swimmer_frac:
             if shape_at is None:
                 shape_at = biflagellate(n_body=swimmer_body, n_flag=swimmer_flag)
+            swim_rids.append(rid)
             out.append(Request("swimmer", {"shape_at": shape_at, "dt": 1.0 / 50, "steps": swimmer_steps,
                                            "eps": 0.01, "n_points": swimmer_body + 2 * swimmer_flag}, t, rid))
             continue
@@ -378,4 +441,18 @@ def serving_mix(n_requests: int, seed: int = 5, n_min: int = 200, n_max: int = 6
             kernel, center = Brinkmanlet3D(rng.uniform(0.0, 10.0)), (0.0, 0.0, 0.0)
         X = ellipsoid(N, axes, center)
         out.append(Request("resistance", {"kernel": kernel, "points": X, "center": center, "eps": eps}, t, rid))
+    if swimmer_mode == "steps" and swim_rids:
+        n_sessions = max(1, int(round(len(swim_rids) / max(1, swimmer_steps))))
+        srng = np.random.default_rng(seed + 1000)
+        specs = []
+        for k in range(n_sessions):
+            q, _ = np.linalg.qr(srng.standard_normal((3, 3)))
+            B0 = q * np.sign(np.linalg.det(q))
+            specs.append({"shape_at": shape_at, "dt": 1.0 / 50, "eps": 0.01, "x0": srng.uniform(-5, 5, 3),
+                          "B0": B0, "n_points": swimmer_body + 2 * swimmer_flag})
+        for j, rid in enumerate(swim_rids):
+            old = out[rid]
+            k = j % n_sessions
+            out[rid] = Request("swim_step", {"session": f"swimmer{k}", "spec": specs[k], "seq": j // n_sessions},
+                               old.arrival, rid)
     return out

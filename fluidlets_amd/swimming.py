@@ -400,6 +400,50 @@ def swim_rk2(shape_at: Callable[[float], SwimmerShape], t0: float, dt: float, st
     return Trajectory(np.array(ts), zs[:, :3], np.stack([b1, b2, np.cross(b1, b2)], axis=2))
 
 
+class SwimmerSession:
+    """A free swimmer advanced by one explicit midpoint (RK2) step per call,
+    its state (time, frame origin, frame axes) kept between calls, for
+    callers that interleave a trajectory's steps with other work (a serving
+    loop's swimmer time-step requests).  ``step`` is one step of swim_rk2
+    (the same time arithmetic and right-hand sides, without the helper
+    threads): with precision "mixed" and a free-space kernel the body-frame
+    factors and inverses live in ``factor_cache`` (shared between sessions,
+    so a shape that any session has met costs refinement only), which the
+    device keeps resident between the calls.  Steps of one session must run
+    one at a time, in order."""
+
+    def __init__(self, shape_at: Callable[[float], SwimmerShape], dt: float, eps, mu=1.0, x0=(0, 0, 0), B0=None,
+                 kernel=None, t0: float = 0.0, precision: str = "mixed"):
+        from ._nearest import nearest_neighbors
+        self.shape_at, self.dt, self.eps, self.mu = shape_at, float(dt), eps, mu
+        self.kernel, self.precision = kernel, precision
+        B0 = np.eye(3) if B0 is None else np.asarray(B0, float)
+        s0 = shape_at(t0)
+        self.nearest = nearest_neighbors(s0.X, s0.x)
+        self.t = float(t0)
+        self.z = np.concatenate([np.asarray(x0, float), B0[:, 0], B0[:, 1]])
+        self.steps = 0
+        self.lu_hint: dict = {}
+
+    def _F(self, t, z, stream, factor_cache):
+        b1, b2 = z[3:6], z[6:9]
+        B = np.c_[b1, b2, np.cross(b1, b2)]
+        cache = factor_cache if self.precision == "mixed" else None
+        st = swimming_problem(self.shape_at(t), z[:3], B, self.eps, self.mu, self.kernel, self.nearest,
+                              precision=self.precision, stream=stream, factor_cache=cache, lu_hint=self.lu_hint)
+        return np.concatenate([st.U, np.cross(st.Omega, b1), np.cross(st.Omega, b2)])
+
+    def step(self, stream=None, factor_cache: FactorCache | None = None) -> np.ndarray:
+        """Advance one step; returns [t, x0 (3), B (3 x 3, row-major)] after it."""
+        dt, t, z = self.dt, self.t, self.z
+        k1 = self._F(t, z, stream, factor_cache)
+        z = z + dt * self._F(t + 0.5 * dt, z + 0.5 * dt * k1, stream, factor_cache)
+        self.t, self.z = t + dt, z
+        self.steps += 1
+        b1, b2 = z[3:6], z[6:9]
+        return np.concatenate([[self.t], z[:3], np.c_[b1, b2, np.cross(b1, b2)].ravel()])
+
+
 def biflagellate(n_body: int = 1000, n_flag: int = 400, amplitude: float = 0.2, wavelength: float = 1.0,
                  length: float = 2.0, period: float = 1.0):
     """shape_at(t) of a model biflagellate: a 2:1 prolate spheroid body

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 def small_mix():
     reqs = serving_mix(10, seed=7, n_min=200, n_max=900, swimmer_steps=2, swimmer_body=150, swimmer_flag=40,
-                       rate=200.0)
+                       rate=200.0, swimmer_mode="trajectory")
     assert any(r.kind == "swimmer" for r in reqs) and any(r.kind == "resistance" for r in reqs)
     return reqs
 

@@ -30,8 +30,18 @@ def test_serving_mix_is_seeded_and_follows_the_spec():
     assert [r.kind for r in a] == [r.kind for r in b]
     arr = np.array([r.arrival for r in a])
     assert np.all(np.diff(arr) > 0)
-    frac = np.mean([r.kind == "swimmer" for r in a])
+    frac = np.mean([r.kind == "swim_step" for r in a])
     assert 0.2 < frac < 0.4
+    # swimmer requests are time steps of sessions of about swimmer_steps steps,
+    # dealt round-robin, in order within each session
+    steps = [r for r in a if r.kind == "swim_step"]
+    sessions = sorted({r.payload["session"] for r in steps})
+    assert len(sessions) == round(len(steps) / 2)
+    for key in sessions:
+        seq = [r.payload["seq"] for r in steps if r.payload["session"] == key]
+        assert seq == list(range(len(seq))) and len(seq) in (2, 3)
+    whole = serving_mix(200, seed=5, swimmer_steps=2, swimmer_mode="trajectory")
+    assert [r.kind == "swimmer" for r in whole] == [r.kind == "swim_step" for r in a]
     for r in a:
         if r.kind == "resistance":
             X = r.payload["points"]
