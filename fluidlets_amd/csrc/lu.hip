@@ -25,7 +25,6 @@
 #include "fl_internal.h"
 #include "mfma.h"
 
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -44,6 +43,11 @@ int gemm_pair_launch(int64_t M1, int64_t N1, int64_t K1, T alpha1, const T* A1, 
                      int64_t ldb1, T beta1, T* C1, int64_t ldc1, unsigned int* absmax1, int64_t M2, int64_t N2,
                      int64_t K2, T alpha2, const T* A2, int64_t lda2, const T* B2, int64_t ldb2, T beta2, T* C2,
                      int64_t ldc2, hipStream_t stream);
+
+// split-bf16 trailing update (gemm_bf16x3.hip)
+size_t split3_workspace_bytes(int64_t M, int64_t N, int64_t K);
+int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                  int64_t ldc, int lower, void* ws, size_t ws_bytes, hipStream_t st);
 
 namespace {
 
@@ -1317,144 +1321,24 @@ __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int
 }
 
 // ---------------------------------------------------------------- trailing update
-// A22 -= L21 U12 is a plain GEMM: large ones go to rocBLAS (one handle per
-// host thread and device, atomics off so results stay bitwise repeatable;
-// 15-20 % faster than gemm_kernel on the K = 256 trailing shapes), the rest,
-// and every product with a fused epilogue, to gemm_kernel.  FL_LU_ROCBLAS=0
-// keeps everything on gemm_kernel.
-// The handles of one host thread, destroyed when the thread ends (so a
-// process that runs many short-lived threads does not accumulate them).
-struct ThreadHandles {
-    std::vector<rocblas_handle> per_dev;
-    ~ThreadHandles()
-    {
-        for (rocblas_handle h : per_dev)
-            if (h) rocblas_destroy_handle(h);
-    }
-};
-
-int rocblas_handle_for_device(rocblas_handle& out)
-{
-    static thread_local ThreadHandles handles;
-    std::vector<rocblas_handle>& per_dev = handles.per_dev;
-    int dev = 0;
-    FL_HIP(hipGetDevice(&dev));
-    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
-    if (!per_dev[dev]) {
-        rocblas_handle h = nullptr;
-        if (rocblas_create_handle(&h) != rocblas_status_success)
-            return set_error(FL_EHIP, "rocblas_create_handle failed");
-        if (rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success) {
-            rocblas_destroy_handle(h);
-            return set_error(FL_EHIP, "rocblas_set_atomics_mode failed");
-        }
-        per_dev[dev] = h;
-    }
-    out = per_dev[dev];
-    return FL_OK;
-}
-
-inline rocblas_status rb_gemm(rocblas_handle h, int m, int n, int k, const float* alpha, const float* A, int lda,
-                              const float* B, int ldb, const float* beta, float* C, int ldc)
-{
-    return rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-}
-inline rocblas_status rb_gemm(rocblas_handle h, int m, int n, int k, const double* alpha, const double* A, int lda,
-                              const double* B, int ldb, const double* beta, double* C, int ldc)
-{
-    return rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-}
-
-// ---------------------------------------------------------------- split-bf16 products
-// An FP32 value x is written as hi + lo, hi = bf16(x) and lo = bf16(x - hi)
-// (round to nearest even; x - hi is exact in FP32), so x agrees with hi + lo
-// to 2^-16 relative.  A product a b is then taken as
-//   a_hi b_hi + a_hi b_lo + a_lo b_hi
-// (the dropped terms are below 2^-16 |a b|), each on bf16 MFMA with FP32
-// sums: one GEMM over K' = 3K with A' = [A_hi | A_hi | A_lo] and
-// B' = [B_hi ; B_lo ; B_hi].  gfx950 runs bf16 MFMA at 16 times the FP32
-// MFMA rate, so the three products cost less than one FP32 product; the FP32
+// A22 -= L21 U12 is a plain GEMM on this library's MFMA kernels: FP32 and
+// FP64 products on gemm_kernel (within a few percent of the vendor GEMM on
+// these K = 256 shapes), and, with `split` workspace (FP32 factorizations
+// with split-bf16 products), updates of 2048 or more rows and columns on the
+// split-bf16 kernel (split3_update): FP32 operands split into bf16 hi and lo
+// parts, three bf16 MFMA products per FP32 product, FP32 sums.  The FP32
 // factors they give serve as FP64 refinement's preconditioner, whose
 // convergence they leave unchanged on these matrices.
-__device__ inline uint16_t bf16_rne(float x)
-{
-    const uint32_t u = __float_as_uint(x);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));  // inf, nan
-    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ inline float bf16_value(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// A' (M x 3K, ld 3K) = [hi | hi | lo] of A (M x K, lda); one workgroup per row.
-__global__ void split_rows_kernel(const float* A, int64_t lda, int K, uint16_t* Ap)
-{
-    const int64_t i = blockIdx.x;
-    uint16_t* o = Ap + i * 3 * K;
-    for (int c = threadIdx.x; c < K; c += blockDim.x) {
-        const float x = A[i * lda + c];
-        const uint16_t h = bf16_rne(x);
-        o[c] = h;
-        o[K + c] = h;
-        o[2 * K + c] = bf16_rne(x - bf16_value(h));
-    }
-}
+size_t split_ws_bytes(int64_t n) { return align_up(split3_workspace_bytes(n, n, NBO_SPLIT), 256); }
 
-// B' (3K x N, ld N) = [hi ; lo ; hi] of B (K x N, ldb); blockIdx.y = row of B.
-__global__ void split_cols_kernel(const float* B, int64_t ldb, int64_t N, int K, uint16_t* Bp)
-{
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t k = blockIdx.y;
-    if (j >= N) return;
-    const float x = B[k * ldb + j];
-    const uint16_t h = bf16_rne(x);
-    Bp[k * N + j] = h;
-    Bp[(K + k) * N + j] = bf16_rne(x - bf16_value(h));
-    Bp[(2 * K + k) * N + j] = h;
-}
-
-size_t split_bytes(int64_t n) { return align_up((size_t)n * 3 * NBO_SPLIT * sizeof(uint16_t), 256); }
-
-// C (M x N, row-major, ldc) -= A (M x K, lda) B (K x N, ldb).  With split
-// (FP32 only: two buffers of split_bytes(max(M, N))) a large update takes the
-// split-bf16 products.
 template <typename T>
 int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
-                    uint16_t* split, hipStream_t st)
+                    void* split, size_t split_bytes, hipStream_t st)
 {
-    static const bool use_rb = [] {
-        const char* e = std::getenv("FL_LU_ROCBLAS");
-        return !(e && *e && std::atoi(e) == 0);
-    }();
-    const bool big = M >= 2048 && N >= 2048 && M < (1LL << 31) && N < (1LL << 31) && lda < (1LL << 31) &&
-                     ldb < (1LL << 31) && ldc < (1LL << 31);
-    if (use_rb && big) {
-        rocblas_handle h = nullptr;
-        if (int e = rocblas_handle_for_device(h)) return e;
-        if (rocblas_set_stream(h, st) != rocblas_status_success) return set_error(FL_EHIP, "rocblas_set_stream failed");
-        const T alpha = T(-1), beta = T(1);
-        if constexpr (std::is_same_v<T, float>) {
-            if (split && K <= NBO_SPLIT) {
-                uint16_t* Ap = split;
-                uint16_t* Bp = split + split_bytes(std::max(M, N)) / sizeof(uint16_t);
-                hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)M), dim3(256), 0, st, A, lda, (int)K, Ap);
-                FL_LAUNCHED();
-                hipLaunchKernelGGL(split_cols_kernel, dim3((unsigned)div_up(N, 256), (unsigned)K), dim3(256), 0, st, B,
-                                   ldb, N, (int)K, Bp);
-                FL_LAUNCHED();
-                // column-major C^T = alpha B'^T A'^T + beta C^T, bf16 operands, FP32 C and sums
-                if (rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, (int)N, (int)M, (int)(3 * K),
-                                    &alpha, Bp, rocblas_datatype_bf16_r, (int)N, Ap, rocblas_datatype_bf16_r,
-                                    (int)(3 * K), &beta, C, rocblas_datatype_f32_r, (int)ldc, C,
-                                    rocblas_datatype_f32_r, (int)ldc, rocblas_datatype_f32_r,
-                                    rocblas_gemm_algo_standard, 0, 0) != rocblas_status_success)
-                    return set_error(FL_EHIP, "rocBLAS gemm_ex (split bf16) failed");
-                return FL_OK;
-            }
-        }
-        // row-major C = alpha A B + beta C  <=>  column-major C^T = alpha B^T A^T + beta C^T
-        if (rb_gemm(h, (int)N, (int)M, (int)K, &alpha, B, (int)ldb, A, (int)lda, &beta, C, (int)ldc) !=
-            rocblas_status_success)
-            return set_error(FL_EHIP, "rocBLAS gemm failed");
-        return FL_OK;
+    if constexpr (std::is_same_v<T, float>) {
+        if (split && M >= 2048 && N >= 2048 && K <= NBO_SPLIT)
+            return split3_update(M, N, K, A, lda, B, ldb, C, ldc, 0, split, split_bytes, st);
     }
     return gemm_launch<T>(M, N, K, T(-1), A, lda, B, ldb, T(1), C, ldc, nullptr, st);
 }
@@ -1465,7 +1349,7 @@ template <typename T>
 size_t getrf_ws_bytes(int64_t n)
 {
     const size_t base = align_up((size_t)n * NB * sizeof(T), 256) + 256;
-    return std::is_same_v<T, float> ? base + 2 * split_bytes(n) : base;
+    return std::is_same_v<T, float> ? base + split_ws_bytes(n) : base;
 }
 
 // End of the asynchronous factorization: status = 0 when no multiplier
@@ -1515,9 +1399,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     // trailing update then reaches the library GEMM (n - NBO_SPLIT >= 2048).
     // Below that the factorization is the FP32-product one (256-column
     // panels, faster there).
-    uint16_t* split = split_products && std::is_same_v<T, float> && n - NBO_SPLIT >= 2048
-                          ? reinterpret_cast<uint16_t*>(tail + 256)
-                          : nullptr;
+    void* split = split_products && std::is_same_v<T, float> && n - NBO_SPLIT >= 2048 ? tail + 256 : nullptr;
     if (spec) {
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, flag, (int64_t)1);
         FL_LAUNCHED();
@@ -1654,7 +1536,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             }
             // trailing update with K = nbo_w
             if (int s = trailing_update<T>(nr, nr, nbo, A + pe * lda + k, lda, A + k * lda + pe, lda, A + pe * lda + pe,
-                                           lda, split, st))
+                                           lda, split, split ? split_ws_bytes(n) : 0, st))
                 return s;
         }
     }
@@ -1968,7 +1850,7 @@ int getri_impl(const float* LU, int64_t n, int64_t lda, const void* aux_c, float
         const int64_t below = n - k0 - nb;
         if (below > 0)
             if (int s = trailing_update<float>(below, n, nb, LU + (k0 + nb) * lda + k0, lda, Xp, ldx, Xp + nb * ldx, ldx,
-                                               nullptr, st))
+                                               nullptr, 0, st))
                 return s;
     }
     for (int64_t p = nblk - 1; p >= 0; --p) {
@@ -1979,7 +1861,7 @@ int getri_impl(const float* LU, int64_t n, int64_t lda, const void* aux_c, float
                                        nullptr, st))
             return s;
         if (k0 > 0)
-            if (int s = trailing_update<float>(k0, n, nb, LU + k0, lda, Xp, ldx, X, ldx, nullptr, st)) return s;
+            if (int s = trailing_update<float>(k0, n, nb, LU + k0, lda, Xp, ldx, X, ldx, nullptr, 0, st)) return s;
     }
     return FL_OK;
 }
