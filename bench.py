@@ -222,13 +222,17 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
         # (condition ~70); one more correction puts it near 1e-14
         refine_tol = 1e-12
         row["refinement_tol"] = refine_tol
-        for products in ("fp32", "bf16x3"):
+        # the symmetric factorization (A = L D L^T, half the flops; these
+        # matrices are symmetric by reciprocity), FP32 and split-bf16
+        # products, then the LU ones; the LU with split-bf16 products last
+        # (its numbers are the row's headline mixed solve)
+        for sym, products in ((True, "fp32"), (True, "bf16x3"), (False, "fp32"), (False, "bf16x3")):
             bestm = None
             for _ in range(repeats):
                 A32 = K32.clone()
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                F32 = linalg.getrf(A32, refill=lambda X: X.copy_(K32), products=products)
+                F32 = linalg.getrf(A32, refill=lambda X: X.copy_(K32), products=products, sym=sym)
                 xm, steps, res = linalg.refine(K64, F32, rhs, tol=refine_tol)
                 torch.cuda.synchronize()
                 dt = time.perf_counter() - t0
@@ -237,13 +241,18 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
                 xm2, _, _ = linalg.refine(K64, F32, rhs, tol=refine_tol)
                 torch.cuda.synchronize()
                 resolve = time.perf_counter() - t1
+                sym_ran = bool(F32.extra.get("sym"))
                 del F32, A32
             xmh = xm.cpu().numpy()
             err = float(np.linalg.norm(xmh - x64h) / np.linalg.norm(x64h))
-            if products == "fp32":
-                row["mixed_fp32_products_lu_refine_s"] = bestm
-                row["mixed_fp32_products_refinement_steps"] = steps
-                row["mixed_fp32_products_vs_fp64_force_relerr"] = err
+            tag = ("mixed_sym_" if sym else "mixed_") + products + "_products"
+            if sym or products == "fp32":
+                row[tag + "_lu_refine_s"] = bestm
+                row[tag + "_refinement_steps"] = steps
+                row[tag + "_vs_fp64_force_relerr"] = err
+                row[tag + "_speedup_vs_fp64_lu"] = best64 / bestm
+            if sym:
+                row[tag + "_factorization_was_symmetric"] = sym_ran
         row["mixed_products"] = "bf16x3"
         row["mixed_lu_refine_s"] = bestm
         row["mixed_resolve_s"] = resolve
@@ -254,6 +263,8 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
         row["grand_resistance_fp64"] = R64.round(6).tolist()
         row["grand_resistance_max_entry_relerr"] = float(np.abs(Rm - R64).max() / np.abs(R64).max())
         row["speedup_vs_fp64_lu"] = best64 / bestm
+        row["best_mixed_s"] = min(bestm, row.get("mixed_sym_bf16x3_products_lu_refine_s", bestm))
+        row["best_speedup_vs_fp64_lu"] = best64 / row["best_mixed_s"]
         # rocSOLVER FP64 (torch.linalg) as the library baseline, best of `repeats` like the others
         try:
             if not vendor:
@@ -296,7 +307,10 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
             except Exception as exc:  # report, do not fail the bench
                 row["cpu_lapack_error"] = repr(exc)[:200]
         out[name] = row
-        progress(f"wall-sphere solve {name}: mixed {row['mixed_lu_refine_s']:.4f} s, FP64 {row['fp64_lu_s']:.4f} s")
+        progress(f"wall-sphere solve {name}: mixed LU {row['mixed_lu_refine_s']:.4f} s, mixed symmetric "
+                 f"{row['mixed_sym_bf16x3_products_lu_refine_s']:.4f} s (FP32 products: LU "
+                 f"{row['mixed_fp32_products_lu_refine_s']:.4f} s, symmetric "
+                 f"{row['mixed_sym_fp32_products_lu_refine_s']:.4f} s), FP64 LU {row['fp64_lu_s']:.4f} s")
         del K64, K32
         torch.cuda.empty_cache()
     return out

@@ -58,6 +58,10 @@ _SIGNATURES = {
                              c_void_p, c_size_t, c_void_p]),
     "fl_sgetrf_async_ex": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                    c_size_t, c_void_p]),
+    "fl_ssytrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_size_t,
+                                c_void_p]),
+    "fl_dsytrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                c_void_p]),
     "fl_dgetrf_async": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                 c_void_p]),
     "fl_getrs_workspace_size": (c_size_t, [c_int64, c_int64, c_int]),

@@ -72,6 +72,10 @@ int sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, uns
                  void* ws, size_t wsb, hipStream_t st);
 int dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
                  size_t wsb, hipStream_t st);
+int ssytrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, int products,
+                 void* ws, size_t wsb, hipStream_t st);
+int dsytrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
+                 size_t wsb, hipStream_t st);
 int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb, void* ws,
            size_t wsb, hipStream_t st);
 int dgetrs(const double* LU, int64_t n, int64_t lda, const void* aux, double* B, int64_t nrhs, int64_t ldb, void* ws,
@@ -229,6 +233,20 @@ int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux,
 {
     clear_error();
     return dgetrf_async(A, n, lda, ipiv, aux, status, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_ssytrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, int products,
+                    void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return ssytrf_async(A, n, lda, ipiv, aux, status, products, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_dsytrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                    void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    return dsytrf_async(A, n, lda, ipiv, aux, status, workspace, workspace_bytes, as_stream(stream));
 }
 
 size_t fl_getrs_workspace_size(int64_t n, int64_t nrhs, int precision)

@@ -47,6 +47,8 @@ struct GemmArgs {
     int aligned;  // A, B base and leading dimensions allow 16-byte loads
     int tiles_m, tiles_n;
     int swizzle;  // 1: XCD-aware grouped tile order
+    int lower;    // only the tiles on or below the diagonal (M == N), enumerated densely
+    int ntiles;   // tiles of the product (all, or the lower ones)
 };
 
 // Tile of this workgroup.  The grid is 1-D; workgroups are dealt to the 8
@@ -55,14 +57,33 @@ struct GemmArgs {
 // of GROUP_M tile rows (column by column inside a group), so the workgroups
 // resident on one XCD share A row blocks and B column blocks in its L2.
 constexpr int NXCD = 8, GROUP_M = 8;
-template <typename T>
-__device__ __forceinline__ void tile_of(const GemmArgs<T>& g, int id, int& tm, int& tn)
+// Row r of the lower triangle of square tiles holds r + 1 tiles: (r, 0 .. r).
+__device__ __forceinline__ void lower_tile(int id, int& tm, int& tn)
 {
+    int r = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= id) ++r;
+    while (r * (r + 1) / 2 > id) --r;
+    tm = r;
+    tn = id - r * (r + 1) / 2;
+}
+
+// false for a workgroup past the last tile (the lower launch is padded to a
+// multiple of the XCD count so that each XCD takes a contiguous share)
+template <typename T>
+__device__ __forceinline__ bool tile_of(const GemmArgs<T>& g, int id, int& tm, int& tn)
+{
+    if (g.lower) {
+        const int nl = ((g.ntiles + NXCD - 1) / NXCD) * NXCD;
+        id = (id % NXCD) * (nl / NXCD) + id / NXCD;
+        if (id >= g.ntiles) return false;
+        lower_tile(id, tm, tn);
+        return true;
+    }
     const int nt = g.tiles_m * g.tiles_n;
     if (!g.swizzle) {
         tm = id / g.tiles_n;
         tn = id % g.tiles_n;
-        return;
+        return true;
     }
     if (nt % NXCD == 0) id = (id % NXCD) * (nt / NXCD) + id / NXCD;
     const int per_group = GROUP_M * g.tiles_n;
@@ -72,6 +93,7 @@ __device__ __forceinline__ void tile_of(const GemmArgs<T>& g, int id, int& tm, i
     const int in = id % per_group;
     tm = first_m + in % gm;
     tn = in / gm;
+    return true;
 }
 
 // One 128 x 128 tile of C for workgroup-local tile index `id` of problem g.
@@ -95,7 +117,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
     const int lane = tid % 64, wave = tid / 64;
     const int wm = wave / 2, wn = wave % 2;
     int tm, tn;
-    tile_of(g, id, tm, tn);
+    if (!tile_of(g, id, tm, tn)) return;
     const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
     const bool full_mn = g.aligned && m0 + TM <= g.M && n0 + TN <= g.N;
 
@@ -293,7 +315,9 @@ GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int6
     static const char* sw_env = std::getenv("FL_GEMM_SWIZZLE");
     const int swizzle = sw_env && *sw_env ? std::atoi(sw_env) : 1;
     const int tiles_m = (int)((M + TM - 1) / TM), tiles_n = (int)((N + TN - 1) / TN);
-    return GemmArgs<T>{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc, absmax_bits, aligned, tiles_m, tiles_n, swizzle};
+    return GemmArgs<T>{M,     N,          K,           alpha,   beta,    A,       lda,   B,
+                       ldb,   C,          ldc,         absmax_bits, aligned, tiles_m, tiles_n, swizzle,
+                       0,     tiles_m * tiles_n};
 }
 
 }  // namespace
@@ -348,5 +372,28 @@ template int gemm_launch<float>(int64_t, int64_t, int64_t, float, const float*, 
                                 float*, int64_t, unsigned int*, hipStream_t);
 template int gemm_launch<double>(int64_t, int64_t, int64_t, double, const double*, int64_t, const double*, int64_t,
                                  double, double*, int64_t, unsigned int*, hipStream_t);
+
+// C -= A B on the tiles of C (n x n) on or below the diagonal only (the tiles
+// straddling it are updated whole): the trailing update of a symmetric
+// factorization, half the flops of the full product.
+template <typename T>
+int gemm_lower_update(int64_t n, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
+                      hipStream_t stream)
+{
+    FL_CHECK(n >= 0 && K >= 0, "negative GEMM size");
+    if (n == 0) return FL_OK;
+    GemmArgs<T> g = gemm_args<T>(n, n, K, T(-1), A, lda, B, ldb, T(1), C, ldc, nullptr);
+    g.lower = 1;
+    g.ntiles = g.tiles_m * (g.tiles_m + 1) / 2;
+    const int launch = (g.ntiles + NXCD - 1) / NXCD * NXCD;
+    hipLaunchKernelGGL(gemm_kernel<T>, dim3((unsigned)launch), dim3(WG), 0, stream, g);
+    FL_LAUNCHED();
+    return FL_OK;
+}
+
+template int gemm_lower_update<float>(int64_t, int64_t, const float*, int64_t, const float*, int64_t, float*, int64_t,
+                                      hipStream_t);
+template int gemm_lower_update<double>(int64_t, int64_t, const double*, int64_t, const double*, int64_t, double*,
+                                       int64_t, hipStream_t);
 
 }  // namespace fl

@@ -21,18 +21,17 @@
 //
 // Product kernel: four waves in a 2 x 2 grid, each wave a (32 WM) x (32 WN)
 // tile of C held in accumulators, which start from -C (exact) and are stored
-// negated: C - A B.  Operand fragments go straight from global memory (L2)
-// to registers, the next k block's loaded while the current one is
-// multiplied: no LDS and no barriers.  Workgroups are dealt to the 8 XCDs
-// round-robin, and tiles are ordered so that each XCD's L2 sees a compact
-// block of C tiles (shared A row panels and B column panels).
+// negated: C - A B.  Each k block's fragments are loaded once per workgroup
+// through a register ring into an LDS double buffer (below).  Workgroups are
+// dealt to the 8 XCDs round-robin, and tiles are ordered so that each XCD's
+// L2 sees a compact block of C tiles (shared A row panels and B column
+// panels).
 //
-// Optional tile filter: only tiles whose rows reach the diagonal or below
-// (the lower triangle of C, for symmetric updates).
+// Optional tile filter: only the tiles on or below the diagonal (the lower
+// triangle of C, for the symmetric factorization's updates).
 #include "fl_internal.h"
 
 #include <algorithm>
-#include <cstdlib>
 
 namespace fl {
 namespace {
@@ -140,20 +139,25 @@ constexpr int NXCD = 8, GROUP = 8;
 // (tm, tn) of logical tile `id` among tiles_m x tiles_n: dealt to XCDs in
 // contiguous ranges (workgroup ids go round-robin over the 8 XCDs), then in
 // groups of GROUP tile rows, column by column inside a group.  With `lower`
-// the logical tiles are the lower triangle's, row by row.
-__device__ inline void bf3_tile(const Bf3Args& g, int id, int& tm, int& tn)
+// the logical tiles are those on or below the diagonal (tile row t holds
+// q (t + 1) tiles, q = tile height / tile width), row by row, and the launch
+// is padded to a multiple of 8 workgroups; false for a workgroup past the
+// last tile.
+__device__ inline bool bf3_tile(const Bf3Args& g, int id, int q, int& tm, int& tn)
 {
+    if (g.lower) {
+        const int nl = (g.ntiles + NXCD - 1) / NXCD * NXCD;
+        id = (id % NXCD) * (nl / NXCD) + id / NXCD;
+        if (id >= g.ntiles) return false;
+        int r = (int)((sqrtf(8.0f * (float)id / (float)q + 1.0f) - 1.0f) * 0.5f);
+        while (q * (r + 1) * (r + 2) / 2 <= id) ++r;
+        while (q * r * (r + 1) / 2 > id) --r;
+        tm = r;
+        tn = id - q * r * (r + 1) / 2;
+        return true;
+    }
     const int nt = g.ntiles;
     if (nt % NXCD == 0) id = (id % NXCD) * (nt / NXCD) + id / NXCD;
-    if (g.lower) {
-        // tiles (tm, tn) with tn <= tm (square tiles): row tm holds tm + 1 tiles
-        int r = (int)((sqrtf(8.0f * (float)id + 1.0f) - 1.0f) * 0.5f);
-        while ((r + 1) * (r + 2) / 2 <= id) ++r;
-        while (r * (r + 1) / 2 > id) --r;
-        tm = r;
-        tn = id - r * (r + 1) / 2;
-        return;
-    }
     const int per_group = GROUP * g.tiles_n;
     const int group = id / per_group;
     const int first_m = group * GROUP;
@@ -161,6 +165,7 @@ __device__ inline void bf3_tile(const Bf3Args& g, int id, int& tm, int& tn)
     const int in = id % per_group;
     tm = first_m + in % gm;
     tn = in / gm;
+    return true;
 }
 
 __device__ inline int d_row(int l, int i) { return 8 * (i / 4) + 4 * (l / 32) + (i % 4); }
@@ -224,66 +229,7 @@ __device__ __forceinline__ void store_neg_c(const Bf3Args& g, int64_t r0, int64_
     }
 }
 
-template <int WM, int WN>
-__global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_kernel(Bf3Args g)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int tm, tn;
-    bf3_tile(g, (int)blockIdx.x, tm, tn);
-    const int64_t rb0 = ((int64_t)tm * 2 + wave / 2) * WM;   // this wave's first row block
-    const int64_t cb0 = ((int64_t)tn * 2 + wave % 2) * WN;   // and first column block
-    const int64_t r0 = rb0 * FE, c0 = cb0 * FE;
-    const bool full = r0 + WM * FE <= g.M && c0 + WN * FE <= g.N;
-    if (r0 >= g.M || c0 >= g.N) return;   // a wave wholly outside C (edge workgroup)
-
-    floatx16 acc[WM][WN];
-    load_neg_c<WM, WN>(g, r0, c0, lane, full, acc);
-
-    // fragment pointers: A block (rb, kb) hi at ((rb KB + kb) 2) 64 lanes, lo 64 lanes after
-    const bf16x8* Ap = reinterpret_cast<const bf16x8*>(g.Apk) + rb0 * g.KB * 128 + lane;
-    const bf16x8* Bp = reinterpret_cast<const bf16x8*>(g.Bpk) + cb0 * g.KB * 128 + lane;
-    const int64_t rstride = (int64_t)g.KB * 128;   // next row (column) block
-
-    bf16x8 ah0[WM], al0[WM], bh0[WN], bl0[WN];
-    bf16x8 ah1[WM], al1[WM], bh1[WN], bl1[WN];
-    auto load = [&](int kb, bf16x8 (&ah)[WM], bf16x8 (&al)[WM], bf16x8 (&bh)[WN], bf16x8 (&bl)[WN]) {
-#pragma unroll
-        for (int i = 0; i < WM; ++i) {
-            ah[i] = Ap[i * rstride + kb * 128];
-            al[i] = Ap[i * rstride + kb * 128 + 64];
-        }
-#pragma unroll
-        for (int j = 0; j < WN; ++j) {
-            bh[j] = Bp[j * rstride + kb * 128];
-            bl[j] = Bp[j * rstride + kb * 128 + 64];
-        }
-    };
-    auto mul = [&](const bf16x8 (&ah)[WM], const bf16x8 (&al)[WM], const bf16x8 (&bh)[WN],
-                   const bf16x8 (&bl)[WN]) {
-#pragma unroll
-        for (int i = 0; i < WM; ++i)
-#pragma unroll
-            for (int j = 0; j < WN; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            }
-    };
-    const int KB = g.KB;
-    load(0, ah0, al0, bh0, bl0);
-    int kb = 0;
-    for (; kb + 2 <= KB; kb += 2) {
-        load(kb + 1, ah1, al1, bh1, bl1);
-        mul(ah0, al0, bh0, bl0);
-        if (kb + 2 < KB) load(kb + 2, ah0, al0, bh0, bl0);
-        mul(ah1, al1, bh1, bl1);
-    }
-    if (kb < KB) mul(ah0, al0, bh0, bl0);
-
-    store_neg_c<WM, WN>(g, r0, c0, lane, full, acc);
-}
-
-// LDS-staged variant: per k block a workgroup's unique fragments (A: its
+// Product kernel.  Per k block a workgroup's unique fragments (A: its
 // 2 WM row blocks, B: its 2 WN column blocks, hi and lo of each) are loaded
 // once, a quarter by each wave, into a two-deep register ring (the loads for
 // k blocks t + 2 and t + 3 are in flight while block t is multiplied), then
@@ -306,7 +252,7 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / 2, wn = wave % 2;
     int tm, tn;
-    bf3_tile(g, (int)blockIdx.x, tm, tn);
+    if (!bf3_tile(g, (int)blockIdx.x, WM / WN, tm, tn)) return;
     const int64_t rbW = (int64_t)tm * 2 * WM, cbW = (int64_t)tn * 2 * WN;   // workgroup's first row / column block
     const int64_t r0 = (rbW + wm * WM) * FE, c0 = (cbW + wn * WN) * FE;
     const bool full = r0 + WM * FE <= g.M && c0 + WN * FE <= g.N;
@@ -387,8 +333,12 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
     store_neg_c<WM, WN>(g, r0, c0, lane, full, acc);
 }
 
-// Packed operands are padded to 256 rows / columns, whole tiles of every
-// variant below.
+// Wave tile 128 x 64 (WM = 4, WN = 2 blocks of 32 x 32), workgroup tile
+// 256 x 128.  Measured at M = N = 24,064, K = 512 on MI355X: 2.2 ms (0.35 of the
+// bf16 MFMA peak with C's read and write); 128 x 128 wave tiles spill, and
+// fragments read straight from global memory without the LDS ring reached
+// 0.29.  Packed operands are padded to 256 rows / columns.
+constexpr int BF3_WM = 4, BF3_WN = 2;
 constexpr int PAD = 256;
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -416,11 +366,7 @@ int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, 
     FL_CHECK(ws && ws_bytes >= split3_workspace_bytes(M, N, K), "split3_update needs %zu workspace bytes",
              split3_workspace_bytes(M, N, K));
     const int KB = (int)cdiv(K, FK);
-    static const int variant = [] {
-        const char* e = std::getenv("FL_BF3_VARIANT");
-        return e && *e ? std::atoi(e) : 1;
-    }();
-    const int wm = variant == 3 ? 2 : 4, wn = (variant == 2 || variant == 3) ? 4 : 2;
+    constexpr int wm = BF3_WM, wn = BF3_WN;
     const int64_t tiles_m = cdiv(M, 2 * wm * FE), tiles_n = cdiv(N, 2 * wn * FE);
     const int64_t MBp = cdiv(M, PAD) * (PAD / FE), NBp = cdiv(N, PAD) * (PAD / FE);
     uint16_t* Apk = static_cast<uint16_t*>(ws);
@@ -442,15 +388,12 @@ int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, 
     g.tiles_m = (int)tiles_m;
     g.tiles_n = (int)tiles_n;
     g.lower = lower;
-    g.ntiles = lower ? (int)(tiles_m * (tiles_m + 1) / 2) : (int)(tiles_m * tiles_n);
-    if (variant == 0)
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<4, 2>), dim3((unsigned)g.ntiles), dim3(WGT), 0, st, g);
-    else if (variant == 2)
-        hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<4, 4>), dim3((unsigned)g.ntiles), dim3(WGT), 0, st, g);
-    else if (variant == 3)
-        hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<2, 4>), dim3((unsigned)g.ntiles), dim3(WGT), 0, st, g);
-    else
-        hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<4, 2>), dim3((unsigned)g.ntiles), dim3(WGT), 0, st, g);
+    // lower: tile row t holds q (t + 1) tiles (q = wm / wn), the last row's
+    // beyond column tiles_n - 1 left to the waves' own bounds checks
+    const int q = wm / wn;
+    g.ntiles = lower ? (int)(q * tiles_m * (tiles_m + 1) / 2) : (int)(tiles_m * tiles_n);
+    const unsigned launch = (unsigned)(lower ? (g.ntiles + NXCD - 1) / NXCD * NXCD : g.ntiles);
+    hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<BF3_WM, BF3_WN>), dim3(launch), dim3(WGT), 0, st, g);
     FL_LAUNCHED();
     return FL_OK;
 }

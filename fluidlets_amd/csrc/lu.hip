@@ -44,6 +44,10 @@ int gemm_pair_launch(int64_t M1, int64_t N1, int64_t K1, T alpha1, const T* A1, 
                      int64_t K2, T alpha2, const T* A2, int64_t lda2, const T* B2, int64_t ldb2, T beta2, T* C2,
                      int64_t ldc2, hipStream_t stream);
 
+template <typename T>
+int gemm_lower_update(int64_t n, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
+                      hipStream_t stream);
+
 // split-bf16 trailing update (gemm_bf16x3.hip)
 size_t split3_workspace_bytes(int64_t M, int64_t N, int64_t K);
 int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
@@ -1332,15 +1336,50 @@ __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int
 
 size_t split_ws_bytes(int64_t n) { return align_up(split3_workspace_bytes(n, n, NBO_SPLIT), 256); }
 
+// With `lower` (M == N; the symmetric factorization) only C's tiles on or
+// below the diagonal are updated.
 template <typename T>
 int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
-                    void* split, size_t split_bytes, hipStream_t st)
+                    void* split, size_t split_bytes, hipStream_t st, bool lower = false)
 {
     if constexpr (std::is_same_v<T, float>) {
         if (split && M >= 2048 && N >= 2048 && K <= NBO_SPLIT)
-            return split3_update(M, N, K, A, lda, B, ldb, C, ldc, 0, split, split_bytes, st);
+            return split3_update(M, N, K, A, lda, B, ldb, C, ldc, lower ? 1 : 0, split, split_bytes, st);
     }
+    if (lower) return gemm_lower_update<T>(M, K, A, lda, B, ldb, C, ldc, st);
     return gemm_launch<T>(M, N, K, T(-1), A, lda, B, ldb, T(1), C, ldc, nullptr, st);
+}
+
+// Symmetric factorization: U = D L^T.  Rows [r0, r0 + h) of U, columns
+// [c0, c1): U[r][c] = U[r][r] L[c][r], the row block formed from the
+// already-final columns of L below it (an LDS-tiled transpose: 32 x 32 tiles,
+// coalesced reads of L's rows and writes of U's rows).
+template <typename T>
+__global__ __launch_bounds__(256) void sym_u_kernel(T* A, int64_t lda, int64_t r0, int h, int64_t c0, int64_t c1)
+{
+    __shared__ T t[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    const int64_t cb = c0 + (int64_t)blockIdx.x * 32, rb = r0 + (int64_t)blockIdx.y * 32;
+    // read L[cb + i][rb + tx] for i = ty, ty + 8, ...
+    for (int i = ty; i < 32; i += 8) {
+        const int64_t c = cb + i, r = rb + tx;
+        t[i][tx] = (c < c1 && r < r0 + h) ? A[c * lda + r] : T(0);
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int64_t r = rb + i, c = cb + tx;
+        if (r < r0 + h && c < c1) A[r * lda + c] = A[r * lda + r] * t[tx][i];
+    }
+}
+
+template <typename T>
+int sym_u_launch(T* A, int64_t lda, int64_t r0, int h, int64_t c0, int64_t c1, hipStream_t st)
+{
+    if (c1 <= c0 || h <= 0) return FL_OK;
+    hipLaunchKernelGGL(sym_u_kernel<T>, dim3((unsigned)div_up(c1 - c0, 32), (unsigned)div_up(h, 32)), dim3(256), 0, st,
+                       A, lda, r0, h, c0, c1);
+    FL_LAUNCHED();
+    return FL_OK;
 }
 
 // Workspace: the sub-panel backup (n x NB), flags, and for FP32 the two
@@ -1373,11 +1412,22 @@ __global__ void spec_finish_kernel(const unsigned int* lmax_bits, unsigned int* 
 // whole matrix is folded into one device word, and nothing is read back: the
 // call only enqueues work, and *status says afterwards whether the result is
 // valid (else A must be refactored from a copy on the verified path).
+//
+// With `sym` (asynchronous path only) A is taken as symmetric and factored
+// as A = L D L^T without interchanges: only the lower triangle is read below
+// the diagonal blocks, each row block of U is formed as D L^T from the
+// finished columns of L (no U12 GEMMs), and trailing updates cover only the
+// tiles on or below the diagonal, which halves the bulk of the work.  The
+// factors are stored as fl_?getrf_async stores them (L below the diagonal,
+// U = D L^T on and above it, the same aux), so the solves and the refinement
+// take them unchanged; status is checked the same way.
 template <typename T>
 int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info_host, int* fallback_panels,
-               void* ws, size_t ws_bytes, hipStream_t st, unsigned int* status = nullptr, bool split_products = false)
+               void* ws, size_t ws_bytes, hipStream_t st, unsigned int* status = nullptr, bool split_products = false,
+               bool sym = false)
 {
     const bool spec = status != nullptr;
+    FL_CHECK(!sym || spec, "the symmetric factorization is asynchronous only");
     FL_CHECK(n >= 0, "negative n");
     FL_CHECK(lda >= n, "lda < n");
     if (n == 0) {
@@ -1432,7 +1482,15 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                                    invU);
                 FL_LAUNCHED();
                 const int64_t rest0 = pe - ks - nb;
-                if (m > nb && rest0 > 0) {
+                if (sym && m > nb) {
+                    // L21 = A21 inv(U11) (the lower part is up to date)
+                    if (int s = gemm_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0),
+                                               Akk + nb * lda, lda, flag, st))
+                        return s;
+                    // the in-panel U12 = D11 L21^T
+                    if (int s = sym_u_launch<T>(A, lda, ks, nb, ks + nb, pe, st)) return s;
+                    u12_done = true;
+                } else if (m > nb && rest0 > 0) {
                     // L21 = A21 inv(U11) and U12 = inv(L11) A12 (in-panel), disjoint, in one launch
                     if (int s = gemm_pair_launch<T>(m - nb, nb, nb, T(1), Akk + nb * lda, lda, invU, NB, T(0),
                                                     Akk + nb * lda, lda, flag, nb, rest0, nb, T(1), invL, NB, Akk + nb,
@@ -1520,7 +1578,13 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         if (int s = swap_cols(k, nbo, 0, k)) return s;
         if (int s = swap_cols(k, nbo, pe, n)) return s;
         const int64_t nr = n - pe;
-        if (nr > 0) {
+        if (nr > 0 && sym) {
+            // U12 = D L21^T for the whole outer panel, then the lower half of the update
+            if (int s = sym_u_launch<T>(A, lda, k, nbo, pe, n, st)) return s;
+            if (int s = trailing_update<T>(nr, nr, nbo, A + pe * lda + k, lda, A + k * lda + pe, lda, A + pe * lda + pe,
+                                           lda, split, split ? split_ws_bytes(n) : 0, st, true))
+                return s;
+        } else if (nr > 0) {
             // U12 = inv(L11) A12 block row by block row
             for (int64_t ks = k; ks < pe; ks += NB) {
                 const int nb = (int)std::min<int64_t>(NB, pe - ks);
@@ -1942,6 +2006,20 @@ int dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, un
 {
     FL_CHECK(status, "null status");
     return getrf_impl<double>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status);
+}
+int ssytrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, int products,
+                 void* ws, size_t wsb, hipStream_t st)
+{
+    FL_CHECK(status, "null status");
+    FL_CHECK(products == FL_LU_FP32 || products == FL_LU_BF16X3, "unknown products %d", products);
+    return getrf_impl<float>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status, products == FL_LU_BF16X3,
+                             true);
+}
+int dsytrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, void* ws,
+                 size_t wsb, hipStream_t st)
+{
+    FL_CHECK(status, "null status");
+    return getrf_impl<double>(A, n, lda, ipiv, aux, nullptr, nullptr, ws, wsb, st, status, false, true);
 }
 int sgetrs(const float* LU, int64_t n, int64_t lda, const void* aux, float* B, int64_t nrhs, int64_t ldb, void* ws,
            size_t wsb, hipStream_t st)

@@ -38,7 +38,7 @@ class LUFactor:
 
 
 def getrf(A, overwrite: bool = True, stream=None, refill=None, hint: dict | None = None,
-          products: str = "fp32") -> LUFactor:
+          products: str = "fp32", sym: bool = False) -> LUFactor:
     """Factor an (n, n) float32/float64 device tensor in place (or a copy).
 
     ``products`` ("fp32" or "bf16x3", float32 only) selects the arithmetic of
@@ -53,7 +53,12 @@ def getrf(A, overwrite: bool = True, stream=None, refill=None, hint: dict | None
     that is not partial pivoting's answer, A is refilled and factored on the
     verified path.  The result is the same either way.  ``hint`` (a dict
     shared by a sequence of similar factorizations) records a failed attempt,
-    after which the sequence goes straight to the verified path."""
+    after which the sequence goes straight to the verified path.
+
+    ``sym`` (with ``refill``): A is symmetric (a kernel matrix with one blob
+    width) and the first attempt is the symmetric factorization A = L D L^T
+    (fl_?sytrf_async: half the flops, the same factor format); a matrix whose
+    multipliers exceed 1 is refilled and factored on the verified LU path."""
     t = D.torch()
     if A.dim() != 2 or A.shape[0] != A.shape[1]:
         raise ValueError("getrf needs a square matrix")
@@ -67,6 +72,8 @@ def getrf(A, overwrite: bool = True, stream=None, refill=None, hint: dict | None
     prec = FL_FP64 if A.dtype == t.float64 else FL_FP32C
     if products not in ("fp32", "bf16x3"):
         raise ValueError(f"unknown products {products!r}")
+    if sym and refill is None:
+        raise ValueError("the symmetric factorization needs refill (its fallback is the verified LU)")
     split = FL_LU_BF16X3 if products == "bf16x3" and prec == FL_FP32C else FL_LU_FP32
     L = lib()
     ipiv = t.empty((max(n, 1),), dtype=t.int32, device=A.device)
@@ -75,16 +82,22 @@ def getrf(A, overwrite: bool = True, stream=None, refill=None, hint: dict | None
     ws = D.workspace(nbytes, stream)
     if refill is not None and (hint is None or hint.get("async", True)):
         status = t.empty((1,), dtype=t.int32, device=A.device)
-        if prec == FL_FP64:
+        if prec == FL_FP64 and sym:
+            check(L.fl_dsytrf_async(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), D.ptr(status), D.ptr(ws), nbytes,
+                                    D.stream_handle(stream)), "sytrf_async")
+        elif prec == FL_FP64:
             check(L.fl_dgetrf_async(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), D.ptr(status), D.ptr(ws), nbytes,
                                     D.stream_handle(stream)), "getrf_async")
+        elif sym:
+            check(L.fl_ssytrf_async(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), D.ptr(status), split, D.ptr(ws),
+                                    nbytes, D.stream_handle(stream)), "sytrf_async")
         else:
             check(L.fl_sgetrf_async_ex(D.ptr(A), n, n, D.ptr(ipiv), D.ptr(aux), D.ptr(status), split, D.ptr(ws),
                                        nbytes, D.stream_handle(stream)), "getrf_async")
         with t.cuda.stream(stream) if stream is not None else nullcontext():
             ok = int(status.cpu().item()) == 0     # waits for the factorization on its stream
             if ok:
-                return LUFactor(A, ipiv, aux, n, 0, 0, {"async": True, "products": products})
+                return LUFactor(A, ipiv, aux, n, 0, 0, {"async": True, "products": products, "sym": bool(sym)})
             if hint is not None:
                 hint["async"] = False
             refill(A)

@@ -211,6 +211,24 @@ int fl_sgetrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, 
 int fl_dgetrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* Symmetric factorization A = L D L^T without interchanges, for the kernel
+ * matrices with one blob width (reciprocity makes them symmetric: G(y, x) =
+ * G(x, y)^T), asynchronous like fl_?getrf_async.  Only A's lower triangle
+ * (and the diagonal blocks) is read.  The factors are written in the format
+ * of fl_?getrf_async (unit L below the diagonal, U = D L^T on and above it,
+ * ipiv the identity, the same aux), so fl_?getrs, fl_refine and the other
+ * consumers take them unchanged.  The trailing updates cover only the lower
+ * triangle, half the flops of the LU.  *status as fl_?getrf_async (0: every
+ * multiplier within 1 + 2^-20; else refill A and use fl_?getrf).  products
+ * as fl_sgetrf_ex.  Same workspace as fl_?getrf.  Replaces the dense solve
+ * of src/fluidlets/core/solve.py:66 (scipy.linalg.solve, whose
+ * assume_a="sym" / "pos" forms are LAPACK's symmetric factorizations) for
+ * these matrices. */
+int fl_ssytrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status, int products,
+                    void* workspace, size_t workspace_bytes, void* stream);
+int fl_dsytrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
 /* Explicit inverse from the factors of fl_sgetrf: Ainv = A^{-1} =
  * U^{-1} L^{-1} P (n x n row-major, ldi >= n), by blocked substitution on the
  * permuted identity with the stored inverted diagonal blocks and the LU's

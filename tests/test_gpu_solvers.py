@@ -416,3 +416,69 @@ def test_mfma_gemm_matches_numpy(dtype, M, N, K):
     assert st == 0
     tol = 1e-5 if dtype == "float32" else 1e-14
     assert rel_err(dC.cpu().numpy().astype(np.float64), ref) < tol
+
+
+@pytest.mark.parametrize("n_points,kernel,products", [
+    (300, "stokeslet", "fp32"),          # n = 900: every update on the FP32 lower-tile GEMM
+    (1200, "brinkman:3.0", "fp32"),      # n = 3,600: FP32 products, lower tiles of 128
+    (1200, "brinkman:3.0", "bf16x3"),    # the same with split-bf16 lower-tile updates
+    (1100, "wall", "bf16x3"),            # symmetric by reciprocity only to rounding
+    (1200, "stokeslet", "fp64"),
+])
+def test_symmetric_factorization_matches_lapack(n_points, kernel, products):
+    """The symmetric factorization A = L D L^T (fl_?sytrf_async) of a kernel
+    matrix with one blob width, in the LU factor format: the factors satisfy
+    L U = A with U = D L^T to the arithmetic's accuracy, and the solves
+    (refined against the FP64 matrix for FP32 factors) equal LAPACK's."""
+    from fluidlets_amd import linalg
+    import torch
+    print(f"case: sytrf {kernel} N={n_points} {products}", flush=True)
+    X = fibonacci_sphere(n_points, 1.0, (0, 0, 1.5) if kernel == "wall" else (0, 0, 0))
+    eps = 0.5 * np.sqrt(4 * np.pi / n_points)
+    K = O.assemble(kernel, X, X, eps)
+    n = 3 * n_points
+    if kernel != "wall":
+        assert np.array_equal(K, K.T)        # reciprocity, bit for bit without images
+    b = np.random.default_rng(n).standard_normal((n, 6))
+    xref = sl.solve(K, b)
+    K64 = dev(K)
+    fp64 = products == "fp64"
+    A = K64.clone() if fp64 else K64.to(torch.float32)
+    src = K64 if fp64 else K64.to(torch.float32)
+    F = linalg.getrf(A, refill=lambda T: T.copy_(src), sym=True, products="fp32" if fp64 else products)
+    assert F.extra.get("sym") and F.extra.get("async") and F.info == 0
+    LU = F.lu.cpu().numpy().astype(np.float64)
+    L, U = np.tril(LU, -1) + np.eye(n), np.triu(LU)
+    unit = 2.0**-52 if fp64 else (2.0**-16 if products == "bf16x3" else 2.0**-23)
+    # U = D L^T: the transpose-scale formed the off-diagonal blocks; the
+    # diagonal blocks come from the LU of the block, equal up to rounding
+    d = np.diag(U)
+    assert np.abs(U - np.triu(d[:, None] * L.T)).max() <= 200 * unit * np.abs(U).max()
+    assert np.linalg.norm(L @ U - K) / np.linalg.norm(K) < 200 * unit
+    if fp64:
+        x = dev(b)
+        linalg.getrs(F, x)
+        assert rel_err(x.cpu().numpy(), xref) < 1e-12
+    else:
+        x, steps, res = linalg.refine(K64, F, dev(b), tol=1e-12)
+        assert res.max() <= 1e-12 and steps <= 6
+        assert rel_err(x.cpu().numpy(), xref) < 1e-10
+
+
+def test_symmetric_factorization_falls_back_to_pivoting():
+    """A symmetric matrix whose elimination without interchanges has
+    multipliers above 1 (a large off-diagonal entry): the symmetric attempt
+    reports failure and getrf refills and factors it on the verified
+    pivoting path, equal to LAPACK."""
+    from fluidlets_amd import linalg
+    print("case: sytrf fallback", flush=True)
+    rng = np.random.default_rng(9)
+    n = 400
+    G = rng.standard_normal((n, n))
+    S = G + G.T
+    x = dev(S)
+    F = linalg.getrf(x.clone(), refill=lambda T: T.copy_(x), sym=True)
+    assert not F.extra.get("async")
+    lu, piv = sl.lu_factor(S)
+    assert np.array_equal(F.ipiv.cpu().numpy(), piv)
+    assert np.allclose(F.lu.cpu().numpy(), lu, rtol=0, atol=1e-11 * np.abs(lu).max())
