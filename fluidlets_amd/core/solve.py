@@ -50,26 +50,38 @@ def _from_columns(x: np.ndarray, dtype, mu):
     return (f * mu).astype(dtype) if np.issubdtype(dtype, np.complexfloating) else f * float(np.real(mu))
 
 
-def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-10, max_iter=30, stream=None):
+def symmetric_matrix(eps_stride: int) -> bool:
+    """True when the kernel matrix is symmetric: every kernel here is
+    reciprocal (G(y, x) = G(x, y)^T), so the matrix of one point set with one
+    blob width is symmetric (per-source widths break it)."""
+    return int(eps_stride) == 0
+
+
+def lu_solve_device(kernel, x_soa, eps_t, stride, rhs, precision="fp64", tol=1e-10, max_iter=30, stream=None,
+                    sym: bool | None = None):
     """Dense solve of K x = rhs (K assembled for mu = 1) on the device.
     rhs: (3N, q) float64 device tensor.  The system is factored at its order
     rounded up to a multiple of 128, identity-padded (assemble_padded_device:
-    the same solution, a faster factorization).  Returns (x (3N, q),
-    refinement_steps, factor of the padded system)."""
+    the same solution, a faster factorization).  A symmetric K (one blob
+    width; ``sym`` None decides by symmetric_matrix) is factored as
+    L D L^T (half the flops), with the verified LU as its fallback.  Returns
+    (x (3N, q), refinement_steps, factor of the padded system)."""
     n = int(rhs.shape[0])
+    if sym is None:
+        sym = symmetric_matrix(stride)
     if precision == "fp64":
         def assemble64():
             return assemble_padded_device(kernel, x_soa, eps_t, stride, rhs, fp32=False, stream=stream)
 
         K64, _, b, _ = assemble64()
-        F = linalg.getrf(K64, stream=stream, refill=lambda A: A.copy_(assemble64()[0]))
+        F = linalg.getrf(K64, stream=stream, refill=lambda A: A.copy_(assemble64()[0]), sym=sym)
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
         linalg.getrs(F, b, stream=stream)
         return b[:n], 0, F
     if precision == "mixed":
         K64, K32, b, _ = assemble_padded_device(kernel, x_soa, eps_t, stride, rhs, fp32=True, stream=stream)
-        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3")
+        F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3", sym=sym)
         if F.info > 0:
             raise np.linalg.LinAlgError("Matrix is singular.")
         x, steps, _ = linalg.refine(K64, F, b, tol=tol, max_iter=max_iter, stream=stream)

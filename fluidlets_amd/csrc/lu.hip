@@ -155,6 +155,11 @@ __device__ __forceinline__ double dpp_mov(double x)
     return __hiloint2double(hi, lo);
 }
 
+// 1 / x: the hardware reciprocal in FP32 (1 ulp; the FP32 factors only
+// precondition FP64 refinement), the correctly rounded division in FP64.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 __device__ __forceinline__ float read_lane(float x, int l)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
@@ -644,6 +649,311 @@ __global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int 
         A[r * lda + c] = S[r * LS + c];
     }
     block_inverses<T>(S, invL, invU, tid);
+}
+
+// ---------------------------------------------------------------- diagonal block, products on MFMA
+// The same factorization without interchanges as diag_nopiv_kernel, and
+// the same outputs (factors, identity ipiv, largest |multiplier| folded into
+// *lmax_bits, inv(L11) and inv(U11)), with every product on MFMA across the
+// four waves.  The 128 x 128 block is a 4 x 4 grid of 32 x 32 sub-blocks;
+// for kb = 0 .. 3:
+//   leaf    wave 0 factors sub-block (kb, kb), one row per lane, the pivot
+//           row broadcast with readlane; then wave 0 inverts its unit lower
+//           triangle and wave 1 its upper triangle (one column per lane, by
+//           substitution), IL_kb and IU_kb;
+//   panels  L_ib = A_ib IU_kb (i > kb) and U_kj = IL_kb A_kj (j > kb), one
+//           32 x 32 x 32 MFMA product per wave;
+//   update  A_ij -= L_ik U_kj for i, j > kb.
+// The off-diagonal blocks of inv(L) and inv(U) then follow from products
+// only: with M_im = -IL_i L_im (i > m) and N_mj = -U_mj IU_j (m < j),
+//   inv(L)_ij = sum_{m = j}^{i - 1} M_im inv(L)_mj,  inv(U)_ij = sum_{m = i}^{j - 1} inv(U)_im N_mj,
+// block diagonals of inv(L) and inv(U) being IL_i and IU_i, level by level in
+// the distance from the diagonal (a barrier between levels).
+//
+// One wave's 32 x 32 MFMA accumulator (FP32: one v_mfma_f32_32x32x2_f32
+// block; FP64: 2 x 2 v_mfma_f64_16x16x4_f64 blocks).
+template <typename T>
+struct Acc32;
+
+template <>
+struct Acc32<float> {
+    mfma::floatx16 c;
+    static __device__ int row(int lane, int i) { return 8 * (i / 4) + 4 * (lane / 32) + (i % 4); }
+    static __device__ int col(int lane) { return lane & 31; }
+    __device__ void zero() { c = mfma::floatx16{0}; }
+    // c += s A B over K = 32: A (32 x 32, pitch la), B (32 x 32, pitch lb)
+    __device__ void mac(const float* A, int la, const float* B, int lb, float s, int lane)
+    {
+        const int r = lane & 31, kk = lane >> 5;
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 2)
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(s * A[r * la + k0 + kk], B[(k0 + kk) * lb + r], c, 0, 0, 0);
+    }
+    __device__ void load(const float* P, int lp, int lane)
+    {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) c[i] = P[row(lane, i) * lp + col(lane)];
+    }
+    __device__ void store(float* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[row(lane, i) * lp + col(lane)] = c[i];
+    }
+    __device__ float absmax() const
+    {
+        float m = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m = fmaxf(m, fabsf(c[i]));
+        return m;
+    }
+};
+
+template <>
+struct Acc32<double> {
+    mfma::doublex4 c[2][2];
+    static __device__ int row(int lane, int b, int i) { return 16 * b + 4 * i + lane / 16; }
+    static __device__ int col(int lane, int b) { return 16 * b + (lane & 15); }
+    __device__ void zero()
+    {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) c[a][b] = mfma::doublex4{0};
+    }
+    __device__ void mac(const double* A, int la, const double* B, int lb, double s, int lane)
+    {
+        const int r = lane & 15, kk = lane >> 4;
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 4) {
+            const double a0 = s * A[r * la + k0 + kk], a1 = s * A[(16 + r) * la + k0 + kk];
+            const double b0 = B[(k0 + kk) * lb + r], b1 = B[(k0 + kk) * lb + 16 + r];
+            c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c[0][0], 0, 0, 0);
+            c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c[0][1], 0, 0, 0);
+            c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c[1][0], 0, 0, 0);
+            c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c[1][1], 0, 0, 0);
+        }
+    }
+    __device__ void load(const double* P, int lp, int lane)
+    {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) c[a][b][i] = P[row(lane, a, i) * lp + col(lane, b)];
+    }
+    __device__ void store(double* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) P[row(lane, a, i) * lp + col(lane, b)] = c[a][b][i];
+    }
+    __device__ double absmax() const
+    {
+        double m = 0.0;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) m = fmax(m, fabs(c[a][b][i]));
+        return m;
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int nb, int64_t k0, int32_t* ipiv,
+                                                         unsigned int* lmax_bits, T* invL, T* invU)
+{
+    constexpr int LS = NB + 1, LB = SUBB + 1, NBB = NB / SUBB;
+    // FP32 keeps every leaf inverse and the assembled inverse blocks in LDS
+    // (IL_k, IU_k, then the six off-diagonal blocks of each inverse); FP64's
+    // block fills most of LDS, so its leaf inverses go through LDS one at a
+    // time and the assembly reads and writes invL and invU in global memory.
+    constexpr bool KEEP = sizeof(T) == 4;
+    constexpr int NIL = KEEP ? NBB : 1, NX = KEEP ? 6 : 1, XSZ = KEEP ? SUBB * LB : 1;
+    __shared__ T S[NB * LS];
+    __shared__ T ILs[NIL][SUBB * LB], IUs[NIL][SUBB * LB];
+    __shared__ T XLs[NX][XSZ], YUs[NX][XSZ];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    load_diag_block<T>(A, lda, nb, S, tid);
+    for (int c = tid; c < nb; c += 256) ipiv[k0 + c] = (int32_t)(k0 + c);
+    __syncthreads();
+    T lmax = T(0);
+    for (int kb = 0; kb < NBB; ++kb) {
+        const int o = kb * SUBB;
+        T* IL = ILs[KEEP ? kb : 0];
+        T* IU = IUs[KEEP ? kb : 0];
+        if (wave == 0) {
+            // leaf LU: rows o .. o + 31 by lanes 0 .. 31
+            T a[SUBB];
+            const int r = o + (lane & (SUBB - 1));
+#pragma unroll
+            for (int j = 0; j < SUBB; ++j) a[j] = S[r * LS + o + j];
+#pragma unroll
+            for (int c = 0; c < SUBB; ++c) {
+                const T pv = read_lane(a[c], c);
+                if (pv == T(0)) lmax = T(INFINITY);  // a zero pivot: leave it to the pivoting tiers
+                const T rp = pv != T(0) ? recip(pv) : T(1);
+                const bool below = (lane & (SUBB - 1)) > c && lane < SUBB;
+                const T l = below ? a[c] * rp : T(0);
+                if (below) {
+                    a[c] = l;
+                    lmax = fmax(lmax, l < T(0) ? -l : l);
+                }
+#pragma unroll
+                for (int j = c + 1; j < SUBB; ++j) a[j] = fma(-l, read_lane(a[j], c), a[j]);
+            }
+            if (lane < SUBB)
+#pragma unroll
+                for (int j = 0; j < SUBB; ++j) S[r * LS + o + j] = a[j];
+        }
+        __syncthreads();
+        const T* F = S + o * LS + o;
+        if (wave == 0 && lane < SUBB) {
+            // column `lane` of IL = inv(L_kk), unit lower
+            const int j = lane;
+            T x[SUBB];
+#pragma unroll
+            for (int i = 0; i < SUBB; ++i) {
+                T acc = T(0);
+#pragma unroll
+                for (int m = 0; m < i; ++m) acc = fma(F[i * LS + m], x[m], acc);
+                x[i] = i < j ? T(0) : (i == j ? T(1) : -acc);
+            }
+#pragma unroll
+            for (int i = 0; i < SUBB; ++i) {
+                IL[i * LB + j] = x[i];
+                if (!KEEP) invL[(o + i) * NB + o + j] = x[i];
+            }
+        } else if (wave == 1 && lane < SUBB) {
+            // column `lane` of IU = inv(U_kk); y[q] holds row SUBB - 1 - q
+            const int j = lane;
+            T y[SUBB];
+#pragma unroll
+            for (int q = 0; q < SUBB; ++q) {
+                const int i = SUBB - 1 - q;
+                T acc = T(0);
+#pragma unroll
+                for (int m = 0; m < q; ++m) acc = fma(F[i * LS + SUBB - 1 - m], y[m], acc);
+                const T rd = recip(F[i * LS + i]);
+                y[q] = i > j ? T(0) : (i == j ? rd : -acc * rd);
+            }
+#pragma unroll
+            for (int q = 0; q < SUBB; ++q) {
+                const int i = SUBB - 1 - q;
+                IU[i * LB + j] = y[q];
+                if (!KEEP) invU[(o + i) * NB + o + j] = y[q];
+            }
+        }
+        __syncthreads();
+        const int rest = NBB - 1 - kb;
+        // panels: L_ib = A_ib IU (p < rest), U_kj = IL A_kj (p >= rest)
+        for (int p = wave; p < 2 * rest; p += 4) {
+            Acc32<T> acc;
+            acc.zero();
+            if (p < rest) {
+                T* blk = S + (kb + 1 + p) * SUBB * LS + o;
+                acc.mac(blk, LS, IU, LB, T(1), lane);
+                acc.store(blk, LS, lane);
+                lmax = fmax(lmax, acc.absmax());
+            } else {
+                T* blk = S + o * LS + (kb + 1 + p - rest) * SUBB;
+                acc.mac(IL, LB, blk, LS, T(1), lane);
+                acc.store(blk, LS, lane);
+            }
+        }
+        __syncthreads();
+        // trailing sub-blocks: A_ij -= L_ik U_kj
+        for (int p = wave; p < rest * rest; p += 4) {
+            const int ib = kb + 1 + p / rest, jb = kb + 1 + p % rest;
+            T* Cb = S + ib * SUBB * LS + jb * SUBB;
+            Acc32<T> acc;
+            acc.load(Cb, LS, lane);
+            acc.mac(S + ib * SUBB * LS + o, LS, S + o * LS + jb * SUBB, LS, T(-1), lane);
+            acc.store(Cb, LS, lane);
+        }
+        __syncthreads();
+    }
+    // largest multiplier of the block -> *lmax_bits; the factors -> A
+    const T wl = wave_max(lmax);
+    if (lane == 0) atomicMax(lmax_bits, __float_as_uint((float)wl));
+    if (nb == NB) {
+        for (int r = tid / NB; r < NB; r += 256 / NB) A[r * lda + tid % NB] = S[r * LS + tid % NB];
+    } else {
+        for (int r = 0; r < nb; ++r)
+            for (int c = tid; c < nb; c += 256) A[r * lda + c] = S[r * LS + c];
+    }
+    // The off-diagonal blocks of the inverses: block (i, j), 0 <= j < i < 4,
+    // has index q(i, j) = i (i - 1) / 2 + j; X (inv(L)) in XLs / invL, Y
+    // (inv(U), block (j, i)) in YUs / invU.
+    auto qi = [](int i, int j) { return i * (i - 1) / 2 + j; };
+    auto ilp = [&](int i) -> const T* { return KEEP ? ILs[i] : invL + i * SUBB * NB + i * SUBB; };
+    auto iup = [&](int i) -> const T* { return KEEP ? IUs[i] : invU + i * SUBB * NB + i * SUBB; };
+    constexpr int LP = KEEP ? LB : NB;   // pitch of those blocks
+    auto xp = [&](int i, int j) -> T* { return KEEP ? XLs[qi(i, j)] : invL + i * SUBB * NB + j * SUBB; };
+    auto yp = [&](int i, int j) -> T* { return KEEP ? YUs[qi(j, i)] : invU + i * SUBB * NB + j * SUBB; };
+    __syncthreads();
+    // level 0: M_im = -IL_i L_im (i > m), N_mj = -U_mj IU_j (m < j), in place of L_im, U_mj
+    for (int p = wave; p < 12; p += 4) {
+        const int q = p % 6;
+        const int hi = q < 1 ? 1 : (q < 3 ? 2 : 3), lo = q < 1 ? 0 : (q < 3 ? q - 1 : q - 3);   // (1,0) (2,0) (2,1) (3,0) (3,1) (3,2)
+        Acc32<T> acc;
+        acc.zero();
+        if (p < 6) {
+            T* blk = S + hi * SUBB * LS + lo * SUBB;
+            acc.mac(ilp(hi), LP, blk, LS, T(-1), lane);
+            acc.store(blk, LS, lane);
+        } else {
+            T* blk = S + lo * SUBB * LS + hi * SUBB;
+            acc.mac(blk, LS, iup(hi), LP, T(-1), lane);
+            acc.store(blk, LS, lane);
+        }
+    }
+    __syncthreads();
+    // levels 1 .. 3: inv(L)_ij (i - j = d) and inv(U)_ji, one output block per wave
+    for (int d = 1; d < NBB; ++d) {
+        const int outs = NBB - d;
+        for (int p = wave; p < 2 * outs; p += 4) {
+            Acc32<T> acc;
+            acc.zero();
+            if (p < outs) {
+                const int j = p, i = p + d;
+                for (int m = j; m < i; ++m)
+                    acc.mac(S + i * SUBB * LS + m * SUBB, LS, m == j ? ilp(j) : xp(m, j), LP, T(1), lane);
+                acc.store(xp(i, j), LP, lane);
+            } else {
+                const int i = p - outs, j = i + d;
+                for (int m = i; m < j; ++m)
+                    acc.mac(m == i ? iup(i) : yp(i, m), LP, S + m * SUBB * LS + j * SUBB, LS, T(1), lane);
+                acc.store(yp(i, j), LP, lane);
+            }
+        }
+        __syncthreads();
+    }
+    // the inverses -> invL, invU, block by block (FP32 from LDS; FP64's
+    // nonzero blocks are already in place, only its zero blocks are written)
+    for (int blk = 0; blk < NBB * NBB; ++blk) {
+        const int bi = blk / NBB, bj = blk % NBB;
+        const T* srcL = nullptr;
+        const T* srcU = nullptr;
+        if (KEEP) {
+            srcL = bj > bi ? nullptr : (bj == bi ? ILs[bi] : XLs[qi(bi, bj)]);
+            srcU = bj < bi ? nullptr : (bj == bi ? IUs[bi] : YUs[qi(bj, bi)]);
+        } else if (bj <= bi && bj >= bi) {
+            continue;   // FP64 diagonal blocks: both inverses already written
+        }
+        T* dL = invL + bi * SUBB * NB + bj * SUBB;
+        T* dU = invU + bi * SUBB * NB + bj * SUBB;
+        for (int e = tid; e < SUBB * SUBB; e += 256) {
+            const int ri = e / SUBB, cj = e % SUBB;
+            if (KEEP || bj > bi) dL[ri * NB + cj] = srcL ? srcL[ri * LB + cj] : T(0);
+            if (KEEP || bj < bi) dU[ri * NB + cj] = srcU ? srcU[ri * LB + cj] : T(0);
+        }
+    }
 }
 
 // info = the first zero pivot: keeps an earlier one.
@@ -1466,7 +1776,12 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     };
     // Outer panels of nbo_w columns (the trailing update has K = nbo_w); each is
     // factored as sub-panels of NB columns.
-    const int64_t nbo_w = split ? NBO_SPLIT : NBO;
+    static const int64_t nbo_env = [] {
+        const char* e = std::getenv("FL_LU_PANEL");   // measurement switch: outer panel width without the split
+        const int v = e && *e ? std::atoi(e) : 0;
+        return (int64_t)((v == 256 || v == 512) ? v : 0);
+    }();
+    const int64_t nbo_w = split ? NBO_SPLIT : (nbo_env ? nbo_env : NBO);
     for (int64_t k = 0; k < n; k += nbo_w) {
         const int64_t pe = std::min<int64_t>(n, k + nbo_w);
         for (int64_t ks = k; ks < pe; ks += NB) {
@@ -1478,7 +1793,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             T* invU = FactorAux<T>::invU(aux, n, p);
             bool u12_done = false;   // the in-panel U12 already formed (grouped with L21)
             if (spec) {
-                hipLaunchKernelGGL(diag_nopiv_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag, invL,
+                hipLaunchKernelGGL(diag_block_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag, invL,
                                    invU);
                 FL_LAUNCHED();
                 const int64_t rest0 = pe - ks - nb;
@@ -1530,7 +1845,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                 }
                 FL_HIP(hipMemsetAsync(flag, 0, 2 * sizeof(unsigned int), st));  // flag and info_blk
                 if (tier == 0)
-                    hipLaunchKernelGGL(diag_nopiv_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag,
+                    hipLaunchKernelGGL(diag_block_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag,
                                        invL, invU);
                 else
                     hipLaunchKernelGGL(diag_lu_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, info_blk,

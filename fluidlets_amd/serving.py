@@ -148,7 +148,9 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     if item is None:
         K64, K32, b, _ = assemble_padded(kernel, X, eps, modes, fp32=(precision == "mixed"), stream=stream)
         if precision == "mixed":
-            F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3")
+            # one blob width: a symmetric matrix, factored as L D L^T
+            F = linalg.getrf(K32, stream=stream, refill=lambda A: A.copy_(K64), products="bf16x3",
+                             sym=np.ndim(eps) == 0 or np.size(eps) == 1)
         else:
             F = linalg.getrf(K64.clone(), stream=stream, refill=lambda A: A.copy_(K64))
         if F.info > 0:
