@@ -124,3 +124,74 @@ def helix_flagellum(n: int, t: float, amplitude: float = 0.2, wavelength: float 
     pts = np.c_[amplitude * np.sin(arg), np.zeros(n), direction * s] + np.asarray(base, float)
     vel = np.c_[-amplitude * omega * np.cos(arg), np.zeros(n), np.zeros(n)]
     return pts, vel
+
+
+def gs_biflagellate(variant: str = "paper", head_axes=(0.5, 0.6, 0.6), insertion_angle: float = 30.0,
+                    n_head: int = 4, n_head_quad: int = 10, n_flag: int = 40, n_flag_quad: int = 400,
+                    dt: float = 1e-5):
+    """shape_at(t) of the model biflagellate alga of Gallagher & Smith, Phys.
+    Rev. Fluids 3, 053101 (2018), Sec. V A, as the reference's example runs it
+    (examples/biflagellate_swimmer/biflagellate.py, input3d): lengths in
+    flagellar lengths, one beat every 2 pi.
+
+    Body: a rigid ellipsoid with semi-axes ``head_axes``, its force and
+    quadrature points the cube-sphere sets of 6 n_head^2 and 6 n_head_quad^2
+    points.  Flagella: unit length, mirror images leaving the body at
+    ``insertion_angle`` degrees from the forward (+x2) axis, their tangent
+    angle (Eqs. 29-30)
+        psi(s, t) = -2.5 s - (0.7 + 0.15 sin 2 pi s) cos(t - 2 pi s),
+    integrated along the arclength by the trapezoidal rule on 4001 points;
+    the chord from the base to s = 4/29 points forward.  ``variant``
+    "authors_code" takes the published code's shorter flagellum instead (the
+    arc 1/30 <= sigma <= 1, the chord to 5/30).  Force points are n_flag and
+    quadrature points n_flag_quad equispaced in s per flagellum; velocities
+    are central differences in t with step ``dt``."""
+    from .swimming import SwimmerShape
+
+    if variant not in ("paper", "authors_code"):
+        raise ValueError(f"unknown variant {variant!r}")
+    a1, a2, a3 = (float(v) for v in head_axes)
+    ang = np.deg2rad(insertion_angle)
+    sf = np.linspace(0.0, 1.0, 4001)
+    ds = np.diff(sf)
+
+    def plane(s, t):
+        if variant == "authors_code":
+            s = 1.0 / 30 + np.asarray(s) * 29.0 / 30
+            s_ref, base = 5.0 / 30, 1.0 / 30
+        else:
+            s_ref, base = 4.0 / 29, 0.0
+        psi = -2.5 * sf - (0.7 + 0.15 * np.sin(2 * np.pi * sf)) * np.cos(t - 2 * np.pi * sf)
+        cx = np.concatenate([[0.0], np.cumsum(0.5 * (np.cos(psi[1:]) + np.cos(psi[:-1])) * ds)])
+        cy = np.concatenate([[0.0], np.cumsum(0.5 * (np.sin(psi[1:]) + np.sin(psi[:-1])) * ds)])
+        cx = cx - np.interp(base, sf, cx)
+        cy = cy - np.interp(base, sf, cy)
+        rot = np.pi / 2 - np.arctan2(np.interp(s_ref, sf, cy), np.interp(s_ref, sf, cx))
+        X = np.cos(rot) * cx - np.sin(rot) * cy
+        Y = np.sin(rot) * cx + np.cos(rot) * cy
+        return np.interp(s, sf, X), np.interp(s, sf, Y)
+
+    def rot_z(P, th):
+        c, s_ = np.cos(th), np.sin(th)
+        return P @ np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]]).T
+
+    def flagella(s, t):
+        X, Y = plane(s, t)
+        z = np.zeros_like(X)
+        cis = rot_z(np.stack([X, Y, z], axis=1), -ang) + [a1 * np.sin(ang), a2 * np.cos(ang), 0.0]
+        trans = rot_z(np.stack([-X, Y, z], axis=1), ang) + [-a1 * np.sin(ang), a2 * np.cos(ang), 0.0]
+        return np.vstack([cis, trans])
+
+    axes = np.array([a1, a2, a3])
+    head = cube_sphere(n_head) * axes
+    head_quad = cube_sphere(n_head_quad) * axes
+    s_force = np.linspace(0.0, 1.0, n_flag)
+    s_quad = np.linspace(0.0, 1.0, n_flag_quad)
+
+    def shape_at(t):
+        flag = flagella(s_force, t)
+        vel = (flagella(s_force, t + dt) - flagella(s_force, t - dt)) / (2 * dt)
+        return SwimmerShape(x=np.vstack([head, flag]), v=np.vstack([np.zeros_like(head), vel]),
+                            X=np.vstack([head_quad, flagella(s_quad, t)]))
+
+    return shape_at

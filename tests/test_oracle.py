@@ -348,3 +348,30 @@ def test_grand_resistance_of_a_sphere():
     off = R - np.diag(np.diag(R))
     assert np.abs(off).max() < 2e-2 * np.abs(R).max()
     assert np.abs(R - R.T).max() < 2e-2 * np.abs(R).max()
+
+
+@pytest.mark.parametrize("variant,published,tol", [("paper", 5.75e-2, 5e-5), ("authors_code", 5.529e-2, 1e-5)])
+def test_biflagellate_swims_the_published_distance(variant, published, tol):
+    """The oracle's bordered swimmer solve (swimming.py:66-102) driven by
+    scipy's RK45 (rtol 1e-6, as swimming.py:143 and the example's input3d) on
+    the Gallagher-Smith biflagellate: one beat moves the cell 5.75e-2
+    flagellar lengths along +x2 (examples/biflagellate_swimmer/README.md:15),
+    or 5.529e-2 with the published code's shorter flagellum (README.md:17)."""
+    from scipy.integrate import solve_ivp
+    from fluidlets_amd.geometry import gs_biflagellate
+
+    shape_at = gs_biflagellate(variant)
+    assert len(shape_at(0.0).x) == 176 and len(shape_at(0.0).X) == 1400
+
+    def rhs(t, z):
+        b1, b2 = z[3:6], z[6:9]
+        sh = shape_at(t)
+        U, W, _, _ = O.swimming_problem("stokeslet", sh.x, sh.v, sh.X, z[:3], np.c_[b1, b2, np.cross(b1, b2)],
+                                        0.0125)
+        return np.concatenate([U, np.cross(W, b1), np.cross(W, b2)])
+
+    z0 = np.concatenate([np.zeros(3), [1.0, 0, 0], [0, 1.0, 0]])
+    sol = solve_ivp(rhs, (0.0, 2 * np.pi), z0, t_eval=[0.0, 2 * np.pi], rtol=1e-6, atol=1e-10)
+    d = sol.y[:3, -1] - sol.y[:3, 0]
+    assert abs(np.linalg.norm(d) - published) < tol
+    assert d[1] > 0 and abs(d[0]) < 1e-12 and abs(d[2]) < 1e-12      # straight forward along +x2
