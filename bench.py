@@ -317,14 +317,15 @@ def c3_solves(torch, fl, D, kernels=("wall",), n_points=8192, repeats=2, vendor=
 
 
 def serving_section(torch, n_requests=2000, baseline_requests=300, swimmer_steps=200, n_max=6000, slots=4, seed=5,
-                    cache_gib=48, load=0.9, small_solves=True):
+                    cache_gib=48, load=0.9, small_solves=True, group=8):
     """Serving mix: n_requests Poisson-arrival requests, 70 % six-mode
     resistance solves on ellipsoids (N uniform in [200, n_max]) and 30 %
     biflagellate swimmer time steps (RK2 steps of swimmer sessions of about
     swimmer_steps steps each, their state kept by the server between steps).
 
-    - The continuous-batching server (mixed precision, `slots` streams, a
-      resident factor cache of cache_gib GiB that starts empty) serves the
+    - The continuous-batching server (mixed precision, `slots` streams,
+      small solves grouped up to `group` per launch, a resident factor
+      cache of cache_gib GiB that starts empty) serves the
       whole trace offline (all queued at t = 0: its capacity) and online
       (Poisson arrivals at `load` x that capacity).
     - The baseline, one request at a time in plain FP64 on the same GPU
@@ -339,11 +340,11 @@ def serving_section(torch, n_requests=2000, baseline_requests=300, swimmer_steps
     warm = [r for r in reqs if r.kind == "resistance"][:2]
     base_srv = SolveServer(slots=1, precision="fp64")
     base_srv.serve(warm, realtime=False)               # warm-up: first allocations and library start-up
-    with SolveServer(slots=slots, precision="mixed") as w:
+    with SolveServer(slots=slots, precision="mixed", group=group) as w:
         w.serve(warm, realtime=False)
 
     def fresh():
-        return SolveServer(slots=slots, precision="mixed", cache_bytes=int(cache_gib) << 30)
+        return SolveServer(slots=slots, precision="mixed", cache_bytes=int(cache_gib) << 30, group=group)
 
     def tick(tag, total):
         done = [0]
@@ -414,35 +415,44 @@ def serving_section(torch, n_requests=2000, baseline_requests=300, swimmer_steps
     return out
 
 
-def small_solves_section(slots=4, seed=5, n_requests=192, passes=3, modes=("graphs", "direct")):
+def small_solves_section(slots=4, seed=5, n_requests=192, passes=3, modes=("grouped", "graphs", "direct"),
+                         group=8):
     """Small six-mode resistance solves (N uniform in [200, 640] points, the
     serving mix's ellipsoids and kernels), all queued at t = 0, served with
-    the captured graphs (padded buckets, fixed corrections) and without them
-    (unpadded, adaptive refinement), on one slot and on `slots` slots.  Each
-    server serves the trace once to warm up (graph captures, allocations),
-    then `passes` times timed; the pass with the median throughput is
-    reported.  ``modes`` selects the paths ("graphs", "direct")."""
+    grouped graphs (up to `group` requests of one bucket per launch), with
+    one captured graph per request (padded buckets, fixed corrections) and
+    without graphs (unpadded, adaptive refinement), on one slot and on
+    `slots` slots.  Each server serves the trace once to warm up
+    (allocations), then `passes` times timed; the pass with the median
+    throughput is reported.  ``modes`` selects the paths."""
     from fluidlets_amd.serving import SolveServer, serving_mix, summarize
 
     reqs = serving_mix(n_requests, seed=seed + 2, n_min=200, n_max=640, swimmer_frac=0.0, rate=1.0)
     out = {"requests": n_requests, "n_points": [200, 640], "passes": passes}
+    out["group"] = group
     for k in sorted({1, slots}):
-        for graphs in [m == "graphs" for m in modes]:
-            srv = SolveServer(slots=k, precision="mixed", graphs=graphs)
+        for mode in modes:
+            t0 = time.perf_counter()
+            srv = SolveServer(slots=k, precision="mixed", graphs=mode != "direct",
+                              group=group if mode == "grouped" else 1)
+            build_s = time.perf_counter() - t0
             srv.serve(reqs, realtime=False)
             runs = sorted((summarize(srv.serve(reqs, realtime=False)) for _ in range(passes)),
                           key=lambda s: s["solves_per_s"])
             s = runs[len(runs) // 2]
-            tag = f"{'graphs' if graphs else 'direct'}_{k}_slots"
+            tag = f"{mode}_{k}_slots"
             out[tag] = {key: s[key] for key in ("solves_per_s", "latency_p50_s", "latency_p99_s", "makespan_s")}
             out[tag]["solves_per_s_range"] = [runs[0]["solves_per_s"], runs[-1]["solves_per_s"]]
-            if graphs:
+            if mode != "direct":
                 out[tag]["graph_captures"] = srv.graphs.captures
+                out[tag]["server_build_s"] = build_s
+            srv.close()
             progress(f"serving: small solves {tag} {s['solves_per_s']:.1f} requests/s")
     for k in sorted({1, slots}):
-        if f"graphs_{k}_slots" in out and f"direct_{k}_slots" in out:
-            out[f"graph_speedup_{k}_slots"] = (out[f"graphs_{k}_slots"]["solves_per_s"]
-                                               / out[f"direct_{k}_slots"]["solves_per_s"])
+        for mode in ("graphs", "grouped"):
+            if f"{mode}_{k}_slots" in out and f"direct_{k}_slots" in out:
+                out[f"{mode}_speedup_{k}_slots"] = (out[f"{mode}_{k}_slots"]["solves_per_s"]
+                                                    / out[f"direct_{k}_slots"]["solves_per_s"])
     return out
 
 

@@ -72,7 +72,24 @@ def _hip_runtime():
         _hip = ctypes.CDLL("libamdhip64.so.7")
         _hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
         _hip.hipStreamCreateWithFlags.restype = ctypes.c_int
+        _hip.hipMemcpy2DAsync.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+        _hip.hipMemcpy2DAsync.restype = ctypes.c_int
     return _hip
+
+
+def copy_rows_async(dst: int, dst_pitch: int, src: int, src_pitch: int, width: int, rows: int, kind: str,
+                    stream) -> None:
+    """`rows` rows of `width` bytes between pitched buffers, one asynchronous
+    hipMemcpy2DAsync on `stream`; kind "h2d" or "d2h" (host memory pinned)."""
+    if rows <= 0 or width <= 0:
+        return
+    if width > min(dst_pitch, src_pitch):
+        raise ValueError("row width exceeds a pitch")
+    rc = _hip_runtime().hipMemcpy2DAsync(dst, dst_pitch, src, src_pitch, width, rows, {"h2d": 1, "d2h": 2}[kind],
+                                         stream_handle(stream))
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy2DAsync failed with HIP error {rc}")
 
 
 def lease_stream():

@@ -51,6 +51,13 @@ _SIGNATURES = {
     "fl_pad_solve": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "fl_refine_steps": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
                                 c_int64, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fl_pad_solve_group": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                   c_int64, c_void_p]),
+    "fl_sgetrf_async_group": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64,
+                                      c_void_p, c_size_t, c_void_p]),
+    "fl_refine_steps_group": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
+                                      c_int64, c_int64, c_int, c_void_p, c_int64, c_int64, c_void_p, c_size_t,
+                                      c_void_p]),
     "fl_refine_inv": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
                               c_int64, c_double, c_int, POINTER(c_int), POINTER(c_double), c_void_p, c_size_t,
                               c_void_p]),

@@ -32,6 +32,12 @@ void clear_error()
     g_msg[0] = '\0';
 }
 
+Batch& batch()
+{
+    static thread_local Batch b;
+    return b;
+}
+
 int device_cus()
 {
     static std::mutex mu;
@@ -306,6 +312,49 @@ int fl_pad_solve(double* K64, float* K32, int64_t n_pad, const double* n_dev, co
 {
     clear_error();
     return pad_solve_impl(K64, K32, n_pad, n_dev, rhs, b, nrhs, as_stream(stream));
+}
+
+// ---------------------------------------------------------------- grouped calls
+namespace {
+int check_group(int64_t count, int64_t slab_bytes)
+{
+    FL_CHECK(count >= 1 && count <= 65535, "group size %lld outside [1, 65535]", (long long)count);
+    FL_CHECK(count == 1 || (slab_bytes > 0 && slab_bytes % 256 == 0),
+             "slab_bytes must be a positive multiple of 256 (got %lld)", (long long)slab_bytes);
+    return FL_OK;
+}
+}  // namespace
+
+int fl_pad_solve_group(double* K64, float* K32, int64_t n_pad, const double* n_dev, const double* rhs, double* b,
+                       int64_t nrhs, int64_t count, int64_t slab_bytes, void* stream)
+{
+    clear_error();
+    if (int e = check_group(count, slab_bytes)) return e;
+    BatchScope scope(count, slab_bytes);
+    return pad_solve_impl(K64, K32, n_pad, n_dev, rhs, b, nrhs, as_stream(stream));
+}
+
+int fl_sgetrf_async_group(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                          int64_t count, int64_t slab_bytes, void* workspace, size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (int e = check_group(count, slab_bytes)) return e;
+    BatchScope scope(count, slab_bytes);
+    return sgetrf_async(A, n, lda, ipiv, aux, status, FL_LU_FP32, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_refine_steps_group(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32,
+                          int64_t n, const double* b, double* x, int64_t nrhs, int64_t ldb, int steps,
+                          double* resid_dev, int64_t count, int64_t slab_bytes, void* workspace,
+                          size_t workspace_bytes, void* stream)
+{
+    clear_error();
+    if (!resid_dev) return set_error(FL_EINVAL, "fl_refine_steps_group needs resid_dev");
+    if (steps < 0) return set_error(FL_EINVAL, "fl_refine_steps_group: negative step count");
+    if (int e = check_group(count, slab_bytes)) return e;
+    BatchScope scope(count, slab_bytes);
+    return refine_impl(A64, lda64, LU32, lda32, aux32, n, b, x, nrhs, ldb, 0.0, steps, nullptr, nullptr, nullptr,
+                       workspace, workspace_bytes, as_stream(stream), nullptr, 0, resid_dev);
 }
 
 int fl_sgetri(const float* LU, int64_t n, int64_t lda, const void* aux, float* Ainv, int64_t ldi, void* stream)

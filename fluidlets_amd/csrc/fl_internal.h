@@ -38,6 +38,34 @@ void clear_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Grouped (batched) calls: one launch per kernel serves `count` problems laid
+// out in equal slabs of `zb` bytes (every pointer of problem z is problem
+// 0's plus z zb).  The batched entry points set this for the calling thread
+// for the duration of the call; outside one it is (1, 0).  Kernels on the
+// batched paths take zb and offset their pointers by blockIdx.z zb; their
+// grids get gridDim.z = count from zgrid().
+struct Batch {
+    int64_t count = 1;
+    int64_t zb = 0;
+};
+Batch& batch();
+inline dim3 zgrid(dim3 g)
+{
+    g.z = (unsigned)batch().count;
+    return g;
+}
+template <class P>
+__device__ __forceinline__ P zoff(P p, int64_t zb)
+{
+    return p ? reinterpret_cast<P>(reinterpret_cast<uintptr_t>(p) + (uint64_t)blockIdx.z * (uint64_t)zb) : p;
+}
+// Sets the calling thread's batch for a scope.
+struct BatchScope {
+    Batch saved;
+    BatchScope(int64_t count, int64_t zb) : saved(batch()) { batch() = Batch{count, zb}; }
+    ~BatchScope() { batch() = saved; }
+};
+
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 int device_cus();

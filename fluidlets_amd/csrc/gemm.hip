@@ -49,7 +49,18 @@ struct GemmArgs {
     int swizzle;  // 1: XCD-aware grouped tile order
     int lower;    // only the tiles on or below the diagonal (M == N), enumerated densely
     int ntiles;   // tiles of the product (all, or the lower ones)
+    int64_t zb;   // grouped launches: problem blockIdx.z's pointers are z zb bytes on
 };
+
+// This workgroup's problem of a grouped launch (blockIdx.z).
+template <typename T>
+__device__ __forceinline__ void to_problem(GemmArgs<T>& g)
+{
+    g.A = zoff(g.A, g.zb);
+    g.Bm = zoff(g.Bm, g.zb);
+    g.C = zoff(g.C, g.zb);
+    g.absmax_bits = zoff(g.absmax_bits, g.zb);
+}
 
 // Tile of this workgroup.  The grid is 1-D; workgroups are dealt to the 8
 // XCDs round-robin (id % 8), so with `swizzle` each XCD is first given a
@@ -280,6 +291,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
 template <typename T>
 __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
 {
+    to_problem(g);
     gemm_tile(g, (int)blockIdx.x);
 }
 
@@ -291,6 +303,8 @@ template <typename T>
 __global__ __launch_bounds__(WG, 2) void gemm_pair_kernel(GemmArgs<T> g1, GemmArgs<T> g2, int tiles1)
 {
     const int id = (int)blockIdx.x;
+    to_problem(g1);
+    to_problem(g2);
     if (id < tiles1) gemm_tile(g1, id);
     else gemm_tile(g2, id - tiles1);
 }
@@ -317,7 +331,7 @@ GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int6
     const int tiles_m = (int)((M + TM - 1) / TM), tiles_n = (int)((N + TN - 1) / TN);
     return GemmArgs<T>{M,     N,          K,           alpha,   beta,    A,       lda,   B,
                        ldb,   C,          ldc,         absmax_bits, aligned, tiles_m, tiles_n, swizzle,
-                       0,     tiles_m * tiles_n};
+                       0,     tiles_m * tiles_n, batch().zb};
 }
 
 }  // namespace
@@ -336,7 +350,7 @@ int gemm_pair_launch(int64_t M1, int64_t N1, int64_t K1, T alpha1, const T* A1, 
     const GemmArgs<T> g1 = gemm_args<T>(M1, N1, K1, alpha1, A1, lda1, B1, ldb1, beta1, C1, ldc1, absmax1);
     const GemmArgs<T> g2 = gemm_args<T>(M2, N2, K2, alpha2, A2, lda2, B2, ldb2, beta2, C2, ldc2, nullptr);
     const int t1 = g1.tiles_m * g1.tiles_n, t2 = g2.tiles_m * g2.tiles_n;
-    hipLaunchKernelGGL(gemm_pair_kernel<T>, dim3((unsigned)(t1 + t2)), dim3(WG), 0, stream, g1, g2, t1);
+    hipLaunchKernelGGL(gemm_pair_kernel<T>, zgrid(dim3((unsigned)(t1 + t2))), dim3(WG), 0, stream, g1, g2, t1);
     FL_LAUNCHED();
     return FL_OK;
 }
@@ -357,13 +371,14 @@ int gemm_launch(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t ld
     FL_CHECK(M >= 0 && N >= 0 && K >= 0, "negative GEMM size");
     if (M == 0 || N == 0) return FL_OK;
     if (alpha == T(0)) {
+        FL_CHECK(batch().count == 1, "a grouped GEMM needs alpha != 0");
         hipLaunchKernelGGL(scale_kernel<T>, dim3((unsigned)((M * N + WG - 1) / WG)), dim3(WG), 0, stream, C, ldc, M,
                            N, beta);
         FL_LAUNCHED();
         return FL_OK;
     }
     const GemmArgs<T> g = gemm_args<T>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, absmax_bits);
-    hipLaunchKernelGGL(gemm_kernel<T>, dim3((unsigned)(g.tiles_m * g.tiles_n)), dim3(WG), 0, stream, g);
+    hipLaunchKernelGGL(gemm_kernel<T>, zgrid(dim3((unsigned)(g.tiles_m * g.tiles_n))), dim3(WG), 0, stream, g);
     FL_LAUNCHED();
     return FL_OK;
 }
@@ -386,7 +401,7 @@ int gemm_lower_update(int64_t n, int64_t K, const T* A, int64_t lda, const T* B,
     g.lower = 1;
     g.ntiles = g.tiles_m * (g.tiles_m + 1) / 2;
     const int launch = (g.ntiles + NXCD - 1) / NXCD * NXCD;
-    hipLaunchKernelGGL(gemm_kernel<T>, dim3((unsigned)launch), dim3(WG), 0, stream, g);
+    hipLaunchKernelGGL(gemm_kernel<T>, zgrid(dim3((unsigned)launch)), dim3(WG), 0, stream, g);
     FL_LAUNCHED();
     return FL_OK;
 }

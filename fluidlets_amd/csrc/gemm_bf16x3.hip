@@ -362,6 +362,7 @@ int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, 
 {
     FL_CHECK(M >= 0 && N >= 0 && K >= 0, "negative size");
     FL_CHECK(!lower || M == N, "a lower-triangle update needs M == N");
+    FL_CHECK(batch().count == 1, "split3_update has no grouped form");
     if (M == 0 || N == 0 || K == 0) return FL_OK;
     FL_CHECK(ws && ws_bytes >= split3_workspace_bytes(M, N, K), "split3_update needs %zu workspace bytes",
              split3_workspace_bytes(M, N, K));

@@ -766,8 +766,14 @@ struct Acc32<double> {
 
 template <typename T>
 __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int nb, int64_t k0, int32_t* ipiv,
-                                                         unsigned int* lmax_bits, T* invL, T* invU)
+                                                         unsigned int* lmax_bits, T* invL, T* invU, int64_t zb)
 {
+    A = zoff(A, zb);
+    ipiv = zoff(ipiv, zb);
+    lmax_bits = zoff(lmax_bits, zb);
+    invL = zoff(invL, zb);
+    invU = zoff(invU, zb);
+
     constexpr int LS = NB + 1, LB = SUBB + 1, NBB = NB / SUBB;
     // FP32 keeps every leaf inverse and the assembled inverse blocks in LDS
     // (IL_k, IU_k, then the six off-diagonal blocks of each inverse); FP64's
@@ -1077,8 +1083,10 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 // sync = [ticket, error], zeroed before the launch.
 constexpr int CH_SPIN = 1 << 22;
 
-__global__ void zero_u32_kernel(unsigned* p, int64_t n)
+__global__ void zero_u32_kernel(unsigned* p, int64_t n, int64_t zb)
 {
+    p = zoff(p, zb);
+
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
@@ -1098,8 +1106,10 @@ struct ChainBits<double> {
 };
 
 template <typename T>
-__global__ void fill_sentinel_kernel(T* x, int64_t n)
+__global__ void fill_sentinel_kernel(T* x, int64_t n, int64_t zb)
 {
+    x = zoff(x, zb);
+
     using U = typename ChainBits<T>::U;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) reinterpret_cast<U*>(x)[i] = ChainBits<T>::sentinel;
@@ -1139,8 +1149,14 @@ __device__ __forceinline__ T row_sum(T v)
 // covering NB / TPR columns of every block product.
 template <typename T, bool LOWER, int NR, int TPR>
 __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_t lda, int64_t n, const T* invs,
-                                                             const T* b, T* x, int nrhs, unsigned* sync)
+                                                             const T* b, T* x, int nrhs, unsigned* sync, int64_t zb)
 {
+    LU = zoff(LU, zb);
+    invs = zoff(invs, zb);
+    b = zoff(b, zb);
+    x = zoff(x, zb);
+    sync = zoff(sync, zb);
+
     constexpr int H = NB / TPR;
     constexpr int NT = NB * TPR;
     constexpr int V = Vec16<T>::n;
@@ -1297,8 +1313,13 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
 // right-hand side by the factor's row permutation).
 template <typename TS, typename TD>
 __global__ __launch_bounds__(256) void gather_rows_kernel(const TS* src, int64_t lds, TD* dst, int64_t n, int nrhs,
-                                                          const int32_t* perm, const double* inv_scale)
+                                                          const int32_t* perm, const double* inv_scale, int64_t zb)
 {
+    src = zoff(src, zb);
+    dst = zoff(dst, zb);
+    perm = zoff(perm, zb);
+    inv_scale = zoff(inv_scale, zb);
+
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= n * nrhs) return;
     const int64_t i = idx / nrhs;
@@ -1350,8 +1371,11 @@ __global__ __launch_bounds__(256) void axpy_rot_kernel(double* x, const float* d
 }
 
 template <typename TS, typename TD>
-__global__ __launch_bounds__(256) void copy_rows_kernel(const TS* src, TD* dst, int64_t ldd, int64_t n, int nrhs)
+__global__ __launch_bounds__(256) void copy_rows_kernel(const TS* src, TD* dst, int64_t ldd, int64_t n, int nrhs, int64_t zb)
 {
+    src = zoff(src, zb);
+    dst = zoff(dst, zb);
+
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= n * nrhs) return;
     const int64_t i = idx / nrhs;
@@ -1372,8 +1396,13 @@ constexpr int RES_ROWS = 8;
 // in flight.  Either variant sums in a fixed order for a given shape.
 template <bool PAIR>
 __global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t lda, int64_t n, const double* b,
-                                                       int64_t ldb, const double* x, double* r, int nrhs)
+                                                       int64_t ldb, const double* x, double* r, int nrhs, int64_t zb)
 {
+    A = zoff(A, zb);
+    b = zoff(b, zb);
+    x = zoff(x, zb);
+    r = zoff(r, zb);
+
     __shared__ double part[4][RES_ROWS * 8];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int64_t row0 = (int64_t)blockIdx.x * RES_ROWS;
@@ -1505,8 +1534,11 @@ __global__ __launch_bounds__(256) void perm_identity_kernel(float* X, int64_t ld
 
 // Per column q of an n x nrhs (ld) array: out[q] = sum_i a_iq^2 and
 // out[8 + q] = max_i |a_iq| (one workgroup per column, fixed order).
-__global__ __launch_bounds__(256) void colnorm_kernel(const double* a, int64_t ld, int64_t n, double* out)
+__global__ __launch_bounds__(256) void colnorm_kernel(const double* a, int64_t ld, int64_t n, double* out, int64_t zb)
 {
+    a = zoff(a, zb);
+    out = zoff(out, zb);
+
     __shared__ double ss[256], sm[256];
     const int q = blockIdx.x, tid = threadIdx.x;
     double s = 0.0, m = 0.0;
@@ -1534,8 +1566,12 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const double* a, int64_t l
 // x[i][q] (ld ldx) += scale[q] * d[i][q] (FP32 or FP64 correction, ld nrhs)
 template <typename TD>
 __global__ __launch_bounds__(256) void axpy_rows_kernel(double* x, int64_t ldx, const TD* d, int64_t n, int nrhs,
-                                                        const double* scale)
+                                                        const double* scale, int64_t zb)
 {
+    x = zoff(x, zb);
+    d = zoff(d, zb);
+    scale = zoff(scale, zb);
+
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= n * nrhs) return;
     const int64_t i = idx / nrhs;
@@ -1543,8 +1579,12 @@ __global__ __launch_bounds__(256) void axpy_rows_kernel(double* x, int64_t ldx, 
     x[i * ldx + q] += scale[q] * (double)d[idx];
 }
 
-__global__ void scale_from_norms_kernel(const double* norms, double* scale, double* inv_scale, int nrhs)
+__global__ void scale_from_norms_kernel(const double* norms, double* scale, double* inv_scale, int nrhs, int64_t zb)
 {
+    norms = zoff(norms, zb);
+    scale = zoff(scale, zb);
+    inv_scale = zoff(inv_scale, zb);
+
     const int q = threadIdx.x;
     if (q >= nrhs) return;
     const double m = norms[8 + q];
@@ -1584,8 +1624,11 @@ struct FactorAux {
 // 8 x 8 tile of the product, K staged through LDS in slices of 32.  blocks =
 // FactorAux::invL(aux, n, 0) (the four blocks of panel p start at p 4 NB^2).
 template <typename T>
-__global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int64_t n, T* blocks)
+__global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int64_t n, T* blocks, int64_t zb)
 {
+    LU = zoff(LU, zb);
+    blocks = zoff(blocks, zb);
+
     constexpr int KC = 32;
     __shared__ T Is[NB * (KC + 1)];
     __shared__ T Bs[KC * NB];
@@ -1705,8 +1748,12 @@ size_t getrf_ws_bytes(int64_t n)
 // exceeded 1 + 2^-20 (then the no-interchange pivots are partial pivoting's
 // and the factors are those of the verified path), 1 otherwise; the row
 // permutation is the identity.
-__global__ void spec_finish_kernel(const unsigned int* lmax_bits, unsigned int* status, int32_t* perm, int64_t n)
+__global__ void spec_finish_kernel(const unsigned int* lmax_bits, unsigned int* status, int32_t* perm, int64_t n, int64_t zb)
 {
+    lmax_bits = zoff(lmax_bits, zb);
+    status = zoff(status, zb);
+    perm = zoff(perm, zb);
+
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) perm[i] = (int32_t)i;
     if (i == 0) {
@@ -1742,7 +1789,7 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     FL_CHECK(lda >= n, "lda < n");
     if (n == 0) {
         if (spec) {
-            hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, status, (int64_t)1);
+            hipLaunchKernelGGL(zero_u32_kernel, zgrid(dim3(1)), dim3(64), 0, st, status, (int64_t)1, batch().zb);
             FL_LAUNCHED();
         }
         return FL_OK;
@@ -1760,8 +1807,10 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     // Below that the factorization is the FP32-product one (256-column
     // panels, faster there).
     void* split = split_products && std::is_same_v<T, float> && n - NBO_SPLIT >= 2048 ? tail + 256 : nullptr;
+    FL_CHECK(batch().count == 1 || (spec && !sym && !split),
+             "a grouped factorization is the asynchronous one with FP32 products");
     if (spec) {
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, flag, (int64_t)1);
+        hipLaunchKernelGGL(zero_u32_kernel, zgrid(dim3(1)), dim3(64), 0, st, flag, (int64_t)1, batch().zb);
         FL_LAUNCHED();
     } else {
         FL_HIP(hipMemsetAsync(info_d, 0, sizeof(int32_t), st));
@@ -1793,8 +1842,8 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             T* invU = FactorAux<T>::invU(aux, n, p);
             bool u12_done = false;   // the in-panel U12 already formed (grouped with L21)
             if (spec) {
-                hipLaunchKernelGGL(diag_block_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag, invL,
-                                   invU);
+                hipLaunchKernelGGL(diag_block_kernel<T>, zgrid(dim3(1)), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag, invL,
+                                   invU, batch().zb);
                 FL_LAUNCHED();
                 const int64_t rest0 = pe - ks - nb;
                 if (sym && m > nb) {
@@ -1845,8 +1894,8 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                 }
                 FL_HIP(hipMemsetAsync(flag, 0, 2 * sizeof(unsigned int), st));  // flag and info_blk
                 if (tier == 0)
-                    hipLaunchKernelGGL(diag_block_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag,
-                                       invL, invU);
+                    hipLaunchKernelGGL(diag_block_kernel<T>, zgrid(dim3(1)), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, flag,
+                                       invL, invU, batch().zb);
                 else
                     hipLaunchKernelGGL(diag_lu_kernel<T>, dim3(1), dim3(256), 0, st, Akk, lda, nb, ks, ipiv, info_blk,
                                        invL, invU);
@@ -1920,12 +1969,12 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         }
     }
     // the chained solves' links, from the finished factors
-    hipLaunchKernelGGL(link_kernel<T>, dim3((unsigned)div_up(n, NB), 2), dim3(256), 0, st, (const T*)A, lda, n,
-                       FactorAux<T>::invL(aux, n, 0));
+    hipLaunchKernelGGL(link_kernel<T>, zgrid(dim3((unsigned)div_up(n, NB), 2)), dim3(256), 0, st, (const T*)A, lda, n,
+                       FactorAux<T>::invL(aux, n, 0), batch().zb);
     FL_LAUNCHED();
     if (spec) {
-        hipLaunchKernelGGL(spec_finish_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, st, (const unsigned*)flag,
-                           status, FactorAux<T>::perm(aux), n);
+        hipLaunchKernelGGL(spec_finish_kernel, zgrid(dim3((unsigned)div_up(n, 256))), dim3(256), 0, st, (const unsigned*)flag,
+                           status, FactorAux<T>::perm(aux), n, batch().zb);
         FL_LAUNCHED();
         return FL_OK;
     }
@@ -1965,12 +2014,12 @@ void launch_chain(const T* LU, int64_t lda, int64_t n, const T* invs, const T* b
 {
     const dim3 grid((unsigned)div_up(n, NB)), blk(NB * CH_TPR);
     if (nrhs <= 1)
-        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 1, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 1, CH_TPR>), zgrid(grid), blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync, batch().zb);
     else if (nrhs <= 2)
-        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 2, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 2, CH_TPR>), zgrid(grid), blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync, batch().zb);
     else if (nrhs <= 4)
-        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 4, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
-    else hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 8, CH_TPR>), grid, blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync);
+        hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 4, CH_TPR>), zgrid(grid), blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync, batch().zb);
+    else hipLaunchKernelGGL((trsm_chain_kernel<T, LOWER, 8, CH_TPR>), zgrid(grid), blk, 0, st, LU, lda, n, invs, b, x, nrhs, sync, batch().zb);
 }
 
 // Solves in place on a contiguous n x nrhs buffer `y` that already holds the
@@ -1981,17 +2030,17 @@ int solve_permuted(const T* LU, int64_t n, int64_t lda, void* aux, T* y, T* tmp,
                    hipStream_t st)
 {
     const int64_t words = (int64_t)(chain_sync_bytes(n) / sizeof(unsigned));
-    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)div_up(words, 256)), dim3(256), 0, st, sync, words);
+    hipLaunchKernelGGL(zero_u32_kernel, zgrid(dim3((unsigned)div_up(words, 256))), dim3(256), 0, st, sync, words, batch().zb);
     FL_LAUNCHED();
     const T* invs = FactorAux<T>::invL(aux, n, 0);
     const int64_t half = div_up(n, NB) + 2;
     const int64_t len = n * nrhs;
     const dim3 fg((unsigned)div_up(len, 256));
-    hipLaunchKernelGGL(fill_sentinel_kernel<T>, fg, dim3(256), 0, st, tmp, len);
+    hipLaunchKernelGGL(fill_sentinel_kernel<T>, zgrid(fg), dim3(256), 0, st, tmp, len, batch().zb);
     FL_LAUNCHED();
     launch_chain<T, true>(LU, lda, n, invs, (const T*)y, tmp, nrhs, sync, st);
     FL_LAUNCHED();
-    hipLaunchKernelGGL(fill_sentinel_kernel<T>, fg, dim3(256), 0, st, y, len);   // y was the forward sweep's input
+    hipLaunchKernelGGL(fill_sentinel_kernel<T>, zgrid(fg), dim3(256), 0, st, y, len, batch().zb);   // y was the forward sweep's input
     FL_LAUNCHED();
     launch_chain<T, false>(LU, lda, n, invs, (const T*)tmp, y, nrhs, sync + half, st);
     FL_LAUNCHED();
@@ -2014,19 +2063,21 @@ int getrs_impl(const T* LU, int64_t n, int64_t lda, const void* aux_c, T* B, int
     T* tmp = reinterpret_cast<T*>(static_cast<char*>(ws) + one);
     unsigned* sync = reinterpret_cast<unsigned*>(static_cast<char*>(ws) + 2 * one);
     const unsigned g = (unsigned)div_up(n * nrhs, 256);
-    hipLaunchKernelGGL((gather_rows_kernel<T, T>), dim3(g), dim3(256), 0, st, (const T*)B, ldb, y, n, (int)nrhs,
-                       (const int32_t*)FactorAux<T>::perm(aux), (const double*)nullptr);
+    hipLaunchKernelGGL((gather_rows_kernel<T, T>), zgrid(dim3(g)), dim3(256), 0, st, (const T*)B, ldb, y, n, (int)nrhs,
+                       (const int32_t*)FactorAux<T>::perm(aux), (const double*)nullptr, batch().zb);
     FL_LAUNCHED();
     if (int s = solve_permuted<T>(LU, n, lda, aux, y, tmp, (int)nrhs, sync, st)) return s;
-    hipLaunchKernelGGL((copy_rows_kernel<T, T>), dim3(g), dim3(256), 0, st, (const T*)y, B, ldb, n, (int)nrhs);
+    hipLaunchKernelGGL((copy_rows_kernel<T, T>), zgrid(dim3(g)), dim3(256), 0, st, (const T*)y, B, ldb, n, (int)nrhs, batch().zb);
     FL_LAUNCHED();
     return FL_OK;
 }
 
 // Zeroes n doubles (the fixed-step refinement clears x with a kernel, not a
 // memset, so that it can be captured in a graph).
-__global__ void zero_f64_kernel(double* p, int64_t n)
+__global__ void zero_f64_kernel(double* p, int64_t n, int64_t zb)
 {
+    p = zoff(p, zb);
+
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
 }
@@ -2035,8 +2086,14 @@ __global__ void zero_f64_kernel(double* p, int64_t n)
 // ||r||_2 / ||b||_2 per column (||r||_2 when b = 0), from the squared norms;
 // NaN when a chained solve's block wait timed out (e0, e1: its flags).
 __global__ void finish_resid_kernel(const double* norms, const double* bnorms, const unsigned* e0, const unsigned* e1,
-                                    double* out, int q)
+                                    double* out, int q, int64_t zb)
 {
+    norms = zoff(norms, zb);
+    bnorms = zoff(bnorms, zb);
+    e0 = zoff(e0, zb);
+    e1 = zoff(e1, zb);
+    out = zoff(out, zb);
+
     const int c = threadIdx.x;
     if (c >= q) return;
     const double bn = sqrt(bnorms[c]);
@@ -2062,6 +2119,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     // host synchronization and no memset (capturable in a HIP graph) and
     // writes the final relative residuals to resid_dev on the device.
     const bool fixed = resid_dev != nullptr;
+    FL_CHECK(batch().count == 1 || (fixed && !frame && !inv32), "grouped refinement takes fixed steps on LU factors");
     FL_CHECK(!inv32 || ldi >= n, "ldi < n");
     FL_CHECK(!fixed || max_iter >= 0, "negative step count");
     FL_CHECK(n >= 0 && nrhs >= 1 && nrhs <= 8, "need 1 <= nrhs <= 8");
@@ -2096,13 +2154,13 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
     const bool pair = reinterpret_cast<uintptr_t>(A64) % 16 == 0 && lda64 % 2 == 0;
 
     // r = b (contiguous, ld nrhs) and ||b|| per column
-    hipLaunchKernelGGL((gather_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, b, ldb, r, n, q,
-                       (const int32_t*)nullptr, (const double*)nullptr);
+    hipLaunchKernelGGL((gather_rows_kernel<double, double>), zgrid(dim3(g)), dim3(256), 0, st, b, ldb, r, n, q,
+                       (const int32_t*)nullptr, (const double*)nullptr, batch().zb);
     FL_LAUNCHED();
-    hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, bnorms);
+    hipLaunchKernelGGL(colnorm_kernel, zgrid(dim3(q)), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, bnorms, batch().zb);
     FL_LAUNCHED();
     if (fixed) {
-        hipLaunchKernelGGL(zero_f64_kernel, dim3(g), dim3(256), 0, st, xc, n * nrhs);
+        hipLaunchKernelGGL(zero_f64_kernel, zgrid(dim3(g)), dim3(256), 0, st, xc, n * nrhs, batch().zb);
         FL_LAUNCHED();
     } else {
         FL_HIP(hipMemsetAsync(xc, 0, (size_t)n * nrhs * sizeof(double), st));
@@ -2119,7 +2177,7 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
         // x = 0 and r = b, whose norms are already known (relative residual 1),
         // so the host looks only from the first correction on.
         if (it > 0 || max_iter <= 0) {
-            hipLaunchKernelGGL(colnorm_kernel, dim3(q), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, norms);
+            hipLaunchKernelGGL(colnorm_kernel, zgrid(dim3(q)), dim3(256), 0, st, (const double*)r, (int64_t)nrhs, n, norms, batch().zb);
             FL_LAUNCHED();
             if (fixed) {
                 if (it >= max_iter) break;   // the norms stay on the device
@@ -2137,8 +2195,8 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
             }
         }
         // d = LU32^{-1} (r / s) s, with s = max |r| per column so FP32 sees O(1) values
-        hipLaunchKernelGGL(scale_from_norms_kernel, dim3(1), dim3(8), 0, st, (const double*)(it == 0 ? bnorms : norms),
-                           scale, inv_scale, q);
+        hipLaunchKernelGGL(scale_from_norms_kernel, zgrid(dim3(1)), dim3(8), 0, st, (const double*)(it == 0 ? bnorms : norms),
+                           scale, inv_scale, q, batch().zb);
         // with an explicit inverse (which includes the row permutation) the
         // scaled residual goes to tmp unpermuted and y = inv32 tmp
         const int32_t* perm = inv32 ? nullptr : (const int32_t*)FactorAux<float>::perm(aux);
@@ -2147,8 +2205,8 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
             hipLaunchKernelGGL(gather_rot_kernel, dim3(g), dim3(256), 0, st, (const double*)r, yin, n, q, perm,
                                (const double*)inv_scale, R);
         else
-            hipLaunchKernelGGL((gather_rows_kernel<double, float>), dim3(g), dim3(256), 0, st, (const double*)r,
-                               (int64_t)nrhs, yin, n, q, perm, (const double*)inv_scale);
+            hipLaunchKernelGGL((gather_rows_kernel<double, float>), zgrid(dim3(g)), dim3(256), 0, st, (const double*)r,
+                               (int64_t)nrhs, yin, n, q, perm, (const double*)inv_scale, batch().zb);
         FL_LAUNCHED();
         if (inv32) {
             hipLaunchKernelGGL(inv_apply_kernel, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, inv32, ldi, n,
@@ -2161,26 +2219,26 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
             hipLaunchKernelGGL(axpy_rot_kernel, dim3(g), dim3(256), 0, st, xc, (const float*)y, n, q,
                                (const double*)scale, R);
         else
-            hipLaunchKernelGGL(axpy_rows_kernel<float>, dim3(g), dim3(256), 0, st, xc, (int64_t)nrhs, (const float*)y,
-                               n, q, (const double*)scale);
+            hipLaunchKernelGGL(axpy_rows_kernel<float>, zgrid(dim3(g)), dim3(256), 0, st, xc, (int64_t)nrhs, (const float*)y,
+                               n, q, (const double*)scale, batch().zb);
         FL_LAUNCHED();
         ++it;
         if (pair)
-            hipLaunchKernelGGL(residual_kernel<true>, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, A64, lda64,
-                               n, b, ldb, (const double*)xc, r, q);
+            hipLaunchKernelGGL(residual_kernel<true>, zgrid(dim3((unsigned)div_up(n, RES_ROWS))), dim3(256), 0, st, A64, lda64,
+                               n, b, ldb, (const double*)xc, r, q, batch().zb);
         else
-            hipLaunchKernelGGL(residual_kernel<false>, dim3((unsigned)div_up(n, RES_ROWS)), dim3(256), 0, st, A64,
-                               lda64, n, b, ldb, (const double*)xc, r, q);
+            hipLaunchKernelGGL(residual_kernel<false>, zgrid(dim3((unsigned)div_up(n, RES_ROWS))), dim3(256), 0, st, A64,
+                               lda64, n, b, ldb, (const double*)xc, r, q, batch().zb);
         FL_LAUNCHED();
     }
-    hipLaunchKernelGGL((copy_rows_kernel<double, double>), dim3(g), dim3(256), 0, st, (const double*)xc, x, ldb, n, q);
+    hipLaunchKernelGGL((copy_rows_kernel<double, double>), zgrid(dim3(g)), dim3(256), 0, st, (const double*)xc, x, ldb, n, q, batch().zb);
     FL_LAUNCHED();
     if (fixed) {
         const int64_t half = div_up(n, NB) + 2;
         const bool waits = !inv32 && it > 0;
-        hipLaunchKernelGGL(finish_resid_kernel, dim3(1), dim3(64), 0, st, (const double*)norms, (const double*)bnorms,
+        hipLaunchKernelGGL(finish_resid_kernel, zgrid(dim3(1)), dim3(64), 0, st, (const double*)norms, (const double*)bnorms,
                            (const unsigned*)(waits ? sync + 1 : nullptr),
-                           (const unsigned*)(waits ? sync + half + 1 : nullptr), resid_dev, q);
+                           (const unsigned*)(waits ? sync + half + 1 : nullptr), resid_dev, q, batch().zb);
         FL_LAUNCHED();
         if (iters_out) *iters_out = it;
         return FL_OK;
@@ -2254,8 +2312,14 @@ namespace {
 // n_dev), so that one captured graph serves every n <= n_pad.  One
 // workgroup per row; the leading n x n block is left alone.
 __global__ __launch_bounds__(256) void pad_solve_kernel(double* K64, float* K32, int64_t n_pad, const double* n_dev,
-                                                        const double* rhs, double* b, int nrhs)
+                                                        const double* rhs, double* b, int nrhs, int64_t zb)
 {
+    K64 = zoff(K64, zb);
+    K32 = zoff(K32, zb);
+    n_dev = zoff(n_dev, zb);
+    rhs = zoff(rhs, zb);
+    b = zoff(b, zb);
+
     const int64_t n = std::min<int64_t>(std::max<int64_t>((int64_t)n_dev[0], 0), n_pad);
     const int64_t i = blockIdx.x;
     double* r64 = K64 ? K64 + i * n_pad : nullptr;
@@ -2277,8 +2341,8 @@ int pad_solve_impl(double* K64, float* K32, int64_t n_pad, const double* n_dev, 
     FL_CHECK(nrhs >= 1 && nrhs <= 8, "need 1 <= nrhs <= 8");
     if (n_pad == 0) return FL_OK;
     FL_CHECK(n_dev && rhs && b, "null pointer argument");
-    hipLaunchKernelGGL(pad_solve_kernel, dim3((unsigned)n_pad), dim3(256), 0, st, K64, K32, n_pad, n_dev, rhs, b,
-                       (int)nrhs);
+    hipLaunchKernelGGL(pad_solve_kernel, zgrid(dim3((unsigned)n_pad)), dim3(256), 0, st, K64, K32, n_pad, n_dev, rhs, b,
+                       (int)nrhs, batch().zb);
     FL_LAUNCHED();
     return FL_OK;
 }

@@ -105,10 +105,9 @@ def _grand(X, center, x, mu) -> np.ndarray:
     """6 x 6 grand resistance from the (3N, 6) forces of the six modes."""
     f = x.reshape(len(X), 3, 6).transpose(2, 0, 1) * float(np.real(mu))
     r = X - np.asarray(center, float)
-    R = np.zeros((6, 6))
-    for m in range(6):
-        R[:3, m] = f[m].sum(axis=0)
-        R[3:, m] = np.cross(r, f[m]).sum(axis=0)
+    R = np.empty((6, 6))
+    R[:3] = f.sum(axis=1).T
+    R[3:] = np.cross(r[None], f).sum(axis=1).T
     return R
 
 
@@ -126,6 +125,11 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     dev = D.device()
     if graphs is not None and precision == "mixed":
         from . import graphs as G
+        if len(X) <= G.GRAPH_MAX_POINTS and graphs.members > 1:
+            R, err = solve_resistance_group([payload], cache, stream, graphs)[0]
+            if err is not None:
+                raise RuntimeError(err)
+            return R
         if len(X) <= G.GRAPH_MAX_POINTS:
             key = geometry_key(kernel, X, eps) + "graph" if cache is not None else None
             item = cache.get(key) if cache is not None else None
@@ -171,6 +175,47 @@ def solve_resistance(payload: dict, precision: str, cache: FactorCache | None, s
     return _grand(X, center, x[:n].cpu().numpy(), payload.get("mu", 1.0))
 
 
+def solve_resistance_group(payloads: list, cache: FactorCache | None, stream, graphs) -> list:
+    """Six-mode grand resistances of several rigid bodies of one graph
+    bucket, as solve_resistance computes each alone (bit for bit): cached
+    geometries are solved on their resident factors, the others together in
+    one grouped graph launch.  Returns (R, None) or (None, error text) per
+    payload."""
+    from . import graphs as G
+    from .core.solve import rigid_modes
+
+    out: list = [None] * len(payloads)
+    todo = []
+    for i, p in enumerate(payloads):
+        X = p["points"]
+        modes = rigid_modes(X, p["center"])
+        key = geometry_key(p["kernel"], X, p["eps"]) + "graph" if cache is not None else None
+        item = cache.get(key) if cache is not None else None
+        if item is not None:
+            try:
+                x, _ = G.solve_resident(item[0], item[1], modes, steps=graphs.steps, stream=stream)
+                out[i] = (_grand(X, p["center"], x, p.get("mu", 1.0)), None)
+            except Exception as exc:  # reported per request
+                out[i] = (None, f"{type(exc).__name__}: {exc}")
+        else:
+            todo.append((i, key, modes))
+    if todo:
+        g = graphs.get_group(len(payloads[todo[0][0]]["points"]), stream)
+        probs = [(payloads[i]["kernel"], payloads[i]["points"], payloads[i]["eps"], modes) for i, _, modes in todo]
+        res = g.solve_many(probs, raise_errors=False)
+        for z, ((i, key, _), r) in enumerate(zip(todo, res)):
+            if isinstance(r, Exception):
+                out[i] = (None, f"{type(r).__name__}: {r}")
+                continue
+            x, _, F = r
+            if cache is not None:
+                K64p, Fp = g.resident(z, F)
+                cache.put(key, (K64p, Fp), K64p.numel() * 8 + Fp.nbytes())
+            p = payloads[i]
+            out[i] = (_grand(p["points"], p["center"], x, p.get("mu", 1.0)), None)
+    return out
+
+
 def new_session(spec: dict, precision: str):
     """The swimmer session a trace's swim_step requests advance (created at
     its first step)."""
@@ -194,10 +239,16 @@ class SolveServer:
     from the budget.  The graphs are captured here, in the constructing
     thread: build the server while no other thread of the process is
     using the GPU (a capture that overlaps other threads' runtime calls
-    can end the process)."""
+    can end the process).
+
+    With ``group`` > 1 (graphs on), a slot that takes a small resistance
+    request also takes up to group - 1 other waiting small resistance
+    requests of the same graph bucket (the earliest arrived) and solves them
+    together in one grouped graph launch (graphs.GroupSolveGraph); each
+    result is bit for bit the one it gets alone."""
 
     def __init__(self, slots: int = 4, precision: str = "mixed", budget_bytes: int | None = None,
-                 cache_bytes: int = 0, graphs: bool = True):
+                 cache_bytes: int = 0, graphs: bool = True, group: int = 1):
         t = D.torch()
         dev = D.device()
         if budget_bytes is None:
@@ -213,10 +264,15 @@ class SolveServer:
         # slot that takes its requests from a queue
         self.streams = [D.lease_stream() for _ in range(slots)]
         self.graphs = None
+        self.group = 1
         if graphs and precision == "mixed":
             from .graphs import GraphPool, pool_bytes
-            self.graphs = GraphPool(streams=self.streams)   # every capture here, before any worker thread
-            self.budget -= pool_bytes(slots)
+            self.group = max(1, int(group))
+            # every capture here, before any worker thread
+            self.graphs = GraphPool(streams=self.streams, members=self.group)
+            self.budget -= pool_bytes(slots, self.group)
+        from .graphs import GRAPH_MAX_POINTS
+        self.graphs_max_points = GRAPH_MAX_POINTS
         self._jobs = [queue.SimpleQueue() for _ in range(slots)]
         self._workers = [threading.Thread(target=self._worker, args=(sid,), daemon=True,
                                           name=f"fl-slot-{sid}") for sid in range(slots)]
@@ -300,23 +356,35 @@ class SolveServer:
                 busy.add(key)
                 heapq.heappush(waiting, (nxt.arrival, nxt.rid, nxt))
 
-        def worker_body(req: Request, sid: int, fp: int):
+        def worker_body(batch: list, sid: int, fp: int):
             start = now()
-            err, value = None, None
-            session = None
-            try:
-                if req.kind == "swim_step":
-                    key = req.payload["session"]
-                    if key not in sessions:
-                        sessions[key] = new_session(req.payload["spec"], self.precision)
-                    session = sessions[key]
-                value = self._execute(req, self.streams[sid], session)
-            except Exception as exc:  # a failing request is reported, the server keeps running
-                err = f"{type(exc).__name__}: {exc}"
+            req = batch[0]
+            if len(batch) > 1:
+                st = self.streams[sid]
+                try:
+                    t = D.torch()
+                    with t.cuda.stream(st):
+                        outs = solve_resistance_group([r.payload for r in batch], self.cache, st, self.graphs)
+                except Exception as exc:  # the whole group failed: each request reports it
+                    outs = [(None, f"{type(exc).__name__}: {exc}")] * len(batch)
+            else:
+                err, value = None, None
+                try:
+                    session = None
+                    if req.kind == "swim_step":
+                        key = req.payload["session"]
+                        if key not in sessions:
+                            sessions[key] = new_session(req.payload["spec"], self.precision)
+                        session = sessions[key]
+                    value = self._execute(req, self.streams[sid], session)
+                except Exception as exc:  # a failing request is reported, the server keeps running
+                    err = f"{type(exc).__name__}: {exc}"
+                outs = [(value, err)]
             end = now()
-            res = Result(req.rid, req.kind, value, req.arrival if realtime else 0.0, start, end, err)
+            done = [Result(r.rid, r.kind, v, r.arrival if realtime else 0.0, start, end, e)
+                    for r, (v, e) in zip(batch, outs)]
             with lock:
-                results.append(res)
+                results.extend(done)
                 state["in_use"] -= fp
                 state["running"] -= 1
                 free_streams.append(sid)
@@ -325,7 +393,33 @@ class SolveServer:
                     offer_next_step(req.payload["session"])
                 lock.notify_all()
             if on_result is not None:
-                on_result(res)
+                for res in done:
+                    on_result(res)
+
+        def groupable(r: Request) -> bool:
+            return (self.group > 1 and r.kind == "resistance"
+                    and len(r.payload["points"]) <= self.graphs_max_points)
+
+        def take_group(head: Request, fp: int):
+            # under lock: the head request and up to group - 1 waiting requests
+            # of its bucket, earliest first, while their footprints fit
+            from .graphs import bucket_rows
+            if not groupable(head):
+                return [head], fp
+            rows = bucket_rows(len(head.payload["points"]))
+            same = sorted(e for e in waiting if groupable(e[2]) and bucket_rows(len(e[2].payload["points"])) == rows)
+            batch = [head]
+            for e in same[:self.group - 1]:
+                f = footprint_bytes(e[2], self.precision, shared_cache=self.cache is not None)
+                if state["in_use"] + fp + f > self.budget:
+                    break
+                batch.append(e[2])
+                fp += f
+            if len(batch) > 1:
+                taken = {r.rid for r in batch[1:]}
+                waiting[:] = [e for e in waiting if e[2].rid not in taken]
+                heapq.heapify(waiting)
+            return batch, fp
 
         if not self._workers:
             raise RuntimeError("the server is closed")
@@ -357,10 +451,11 @@ class SolveServer:
                     if state["in_use"] + fp > self.budget and state["running"] > 0:
                         break  # wait for memory to free up
                     heapq.heappop(waiting)
+                    batch, fp = take_group(req, fp)
                     sid = free_streams.pop()
                     state["in_use"] += fp
                     state["running"] += 1
-                    self._jobs[sid].put(lambda req=req, sid=sid, fp=fp: worker_body(req, sid, fp))
+                    self._jobs[sid].put(lambda batch=batch, sid=sid, fp=fp: worker_body(batch, sid, fp))
                     admitted = True
                 if admitted:
                     continue

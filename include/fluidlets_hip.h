@@ -229,6 +229,29 @@ int fl_ssytrf_async(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, 
 int fl_dsytrf_async(double* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* Grouped small solves: one launch per kernel serves `count` problems of the
+ * same order.  Problem z's buffers (every pointer argument, its workspace
+ * included) are problem 0's plus z * slab_bytes (a multiple of 256): each
+ * problem lives in its own slab of one allocation.  Each problem's results are
+ * bit for bit those of the single call on its slab (the same kernels; each
+ * problem is computed by its own workgroups, blockIdx.z).  No counterpart in
+ * the reference, which solves one system per scipy.linalg.solve call
+ * (src/fluidlets/core/solve.py:66); they batch the many small resistance
+ * solves a server receives (the six-mode pattern of
+ * examples/translating_sphere/run.py:45-46).
+ *   fl_pad_solve_group     fl_pad_solve on each slab
+ *   fl_sgetrf_async_group  fl_sgetrf_async (FP32 products) on each slab;
+ *                          status per problem
+ *   fl_refine_steps_group  fl_refine_steps on each slab */
+int fl_pad_solve_group(double* K64, float* K32, int64_t n_pad, const double* n_dev, const double* rhs, double* b,
+                       int64_t nrhs, int64_t count, int64_t slab_bytes, void* stream);
+int fl_sgetrf_async_group(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, unsigned int* status,
+                          int64_t count, int64_t slab_bytes, void* workspace, size_t workspace_bytes, void* stream);
+int fl_refine_steps_group(const double* A64, int64_t lda64, const float* LU32, int64_t lda32, const void* aux32,
+                          int64_t n, const double* b, double* x, int64_t nrhs, int64_t ldb, int steps,
+                          double* resid_dev, int64_t count, int64_t slab_bytes, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /* Explicit inverse from the factors of fl_sgetrf: Ainv = A^{-1} =
  * U^{-1} L^{-1} P (n x n row-major, ldi >= n), by blocked substitution on the
  * permuted identity with the stored inverted diagonal blocks and the LU's
