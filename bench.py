@@ -515,6 +515,8 @@ def main():
     ap.add_argument("--serve-cache-gib", type=int, default=48, help="resident factor cache of the mixed server")
     ap.add_argument("--serve-nmax", type=int, default=6000)
     ap.add_argument("--serve-load", type=float, default=0.9, help="online arrival rate as a fraction of capacity")
+    ap.add_argument("--small-solves-only", action="store_true",
+                    help="of the serving section, run only the small-solve comparison")
     ap.add_argument("--no-gmres", action="store_true", help="skip the 131,072-point GMRES rod-carpet section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (the reference point for HBM byte counters)")
@@ -641,7 +643,10 @@ def main():
                                                 cpu_lapack=not args.no_cpu)
         progress("wall-sphere solves done")
 
-    if not args.no_serve:
+    if not args.no_serve and args.small_solves_only:
+        result["serving_mix"] = {"small_solves": small_solves_section()}
+        progress("small solves done")
+    elif not args.no_serve:
         result["serving_mix"] = serving_section(torch, args.serve_requests, args.serve_baseline_requests,
                                                 args.serve_swimmer_steps, args.serve_nmax,
                                                 cache_gib=args.serve_cache_gib, load=args.serve_load)

@@ -14,11 +14,13 @@
 //   Stokeslet   kernels/stokeslet3d.py:22-50
 //   Wall        kernels/images.py:26-89, free + image fused in one pass that
 //               shares d0, d1, d0^2 + d1^2 and the source tile
-//   Brinkmanlet kernels/brinkmanlet3d.py:52-104, with 1/z = rsqrt(Q)/alpha
-//               and the 24-term series below z = 0.5 in FP64
+//   Brinkmanlet kernels/brinkmanlet3d.py:52-104, with 1/z = rsqrt(Q)/alpha,
+//               e^{-z} from a table of 2^(j/64) in LDS and a degree-5
+//               polynomial, and the 24-term series below z = 0.5 in FP64
 #include "fl_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 namespace fl {
@@ -52,18 +54,6 @@ constexpr SeriesCoef make_series()
 
 __constant__ SeriesCoef kSeries = make_series();
 
-__device__ __forceinline__ void brinkman_series(double z, double& f1, double& f2)
-{
-    double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-    for (int n = NTERMS - 1; n >= 0; --n) {
-        s1 = s1 * z + kSeries.c1[n];
-        s2 = s2 * z + kSeries.c2[n];
-    }
-    f1 = s1;
-    f2 = s2;
-}
-
 // 1/sqrt(x) for finite x > 0 (here x = r^2 + eps^2 >= eps^2 > 0): the
 // hardware estimate y0 (v_rsq_f64) refined by one second-order step of the
 // expansion (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + ..., e = 1 - x y0^2, which
@@ -73,6 +63,93 @@ __device__ __forceinline__ double rsqrt_pos(double x)
     double y0 = __builtin_amdgcn_rsq(x);
     double e = fma(-x * y0, y0, 1.0);
     return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
+// 2^(j/64), j = 0..63, each the double nearest the exact value (computed to
+// 60 digits and rounded once).
+__constant__ double kExp2Frac[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+};
+
+// LDS of a Brinkmanlet kernel: the table of exp_neg (64 doubles; the lanes
+// of a wave index it at different j, which LDS serves in one pass) and the
+// series coefficients c1, c2 (read only on the rare near-pair path, so they
+// hold no registers across the pair loop).
+constexpr int BRINK_LDS = 64 + 2 * NTERMS + 1;
+constexpr int BRINK_C24 = 64 + 2 * NTERMS;   // 1/24, the exp polynomial's coefficient kept in a vector register
+
+__device__ __forceinline__ void stage_brinkman(double* bl)
+{
+    const int i = threadIdx.x;
+    if (i < 64) bl[i] = kExp2Frac[i];
+    if (i < NTERMS) {
+        bl[64 + i] = kSeries.c1[i];
+        bl[64 + NTERMS + i] = kSeries.c2[i];
+    }
+    if (i == 0) bl[BRINK_C24] = 1.0 / 24.0;
+}
+
+// a * b + c, one v_fma_f64 with b from a scalar and c from a vector register
+// (the compiler otherwise spends a register copy on each fused multiply-add
+// whose loop-invariant addend it would overwrite).
+__device__ __forceinline__ double fma_sv(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+
+__device__ __forceinline__ double fma_vs(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
+// e^{-z} for z >= 0, from the LDS block etab of a Brinkmanlet kernel
+// (etab[j] = 2^(j/64), etab[BRINK_C24] = 1/24): with
+// m = rint(-64 z / ln 2) = 64 n + j (0 <= j < 64, n = floor(m / 64)),
+//   e^{-z} = 2^n * 2^(j/64) * e^r,   r = -z - m ln2/64,  |r| <= ln2/128,
+// where m ln2/64 is taken with ln2/64 split in a high and a low part (the
+// two fused multiply-adds leave r exact to about 1e-19), and e^r is its
+// degree-5 Taylor polynomial (the first omitted term r^6/720 < 4e-17).  The
+// result is within about 2 ulp.  m is rounded by adding 1.5 * 2^52 (the sum's
+// low word is then m as a two's-complement integer).  z is clamped at 1000,
+// where e^{-z} underflows to 0 anyway.
+__device__ __forceinline__ double exp_neg(double z, const double* etab)
+{
+    constexpr double kScale = -92.33248261689366;       // -64 / ln 2
+    constexpr double kLhi = 0.010830424696249145;       // ln 2 / 64, high part
+    constexpr double kLlo = 3.623510646634843e-19;      // ln 2 / 64 - kLhi
+    constexpr double kShift = 6755399441055744.0;       // 1.5 * 2^52
+    asm("v_min_f64 %0, %1, %2" : "=v"(z) : "v"(z), "s"(1000.0));   // z is never NaN here
+    const double ms = fma(z, kScale, kShift);
+    const double m = ms - kShift;
+    const int mi = __double2loint(ms);
+    double r = fma(m, -kLhi, -z);
+    r = fma(m, -kLlo, r);
+    const double T = etab[mi & 63];
+    double p = fma_sv(r, 1.0 / 120.0, etab[BRINK_C24]);
+    p = fma_vs(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(T * p, mi >> 6);
 }
 
 // ---------------------------------------------------------------- FP64 pairs
@@ -161,33 +238,93 @@ __device__ __forceinline__ void pair_wall(double d0, double d1, double z, double
     }
 }
 
-template <int NF>
-__device__ __forceinline__ void pair_brinkman(double d0, double d1, double d2, double e2, double alpha,
-                                              double inv_alpha, const double* f, double* acc)
+// Brinkmanlet block G = H1 I + H2 d d^T (kernels/brinkmanlet3d.py:52-104):
+// with z = alpha sqrt(Q), Q = r^2 + e2, rs = Q^{-1/2},
+//   H2 = b2(z) rs^3,  H1 = b1(z) rs + e2 H2,
+//   b1 = e^{-z}(1 + 1/z + 1/z^2) - 1/z^2,  b2 = 3/z^2 - e^{-z}(1 + 3/z + 3/z^2)
+//      = 2 e^{-z} - 3 b1.
+// The FP64 kernels work in coordinates scaled by alpha (x' = alpha x,
+// e2' = alpha^2 e2; alpha > 0): then Q' = alpha^2 Q, u = Q'^{-1/2} = 1/z,
+// z = Q' u, and G f = alpha [H1' f + H2' (d'.f) d'] with
+//   H2' = b2 u^3,  H1' = b1 u + e2' H2',
+// the factor alpha applied once per target with the kernel's scale.
+// brinkman_radial evaluates the closed form for every pair (straight-line
+// code, so the pairs of several targets interleave) and returns z; pairs with
+// z < 0.5, where the closed form cancels, are then redone by
+// brinkman_radial_near from the 24-term series (a branch that is rarely taken:
+// such pairs are closer than 0.5 / alpha).  bl is the kernel's LDS block.
+__device__ __forceinline__ double brinkman_radial(double Q, double u, double e2, const double* bl, double& H1,
+                                                  double& H2)
 {
-    double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    double Q = r2 + e2;
-    double rs = rsqrt_pos(Q);
-    double z = alpha * (Q * rs);
-    double f1, f2;
-    if (z < 0.5) {
-        brinkman_series(z, f1, f2);
-    } else {
-        double E = exp(-z);
-        double iz = rs * inv_alpha;
-        double iz2 = iz * iz;
-        f1 = E * (1.0 + iz + iz2) - iz2;
-        f2 = 3.0 * iz2 - E * (1.0 + 3.0 * iz + 3.0 * iz2);
-    }
-    double h2 = f2 * (rs * rs * rs);
-    double H1 = f1 * rs + e2 * h2;
+    const double z = Q * u;
+    const double E = exp_neg(z, bl);
+    const double v = u * u;
+    const double b1 = fma(E, fma(u, u, u + 1.0), -v);
+    const double b2 = fma(-3.0, b1, E + E);
+    H2 = b2 * (u * v);
+    H1 = fma(b1, u, e2 * H2);
+    return z;
+}
+
+// b1(z), b2(z) from their Taylor series with the coefficients in LDS.
+__device__ __forceinline__ void brinkman_series_lds(double z, const double* bl, double& b1, double& b2)
+{
+    int k = 0;
+    asm volatile("" : "+s"(k));   // the coefficient reads stay inside the calling branch
+    const double* c1 = bl + 64 + k;
+    const double* c2 = c1 + NTERMS;
+    b1 = 0.0;
+    b2 = 0.0;
 #pragma unroll
-    for (int r = 0; r < NF; ++r) {
-        double fd = f[3 * r] * d0 + f[3 * r + 1] * d1 + f[3 * r + 2] * d2;
-        double t = h2 * fd;
-        acc[3 * r] += H1 * f[3 * r] + t * d0;
-        acc[3 * r + 1] += H1 * f[3 * r + 1] + t * d1;
-        acc[3 * r + 2] += H1 * f[3 * r + 2] + t * d2;
+    for (int n = NTERMS - 1; n >= 0; --n) {
+        b1 = fma(b1, z, c1[n]);
+        b2 = fma(b2, z, c2[n]);
+    }
+}
+
+__device__ __forceinline__ void brinkman_radial_near(double z, double u, double e2, const double* bl, double& H1,
+                                                     double& H2)
+{
+    double b1, b2;
+    brinkman_series_lds(z, bl, b1, b2);
+    H2 = b2 * (u * u * u);
+    H1 = fma(b1, u, e2 * H2);
+}
+
+// The Brinkmanlet pairs of TPL targets and one source (strengths f, NF
+// sets), all in alpha-scaled coordinates.
+template <int NF, int TPL>
+__device__ __forceinline__ void pairs_brinkman(const double* tx, const double* ty, const double* tz, const double* r,
+                                               const double* bl, const double* f, double (*acc)[3 * NF])
+{
+    const double e2 = r[3];
+    double d[TPL][3], H1[TPL], H2[TPL], z[TPL], rs[TPL];
+    bool near = false;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        d[t][0] = tx[t] - r[0];
+        d[t][1] = ty[t] - r[1];
+        d[t][2] = tz[t] - r[2];
+        const double Q = fma(d[t][0], d[t][0], fma(d[t][1], d[t][1], fma(d[t][2], d[t][2], e2)));
+        rs[t] = rsqrt_pos(Q);
+        z[t] = brinkman_radial(Q, rs[t], e2, bl, H1[t], H2[t]);
+        near |= z[t] < 0.5;
+    }
+    if (near) {
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+            if (z[t] < 0.5) brinkman_radial_near(z[t], rs[t], e2, bl, H1[t], H2[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const double* fq = f + 3 * q;
+            const double u = H2[t] * fma(d[t][0], fq[0], fma(d[t][1], fq[1], d[t][2] * fq[2]));
+            acc[t][3 * q] = fma(H1[t], fq[0], fma(u, d[t][0], acc[t][3 * q]));
+            acc[t][3 * q + 1] = fma(H1[t], fq[1], fma(u, d[t][1], acc[t][3 * q + 1]));
+            acc[t][3 * q + 2] = fma(H1[t], fq[2], fma(u, d[t][2], acc[t][3 * q + 2]));
+        }
     }
 }
 
@@ -255,7 +392,8 @@ __device__ __forceinline__ void pair_wall32(float d0, float d1, float dzf, float
 
 template <int NF>
 __device__ __forceinline__ void pair_brinkman32(double dd0, double dd1, double dd2, float e2, double e2d, float alpha,
-                                                float inv_alpha, double alpha_d, const float* f, float* acc)
+                                                float inv_alpha, double alpha_d, const double* bl, const float* f,
+                                                float* acc)
 {
     float d0 = (float)dd0, d1 = (float)dd1, d2 = (float)dd2;
     float r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -268,7 +406,7 @@ __device__ __forceinline__ void pair_brinkman32(double dd0, double dd1, double d
         double Qd = dd0 * dd0 + dd1 * dd1 + dd2 * dd2 + e2d;
         double rsd = rsqrt_pos(Qd);
         double f1, f2;
-        brinkman_series(alpha_d * (Qd * rsd), f1, f2);
+        brinkman_series_lds(alpha_d * (Qd * rsd), bl, f1, f2);
         double h2d = f2 * (rsd * rsd * rsd);
         H1 = (float)(f1 * rsd + e2d * h2d);
         h2 = (float)h2d;
@@ -358,17 +496,21 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
     constexpr int W = Src64<KIND, NF>::W;
     constexpr int FO = Src64<KIND, NF>::FOFF;
     __shared__ __attribute__((aligned(16))) double lds[TILE * W];
+    __shared__ double bl[BRINK_LDS];
     const int tid = threadIdx.x;
     const int64_t M = a.M, N = a.N;
+    if (KIND == FL_BRINKMANLET3D) stage_brinkman(bl);   // read after the tile loop's first barrier
 
+    // the Brinkmanlet pairs run in alpha-scaled coordinates (brinkman_radial)
+    const double sc = KIND == FL_BRINKMANLET3D ? a.alpha : 1.0;
     double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
         int64_t i = (int64_t)blockIdx.x * WG * TPL + t * WG + tid;
         int64_t ic = i < M ? i : M - 1;
-        tx[t] = a.tgt[ic];
-        ty[t] = a.tgt[M + ic];
-        tz[t] = a.tgt[2 * M + ic];
+        tx[t] = sc * a.tgt[ic];
+        ty[t] = sc * a.tgt[M + ic];
+        tz[t] = sc * a.tgt[2 * M + ic];
 #pragma unroll
         for (int k = 0; k < 3 * NF; ++k) acc[t][k] = 0.0;
     }
@@ -381,8 +523,8 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
 #pragma unroll
         for (int k = 0; k < W; ++k) rec[k] = 0.0;
         if (j < je) {
-            const double e = a.eps[j * a.eps_stride];
-            fill_src64<KIND, NF>(rec, a.src[j], a.src[N + j], a.src[2 * N + j], e * e);
+            const double e = sc * a.eps[j * a.eps_stride];
+            fill_src64<KIND, NF>(rec, sc * a.src[j], sc * a.src[N + j], sc * a.src[2 * N + j], e * e);
 #pragma unroll
             for (int k = 0; k < 3 * NF; ++k)
                 rec[FO + k] = k < 3 ? a.f[(int64_t)k * N + j] : a.fi[(int64_t)(k - 3) * N + j];
@@ -401,16 +543,17 @@ __global__ __launch_bounds__(WG) void nbody64(NbArgs a)
                 r[k] = v.x;
                 r[k + 1] = v.y;
             }
+            if constexpr (KIND == FL_BRINKMANLET3D) {
+                pairs_brinkman<NF, TPL>(tx, ty, tz, r, bl, r + FO, acc);
+            } else {
 #pragma unroll
-            for (int t = 0; t < TPL; ++t) {
-                if (KIND == FL_STOKESLET3D)
-                    pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + FO, acc[t]);
-                else if (KIND == FL_WALL_STOKESLET3D)
-                    pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t], r[2], r[3], r[4], r[5], r[6], r + FO,
-                                  acc[t]);
-                else
-                    pair_brinkman<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], a.alpha, a.inv_alpha, r + FO,
+                for (int t = 0; t < TPL; ++t) {
+                    if (KIND == FL_STOKESLET3D)
+                        pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + FO, acc[t]);
+                    else
+                        pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t], r[2], r[3], r[4], r[5], r[6], r + FO,
                                       acc[t]);
+                }
             }
         }
     }
@@ -434,9 +577,11 @@ template <int KIND, int NF, int TPL>
 __global__ __launch_bounds__(WG) void nbody32c(NbArgs a)
 {
     __shared__ Src32<NF> lds[TILE];
+    __shared__ double bl[BRINK_LDS];
     const int tid = threadIdx.x;
     const int64_t M = a.M, N = a.N;
     const float alpha = (float)a.alpha, inv_alpha = (float)a.inv_alpha;
+    if (KIND == FL_BRINKMANLET3D) stage_brinkman(bl);   // read after the tile loop's first barrier
 
     double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
 #pragma unroll
@@ -485,7 +630,8 @@ __global__ __launch_bounds__(WG) void nbody32c(NbArgs a)
                     pair_wall32<NF>((float)dd0, (float)dd1, (float)dd2, (float)(tz[t] + r.z), r.e2, (float)r.z,
                                     r.hh, r.f, part[t]);
                 else
-                    pair_brinkman32<NF>(dd0, dd1, dd2, r.e2, (double)r.e2, alpha, inv_alpha, a.alpha, r.f, part[t]);
+                    pair_brinkman32<NF>(dd0, dd1, dd2, r.e2, (double)r.e2, alpha, inv_alpha, a.alpha, bl, r.f,
+                                        part[t]);
             }
         }
 #pragma unroll
@@ -569,10 +715,55 @@ __device__ __forceinline__ int64_t sym_pair_index(int64_t K, int64_t L, int64_t 
 
 // Block of (target, source) applied to the source strength f (row) and, when
 // COL, its transpose applied to the target strength g (added to col).
+// H1 I + H2 d d^T applied to the source strength f (row) and, when COL, to
+// the target strength g (added to col; the block is symmetric).
+template <bool COL>
+__device__ __forceinline__ void sym_apply(double d0, double d1, double d2, double H1, double H2, const double* f,
+                                          const double* g, double* acc, double* col)
+{
+    const double t = H2 * fma(d0, f[0], fma(d1, f[1], d2 * f[2]));
+    acc[0] = fma(H1, f[0], fma(t, d0, acc[0]));
+    acc[1] = fma(H1, f[1], fma(t, d1, acc[1]));
+    acc[2] = fma(H1, f[2], fma(t, d2, acc[2]));
+    if (COL) {
+        const double tg = H2 * fma(d0, g[0], fma(d1, g[1], d2 * g[2]));
+        col[0] = fma(H1, g[0], fma(tg, d0, col[0]));
+        col[1] = fma(H1, g[1], fma(tg, d1, col[1]));
+        col[2] = fma(H1, g[2], fma(tg, d2, col[2]));
+    }
+}
+
+// The Brinkmanlet blocks of the lane's two targets and one source record r.
+template <bool COL>
+__device__ __forceinline__ void sym_brinkman2(const double* tx, const double* ty, const double* tz, const double* r,
+                                              const double (*g)[3], const double* bl, double (*acc)[3], double* col)
+{
+    constexpr int FO = Src64<FL_BRINKMANLET3D, 1>::FOFF;
+    const double e2 = r[3];
+    double d[2][3], H1[2], H2[2], z[2], rs[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        d[t][0] = tx[t] - r[0];
+        d[t][1] = ty[t] - r[1];
+        d[t][2] = tz[t] - r[2];
+        const double Q = fma(d[t][0], d[t][0], fma(d[t][1], d[t][1], fma(d[t][2], d[t][2], e2)));
+        rs[t] = rsqrt_pos(Q);
+        z[t] = brinkman_radial(Q, rs[t], e2, bl, H1[t], H2[t]);
+    }
+    if (z[0] < 0.5 || z[1] < 0.5) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+            if (z[t] < 0.5) brinkman_radial_near(z[t], rs[t], e2, bl, H1[t], H2[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) sym_apply<COL>(d[t][0], d[t][1], d[t][2], H1[t], H2[t], r + FO, g[t], acc[t], col);
+}
+
 template <int KIND, bool COL>
 __device__ __forceinline__ void sym_pair(double d0, double d1, double d2, double z, const double* r,
-                                         const double* g, double alpha, double inv_alpha, double* acc, double* col)
+                                         const double* g, double* acc, double* col)
 {
+    static_assert(KIND != FL_BRINKMANLET3D, "Brinkmanlet pairs go through sym_brinkman2");
     constexpr int FO = Src64<KIND, 1>::FOFF;
     const double f0 = r[FO], f1 = r[FO + 1], f2 = r[FO + 2];
     if (KIND == FL_WALL_STOKESLET3D) {
@@ -590,40 +781,12 @@ __device__ __forceinline__ void sym_pair(double d0, double d1, double d2, double
             col[2] = fma(B.g, wg, fma(B.Z, g[2], col[2]));
         }
     } else {
-        // H1 I + H2 d d^T (symmetric)
-        double H1, H2;
+        // Stokeslet: H1 I + H2 d d^T (symmetric)
         const double e2 = r[3];
         const double Q = fma(d0, d0, fma(d1, d1, fma(d2, d2, e2)));
         const double rs = rsqrt_pos(Q);
         const double s3 = rs * rs * rs;
-        if (KIND == FL_STOKESLET3D) {
-            H2 = s3;
-            H1 = (Q + e2) * s3;
-        } else {
-            const double zb = alpha * (Q * rs);
-            double b1, b2;
-            if (zb < 0.5) {
-                brinkman_series(zb, b1, b2);
-            } else {
-                const double E = exp(-zb);
-                const double iz = rs * inv_alpha;
-                const double iz2 = iz * iz;
-                b1 = E * (1.0 + iz + iz2) - iz2;
-                b2 = 3.0 * iz2 - E * (1.0 + 3.0 * iz + 3.0 * iz2);
-            }
-            H2 = b2 * s3;
-            H1 = fma(b1, rs, e2 * H2);
-        }
-        const double t = H2 * fma(d0, f0, fma(d1, f1, d2 * f2));
-        acc[0] = fma(H1, f0, fma(t, d0, acc[0]));
-        acc[1] = fma(H1, f1, fma(t, d1, acc[1]));
-        acc[2] = fma(H1, f2, fma(t, d2, acc[2]));
-        if (COL) {
-            const double tg = H2 * fma(d0, g[0], fma(d1, g[1], d2 * g[2]));
-            col[0] = fma(H1, g[0], fma(tg, d0, col[0]));
-            col[1] = fma(H1, g[1], fma(tg, d1, col[1]));
-            col[2] = fma(H1, g[2], fma(tg, d2, col[2]));
-        }
+        sym_apply<COL>(d0, d1, d2, (Q + e2) * s3, s3, r + FO, g, acc, col);
     }
 }
 
@@ -635,22 +798,26 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
     __shared__ __attribute__((aligned(16))) double lds[WG * W];          // 256 sources at a time
     __shared__ double red[4][SBATCH * 3 * SRS];  // per wave: [source][comp][lane], rows padded to SRS
     __shared__ double colw[4][64 * 3];                                   // per wave: [source][comp]
+    __shared__ double bl[BRINK_LDS];
     const int K = blockIdx.x, q = blockIdx.y;
     const int L0 = K + q * a.c;
     if (L0 >= a.T) return;
     const int L1 = min(a.T, L0 + a.c);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (KIND == FL_BRINKMANLET3D) stage_brinkman(bl);   // read after the tile loop's first barrier
     const int64_t N = a.N;
-    const double e2 = a.eps[0] * a.eps[0];
+    // the Brinkmanlet pairs run in alpha-scaled coordinates (brinkman_radial)
+    const double sc = KIND == FL_BRINKMANLET3D ? a.alpha : 1.0;
+    const double e2 = (sc * a.eps[0]) * (sc * a.eps[0]);
 
     double tx[2], ty[2], tz[2], g[2][3], acc[2][3];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int64_t i = (int64_t)K * SB + t * WG + tid;
         const int64_t ic = i < N ? i : N - 1;
-        tx[t] = a.x[ic];
-        ty[t] = a.x[N + ic];
-        tz[t] = a.x[2 * N + ic];
+        tx[t] = sc * a.x[ic];
+        ty[t] = sc * a.x[N + ic];
+        tz[t] = sc * a.x[2 * N + ic];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             g[t][c] = i < N ? a.f[c * N + i] : 0.0;
@@ -668,7 +835,7 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
                 double rec[W];
 #pragma unroll
                 for (int k = 0; k < W; ++k) rec[k] = 0.0;
-                fill_src64<KIND, 1>(rec, a.x[jc], a.x[N + jc], a.x[2 * N + jc], e2);
+                fill_src64<KIND, 1>(rec, sc * a.x[jc], sc * a.x[N + jc], sc * a.x[2 * N + jc], e2);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) rec[FO + k] = j < N ? a.f[(int64_t)k * N + j] : 0.0;
                 __syncthreads();
@@ -686,10 +853,14 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
                         r[k] = v.x;
                         r[k + 1] = v.y;
                     }
+                    if constexpr (KIND == FL_BRINKMANLET3D) {
+                        sym_brinkman2<false>(tx, ty, tz, r, g, bl, acc, nullptr);
+                    } else {
 #pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        sym_pair<KIND, false>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], r, g[t],
-                                              a.alpha, a.inv_alpha, acc[t], nullptr);
+                        for (int t = 0; t < 2; ++t)
+                            sym_pair<KIND, false>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], r, g[t], acc[t],
+                                                  nullptr);
+                    }
                 }
                 continue;
             }
@@ -706,10 +877,14 @@ __global__ __launch_bounds__(WG) void nbody64_sym(SymArgs a)
                             r[k + 1] = v.y;
                         }
                         double col[3] = {0.0, 0.0, 0.0};
+                        if constexpr (KIND == FL_BRINKMANLET3D) {
+                            sym_brinkman2<true>(tx, ty, tz, r, g, bl, acc, col);
+                        } else {
 #pragma unroll
-                        for (int t = 0; t < 2; ++t)
-                            sym_pair<KIND, true>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], r, g[t],
-                                                 a.alpha, a.inv_alpha, acc[t], col);
+                            for (int t = 0; t < 2; ++t)
+                                sym_pair<KIND, true>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], tz[t], r, g[t],
+                                                     acc[t], col);
+                        }
 #pragma unroll
                         for (int c = 0; c < 3; ++c) red[wave][(b * 3 + c) * SRS + lane] = col[c];
                     }
@@ -902,6 +1077,23 @@ int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt,
         return FL_OK;
     }
     FL_CHECK(tgt && src && eps && f && u_re, "null pointer argument");
+    FL_CHECK(kind != FL_BRINKMANLET3D || (alpha >= 0.0 && std::isfinite(alpha)), "alpha must be finite and >= 0");
+    if (kind == FL_BRINKMANLET3D && precision == FL_FP64) {
+        if (alpha < 1e-150) {
+            // the Brinkmanlet at alpha = 0 is the Stokeslet: b1 = b2 = 1/2 at z = 0,
+            // so its bracket is half the Stokeslet's (exact in the scale); below
+            // 1e-150 (where scaled coordinates would underflow) it differs from
+            // it by O(alpha r), far below rounding
+            kind = FL_STOKESLET3D;
+            w_re *= 0.5;
+            w_im *= 0.5;
+        } else {
+            // the FP64 Brinkmanlet kernels sum in alpha-scaled coordinates:
+            // the sums come out divided by alpha
+            w_re *= alpha;
+            w_im *= alpha;
+        }
+    }
     if (tgt == src && M == N && eps_stride == 0 && nf == 1 && precision == FL_FP64) {
         const SymPlan p = sym_plan(N);
         if (p.ok && ws && ws_bytes >= p.bytes) {
