@@ -609,8 +609,6 @@ def main():
         x_copy = x.clone()  # the same points as a separate buffer: the general (non-reciprocal) path
         for name, K, mu in cases:
             for prec in ["fp64", "fp64_general", "fp32c"]:
-                if prec == "fp64_general" and name != "wall":
-                    continue
                 src_t = x_copy if prec == "fp64_general" else x
                 p_arg = "fp64" if prec == "fp64_general" else prec
                 evaluate_device(K, x, src_t, eps_t, stride, f, mu=mu, precision=p_arg)

@@ -30,6 +30,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -1696,7 +1698,7 @@ int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, co
                     void* split, size_t split_bytes, hipStream_t st, bool lower = false)
 {
     if constexpr (std::is_same_v<T, float>) {
-        if (split && M >= 2048 && N >= 2048 && K <= NBO_SPLIT)
+        if (split && M >= 2048 && N >= 256 && K <= NBO_SPLIT)
             return split3_update(M, N, K, A, lda, B, ldb, C, ldc, lower ? 1 : 0, split, split_bytes, st);
     }
     if (lower) return gemm_lower_update<T>(M, K, A, lda, B, ldb, C, ldc, st);
@@ -1734,6 +1736,62 @@ int sym_u_launch(T* A, int64_t lda, int64_t r0, int h, int64_t c0, int64_t c1, h
     FL_LAUNCHED();
     return FL_OK;
 }
+
+// ---------------------------------------------------------------- look-ahead
+// Orders of at least LOOKAHEAD_MIN factor with look-ahead (FL_LU_LOOKAHEAD=0
+// turns it off, a measurement switch).
+constexpr int64_t LOOKAHEAD_MIN = 4096;
+
+bool sym_or_lu_lookahead(int64_t n)
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FL_LU_LOOKAHEAD");
+        return !(e && *e == '0');
+    }();
+    return on && n >= LOOKAHEAD_MIN;
+}
+
+// A stream being captured into a graph keeps the single-stream order.
+bool stream_capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// The look-ahead stream of (device, caller stream): created once, at the
+// default priority, kept for the process.  Callers that share a stream share
+// it too, which only serializes their panels (each call orders its work with
+// its own events).
+hipStream_t lookahead_stream(hipStream_t caller)
+{
+    static std::mutex mu;
+    static std::vector<std::tuple<int, hipStream_t, hipStream_t>> pool;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& [d, c, a] : pool)
+        if (d == dev && c == caller) return a;
+    hipStream_t aux = nullptr;
+    if (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    pool.emplace_back(dev, caller, aux);
+    return aux;
+}
+
+// Events of one factorization (released when it ends; the device finishes
+// with them on its own).
+struct EventSet {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    int create()
+    {
+        for (auto& x : e) FL_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        return FL_OK;
+    }
+    ~EventSet()
+    {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
 
 // Workspace: the sub-panel backup (n x NB), flags, and for FP32 the two
 // split-bf16 operand buffers of the trailing update.
@@ -1831,7 +1889,23 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         return (int64_t)((v == 256 || v == 512) ? v : 0);
     }();
     const int64_t nbo_w = split ? NBO_SPLIT : (nbo_env ? nbo_env : NBO);
-    for (int64_t k = 0; k < n; k += nbo_w) {
+    // Look-ahead (asynchronous path, n >= LOOKAHEAD_MIN): while the caller's
+    // stream updates the trailing matrix beyond the next outer panel, the next
+    // panel (updated first) is factored on a second stream, so its diagonal
+    // blocks and panel products overlap the bulk of the update.  Events order
+    // the two streams; each touches disjoint parts of A in the meantime.
+    const bool ahead = spec && sym_or_lu_lookahead(n) && batch().count == 1 && !stream_capturing(st);
+    hipStream_t sp = st;
+    EventSet ev;
+    if (ahead) {
+        if (int s = ev.create()) return s;
+        sp = lookahead_stream(st);
+        if (!sp) return set_error(FL_EHIP, "could not create the look-ahead stream");
+        FL_HIP(hipEventRecord(ev.e[0], st));
+        FL_HIP(hipStreamWaitEvent(sp, ev.e[0], 0));
+    }
+    auto factor_panel = [&](int64_t k) -> int {
+        hipStream_t st = sp;   // the panel's work runs on the look-ahead stream when there is one
         const int64_t pe = std::min<int64_t>(n, k + nbo_w);
         for (int64_t ks = k; ks < pe; ks += NB) {
             const int nb = (int)std::min<int64_t>(NB, pe - ks);
@@ -1937,18 +2011,26 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                         return s;
             }
         }
+        return FL_OK;
+    };
+    if (int s = factor_panel(0)) return s;
+    for (int64_t k = 0; k < n; k += nbo_w) {
+        const int64_t pe = std::min<int64_t>(n, k + nbo_w);
+        if (ahead) {   // the panel's factors, before anything reads them
+            FL_HIP(hipEventRecord(ev.e[1], sp));
+            FL_HIP(hipStreamWaitEvent(st, ev.e[1], 0));
+        }
         const int nbo = (int)(pe - k);
         // the outer panel's interchanges to its left and right
         if (int s = swap_cols(k, nbo, 0, k)) return s;
         if (int s = swap_cols(k, nbo, pe, n)) return s;
         const int64_t nr = n - pe;
-        if (nr > 0 && sym) {
-            // U12 = D L21^T for the whole outer panel, then the lower half of the update
+        if (nr <= 0) break;
+        const size_t sb = split ? split_ws_bytes(n) : 0;
+        if (sym) {
+            // U12 = D L21^T for the whole outer panel
             if (int s = sym_u_launch<T>(A, lda, k, nbo, pe, n, st)) return s;
-            if (int s = trailing_update<T>(nr, nr, nbo, A + pe * lda + k, lda, A + k * lda + pe, lda, A + pe * lda + pe,
-                                           lda, split, split ? split_ws_bytes(n) : 0, st, true))
-                return s;
-        } else if (nr > 0) {
+        } else {
             // U12 = inv(L11) A12 block row by block row
             for (int64_t ks = k; ks < pe; ks += NB) {
                 const int nb = (int)std::min<int64_t>(NB, pe - ks);
@@ -1962,10 +2044,34 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
                                                A + (ks + nb) * lda + pe, lda, nullptr, st))
                         return s;
             }
-            // trailing update with K = nbo_w
-            if (int s = trailing_update<T>(nr, nr, nbo, A + pe * lda + k, lda, A + k * lda + pe, lda, A + pe * lda + pe,
-                                           lda, split, split ? split_ws_bytes(n) : 0, st))
+        }
+        T* L21 = A + pe * lda + k;   // rows pe .. n of the panel's L
+        const T* U12 = A + k * lda;  // row block k .. pe of U (column offset added below)
+        if (!ahead) {
+            // trailing update with K = nbo (symmetric: the tiles on or below the diagonal)
+            if (int s = trailing_update<T>(nr, nr, nbo, L21, lda, U12 + pe, lda, A + pe * lda + pe, lda, split, sb, st, sym))
                 return s;
+            if (int s = factor_panel(pe)) return s;
+            continue;
+        }
+        // (a) the next panel's columns pe .. pe2, all rows from pe; then that
+        // panel is factored on the look-ahead stream while (b) updates the rest
+        const int64_t pe2 = std::min<int64_t>(n, pe + nbo_w);
+        if (int s = trailing_update<T>(nr, pe2 - pe, nbo, L21, lda, U12 + pe, lda, A + pe * lda + pe, lda, split, sb, st))
+            return s;
+        FL_HIP(hipEventRecord(ev.e[2], st));
+        FL_HIP(hipStreamWaitEvent(sp, ev.e[2], 0));
+        if (int s = factor_panel(pe)) return s;
+        if (pe2 < n) {
+            if (sym) {
+                const int64_t r2 = n - pe2;
+                if (int s = trailing_update<T>(r2, r2, nbo, A + pe2 * lda + k, lda, U12 + pe2, lda, A + pe2 * lda + pe2,
+                                               lda, split, sb, st, true))
+                    return s;
+            } else if (int s = trailing_update<T>(nr, n - pe2, nbo, L21, lda, U12 + pe2, lda, A + pe * lda + pe2, lda,
+                                                  split, sb, st)) {
+                return s;
+            }
         }
     }
     // the chained solves' links, from the finished factors
