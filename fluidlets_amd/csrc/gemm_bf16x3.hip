@@ -29,6 +29,12 @@
 //
 // Optional tile filter: only the tiles on or below the diagonal (the lower
 // triangle of C, for the symmetric factorization's updates).
+//
+// The same product kernel also runs the update with FP32 products
+// (v_mfma_f32_32x32x2_f32): a fragment then holds one FP32 MFMA operand for
+// four k steps (64 lanes x 4 floats = 1 KB, lane l: row 32 rb + l % 32 (A) or
+// column 32 cb + l % 32 (B), k = 8 kb + l / 32 + 2 s for s = 0 .. 3), one
+// fragment per block and k block of 8.
 #include "fl_internal.h"
 
 #include <algorithm>
@@ -39,6 +45,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int FE = 32;          // fragment edge: rows of an A block, columns of a B block
 constexpr int FK = 16;          // k per block
@@ -121,6 +128,42 @@ __global__ __launch_bounds__(256) void pack_b_kernel(const float* B, int64_t ldb
     uint4* o = reinterpret_cast<uint4*>(out + (f * 2) * FRAG) + l;
     o[0] = p.hi;
     o[64] = p.lo;
+}
+
+// FP32 fragments: A (M x K, lda) -> MBp row blocks x KB k blocks of 8.
+__global__ __launch_bounds__(256) void pack_a_f32_kernel(const float* A, int64_t lda, int64_t M, int K, int64_t MBp,
+                                                         int KB, float4* out)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= MBp * KB * 64) return;
+    const int l = (int)(g & 63);
+    const int64_t f = g >> 6;
+    const int64_t rb = f / KB;
+    const int kb = (int)(f % KB);
+    const int64_t row = rb * FE + (l & 31);
+    const int k0 = kb * 8 + (l >> 5);
+    float x[4];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) x[s2] = (row < M && k0 + 2 * s2 < K) ? A[row * lda + k0 + 2 * s2] : 0.0f;
+    out[f * 64 + l] = make_float4(x[0], x[1], x[2], x[3]);
+}
+
+// FP32 fragments: B (K x N, ldb) -> NBp column blocks x KB k blocks of 8.
+__global__ __launch_bounds__(256) void pack_b_f32_kernel(const float* B, int64_t ldb, int64_t N, int K, int64_t NBp,
+                                                         int KB, float4* out)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= NBp * KB * 64) return;
+    const int l = (int)(g & 63);
+    const int64_t f = g >> 6;
+    const int64_t cb = f / KB;
+    const int kb = (int)(f % KB);
+    const int64_t col = cb * FE + (l & 31);
+    const int k0 = kb * 8 + (l >> 5);
+    float x[4];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) x[s2] = (col < N && k0 + 2 * s2 < K) ? B[(int64_t)(k0 + 2 * s2) * ldb + col] : 0.0f;
+    out[f * 64 + l] = make_float4(x[0], x[1], x[2], x[3]);
 }
 
 struct Bf3Args {
@@ -242,11 +285,12 @@ __device__ __forceinline__ void store_neg_c(const Bf3Args& g, int64_t r0, int64_
 // keeps the compiler from moving memory accesses across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <int WM, int WN>
+template <int WM, int WN, bool F32>
 __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_kernel(Bf3Args g)
 {
-    constexpr int SA = 4 * WM;               // A fragments per k block (2 WM row blocks x hi, lo)
-    constexpr int SF = 4 * WM + 4 * WN;      // fragments per k block
+    constexpr int FPB = F32 ? 1 : 2;         // fragments per block and k block (FP32; bf16 hi and lo)
+    constexpr int SA = 2 * FPB * WM;         // A fragments per k block (2 WM row blocks)
+    constexpr int SF = SA + 2 * FPB * WN;    // fragments per k block
     constexpr int NQ = SF / 4;               // loaded by each wave
     __shared__ u32x4 lds[2][SF * 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -264,8 +308,11 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int sl = wave + 4 * q;
-        if (sl < SA) src[q] = reinterpret_cast<const u32x4*>(g.Apk) + ((rbW + sl / 2) * KB * 2 + sl % 2) * 64 + lane;
-        else src[q] = reinterpret_cast<const u32x4*>(g.Bpk) + ((cbW + (sl - SA) / 2) * KB * 2 + (sl - SA) % 2) * 64 + lane;
+        if (sl < SA)
+            src[q] = reinterpret_cast<const u32x4*>(g.Apk) + ((rbW + sl / FPB) * KB * FPB + sl % FPB) * 64 + lane;
+        else
+            src[q] = reinterpret_cast<const u32x4*>(g.Bpk) +
+                     ((cbW + (sl - SA) / FPB) * KB * FPB + (sl - SA) % FPB) * 64 + lane;
     }
     u32x4 p0[NQ], p1[NQ];
     // the scheduler barriers keep the loads where they are issued (otherwise
@@ -276,7 +323,7 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
         const int64_t k = min(kb, KB - 1);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) p[q] = src[q][k * 128];
+        for (int q = 0; q < NQ; ++q) p[q] = src[q][k * 64 * FPB];
         __builtin_amdgcn_sched_barrier(0);
     };
     auto lwrite = [&](int buf, const u32x4 (&p)[NQ]) {
@@ -288,6 +335,22 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
     if (live) load_neg_c<WM, WN>(g, r0, c0, lane, full, acc);
 
     auto mul = [&](int buf) {
+        if constexpr (F32) {
+            const floatx4* Lf = reinterpret_cast<const floatx4*>(lds[buf]);
+            floatx4 a[WM], b[WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i) a[i] = Lf[(wm * WM + i) * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) b[j] = Lf[(SA + wn * WN + j) * 64 + lane];
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int i = 0; i < WM; ++i)
+#pragma unroll
+                    for (int j = 0; j < WN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i][s2], b[j][s2], acc[i][j], 0, 0, 0);
+            return;
+        }
         const bf16x8* Ls = reinterpret_cast<const bf16x8*>(lds[buf]);
         bf16x8 ah[WM], al[WM], bh[WN], bl[WN];
 #pragma unroll
@@ -394,7 +457,57 @@ int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, 
     const int q = wm / wn;
     g.ntiles = lower ? (int)(q * tiles_m * (tiles_m + 1) / 2) : (int)(tiles_m * tiles_n);
     const unsigned launch = (unsigned)(lower ? (g.ntiles + NXCD - 1) / NXCD * NXCD : g.ntiles);
-    hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<BF3_WM, BF3_WN>), dim3(launch), dim3(WGT), 0, st, g);
+    hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<BF3_WM, BF3_WN, false>), dim3(launch), dim3(WGT), 0, st, g);
+    FL_LAUNCHED();
+    return FL_OK;
+}
+
+// Workspace of f32_update (no more than split3_update's for the same sizes).
+size_t f32_workspace_bytes(int64_t M, int64_t N, int64_t K)
+{
+    const int64_t KB = cdiv(K, 8);
+    const int64_t MBp = cdiv(M, PAD) * (PAD / FE), NBp = cdiv(N, PAD) * (PAD / FE);
+    return align_up((size_t)MBp * KB * 64 * sizeof(float4), 256) + align_up((size_t)NBp * KB * 64 * sizeof(float4), 256);
+}
+
+// C (M x N, ldc) -= A (M x K, lda) B (K x N, ldb) with FP32 MFMA products on
+// packed fragments (the same product kernel as split3_update); lower as there.
+int f32_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+               int64_t ldc, int lower, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    FL_CHECK(M >= 0 && N >= 0 && K >= 0, "negative size");
+    FL_CHECK(!lower || M == N, "a lower-triangle update needs M == N");
+    FL_CHECK(batch().count == 1, "f32_update has no grouped form");
+    if (M == 0 || N == 0 || K == 0) return FL_OK;
+    FL_CHECK(ws && ws_bytes >= f32_workspace_bytes(M, N, K), "f32_update needs %zu workspace bytes",
+             f32_workspace_bytes(M, N, K));
+    const int KB = (int)cdiv(K, 8);
+    constexpr int wm = BF3_WM, wn = BF3_WN;
+    const int64_t tiles_m = cdiv(M, 2 * wm * FE), tiles_n = cdiv(N, 2 * wn * FE);
+    const int64_t MBp = cdiv(M, PAD) * (PAD / FE), NBp = cdiv(N, PAD) * (PAD / FE);
+    float4* Apk = static_cast<float4*>(ws);
+    float4* Bpk = Apk + align_up((size_t)MBp * KB * 64 * sizeof(float4), 256) / sizeof(float4);
+    hipLaunchKernelGGL(pack_a_f32_kernel, dim3((unsigned)cdiv(MBp * KB * 64, 256)), dim3(256), 0, st, A, lda, M,
+                       (int)K, MBp, KB, Apk);
+    FL_LAUNCHED();
+    hipLaunchKernelGGL(pack_b_f32_kernel, dim3((unsigned)cdiv(NBp * KB * 64, 256)), dim3(256), 0, st, B, ldb, N,
+                       (int)K, NBp, KB, Bpk);
+    FL_LAUNCHED();
+    Bf3Args g;
+    g.Apk = reinterpret_cast<const uint16_t*>(Apk);
+    g.Bpk = reinterpret_cast<const uint16_t*>(Bpk);
+    g.C = C;
+    g.M = M;
+    g.N = N;
+    g.ldc = ldc;
+    g.KB = KB;
+    g.tiles_m = (int)tiles_m;
+    g.tiles_n = (int)tiles_n;
+    g.lower = lower;
+    const int q = wm / wn;
+    g.ntiles = lower ? (int)(q * tiles_m * (tiles_m + 1) / 2) : (int)(tiles_m * tiles_n);
+    const unsigned launch = (unsigned)(lower ? (g.ntiles + NXCD - 1) / NXCD * NXCD : g.ntiles);
+    hipLaunchKernelGGL((gemm_bf16x3_lds_kernel<BF3_WM, BF3_WN, true>), dim3(launch), dim3(WGT), 0, st, g);
     FL_LAUNCHED();
     return FL_OK;
 }

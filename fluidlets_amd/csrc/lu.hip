@@ -54,6 +54,9 @@ int gemm_lower_update(int64_t n, int64_t K, const T* A, int64_t lda, const T* B,
 size_t split3_workspace_bytes(int64_t M, int64_t N, int64_t K);
 int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
                   int64_t ldc, int lower, void* ws, size_t ws_bytes, hipStream_t st);
+size_t f32_workspace_bytes(int64_t M, int64_t N, int64_t K);
+int f32_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+               int64_t ldc, int lower, void* ws, size_t ws_bytes, hipStream_t st);
 
 namespace {
 
@@ -1690,16 +1693,23 @@ __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int
 // convergence they leave unchanged on these matrices.
 
 size_t split_ws_bytes(int64_t n) { return align_up(split3_workspace_bytes(n, n, NBO_SPLIT), 256); }
+// the packed FP32 update's need for an update of M rows (M >= N, K <= NBO_SPLIT), within split_ws_bytes(n)
+size_t split_ws_bytes_f(int64_t M) { return f32_workspace_bytes(M, M, NBO_SPLIT); }
 
 // With `lower` (M == N; the symmetric factorization) only C's tiles on or
 // below the diagonal are updated.
+// With `packed` (FP32 products) the large updates run on the packed-operand
+// MFMA kernel, which reaches a higher fraction of the FP32 MFMA peak than
+// gemm_kernel on these shapes; `packed` and `split` share one workspace.
 template <typename T>
 int trailing_update(int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc,
-                    void* split, size_t split_bytes, hipStream_t st, bool lower = false)
+                    void* split, size_t split_bytes, hipStream_t st, bool lower = false, void* packed = nullptr)
 {
     if constexpr (std::is_same_v<T, float>) {
         if (split && M >= 2048 && N >= 256 && K <= NBO_SPLIT)
             return split3_update(M, N, K, A, lda, B, ldb, C, ldc, lower ? 1 : 0, split, split_bytes, st);
+        if (packed && M >= 2048 && N >= 256 && K <= NBO_SPLIT)
+            return f32_update(M, N, K, A, lda, B, ldb, C, ldc, lower ? 1 : 0, packed, split_ws_bytes_f(M), st);
     }
     if (lower) return gemm_lower_update<T>(M, K, A, lda, B, ldb, C, ldc, st);
     return gemm_launch<T>(M, N, K, T(-1), A, lda, B, ldb, T(1), C, ldc, nullptr, st);
@@ -1867,6 +1877,14 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
     void* split = split_products && std::is_same_v<T, float> && n - NBO_SPLIT >= 2048 ? tail + 256 : nullptr;
     FL_CHECK(batch().count == 1 || (spec && !sym && !split),
              "a grouped factorization is the asynchronous one with FP32 products");
+    // FP32 products: the large trailing updates on packed operands (FL_LU_F32_PACKED=0 turns it off, a
+    // measurement switch)
+    static const bool packed_on = [] {
+        const char* e = std::getenv("FL_LU_F32_PACKED");
+        return !(e && *e == '0');
+    }();
+    void* packed = std::is_same_v<T, float> && !split && packed_on && batch().count == 1 && n >= 2048 + 256
+                       ? tail + 256 : nullptr;
     if (spec) {
         hipLaunchKernelGGL(zero_u32_kernel, zgrid(dim3(1)), dim3(64), 0, st, flag, (int64_t)1, batch().zb);
         FL_LAUNCHED();
@@ -2049,7 +2067,8 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         const T* U12 = A + k * lda;  // row block k .. pe of U (column offset added below)
         if (!ahead) {
             // trailing update with K = nbo (symmetric: the tiles on or below the diagonal)
-            if (int s = trailing_update<T>(nr, nr, nbo, L21, lda, U12 + pe, lda, A + pe * lda + pe, lda, split, sb, st, sym))
+            if (int s = trailing_update<T>(nr, nr, nbo, L21, lda, U12 + pe, lda, A + pe * lda + pe, lda, split, sb, st, sym,
+                                           packed))
                 return s;
             if (int s = factor_panel(pe)) return s;
             continue;
@@ -2057,7 +2076,8 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         // (a) the next panel's columns pe .. pe2, all rows from pe; then that
         // panel is factored on the look-ahead stream while (b) updates the rest
         const int64_t pe2 = std::min<int64_t>(n, pe + nbo_w);
-        if (int s = trailing_update<T>(nr, pe2 - pe, nbo, L21, lda, U12 + pe, lda, A + pe * lda + pe, lda, split, sb, st))
+        if (int s = trailing_update<T>(nr, pe2 - pe, nbo, L21, lda, U12 + pe, lda, A + pe * lda + pe, lda, split, sb, st,
+                                       false, packed))
             return s;
         FL_HIP(hipEventRecord(ev.e[2], st));
         FL_HIP(hipStreamWaitEvent(sp, ev.e[2], 0));
@@ -2066,10 +2086,10 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
             if (sym) {
                 const int64_t r2 = n - pe2;
                 if (int s = trailing_update<T>(r2, r2, nbo, A + pe2 * lda + k, lda, U12 + pe2, lda, A + pe2 * lda + pe2,
-                                               lda, split, sb, st, true))
+                                               lda, split, sb, st, true, packed))
                     return s;
             } else if (int s = trailing_update<T>(nr, n - pe2, nbo, L21, lda, U12 + pe2, lda, A + pe * lda + pe2, lda,
-                                                  split, sb, st)) {
+                                                  split, sb, st, false, packed)) {
                 return s;
             }
         }
