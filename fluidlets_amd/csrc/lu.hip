@@ -892,7 +892,19 @@ __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int 
     // largest multiplier of the block -> *lmax_bits; the factors -> A
     const T wl = wave_max(lmax);
     if (lane == 0) atomicMax(lmax_bits, __float_as_uint((float)wl));
-    if (nb == NB) {
+    using VT = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n;
+    if (nb == NB && lda % VW == 0 && ((uintptr_t)A % 16) == 0) {
+        // 16-byte stores (the LDS copy's rows are not 16-byte aligned: scalar reads)
+        for (int idx = tid; idx < NB * NB / VW; idx += 256) {
+            const int rr = idx / (NB / VW), c = (idx % (NB / VW)) * VW;
+            VT v;
+            T* vp = reinterpret_cast<T*>(&v);
+#pragma unroll
+            for (int q = 0; q < VW; ++q) vp[q] = S[rr * LS + c + q];
+            *reinterpret_cast<VT*>(A + rr * lda + c) = v;
+        }
+    } else if (nb == NB) {
         for (int r = tid / NB; r < NB; r += 256 / NB) A[r * lda + tid % NB] = S[r * LS + tid % NB];
     } else {
         for (int r = 0; r < nb; ++r)
@@ -959,10 +971,19 @@ __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int 
         }
         T* dL = invL + bi * SUBB * NB + bj * SUBB;
         T* dU = invU + bi * SUBB * NB + bj * SUBB;
-        for (int e = tid; e < SUBB * SUBB; e += 256) {
-            const int ri = e / SUBB, cj = e % SUBB;
-            if (KEEP || bj > bi) dL[ri * NB + cj] = srcL ? srcL[ri * LB + cj] : T(0);
-            if (KEEP || bj < bi) dU[ri * NB + cj] = srcU ? srcU[ri * LB + cj] : T(0);
+        // 16-byte stores (the blocks start 16-byte aligned in the aux)
+        for (int e = tid; e < SUBB * SUBB / VW; e += 256) {
+            const int ri = e / (SUBB / VW), cj = (e % (SUBB / VW)) * VW;
+            VT vl, vu;
+            T* pl = reinterpret_cast<T*>(&vl);
+            T* pu = reinterpret_cast<T*>(&vu);
+#pragma unroll
+            for (int q = 0; q < VW; ++q) {
+                pl[q] = srcL ? srcL[ri * LB + cj + q] : T(0);
+                pu[q] = srcU ? srcU[ri * LB + cj + q] : T(0);
+            }
+            if (KEEP || bj > bi) *reinterpret_cast<VT*>(dL + ri * NB + cj) = vl;
+            if (KEEP || bj < bi) *reinterpret_cast<VT*>(dU + ri * NB + cj) = vu;
         }
     }
 }
