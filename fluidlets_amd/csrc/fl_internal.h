@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <type_traits>
 
 #include "../../include/fluidlets_hip.h"
 
@@ -54,10 +55,16 @@ inline dim3 zgrid(dim3 g)
     g.z = (unsigned)batch().count;
     return g;
 }
-template <class P>
-__device__ __forceinline__ P zoff(P p, int64_t zb)
+// The offset is taken on a byte pointer, not on an integer: a pointer that
+// round-trips through uintptr_t loses its address space, and every access
+// through it then compiles to FLAT instructions, which count on the LDS
+// wait counter too (a wait for an LDS read would also wait for every global
+// load in flight).
+template <class T>
+__device__ __forceinline__ T* zoff(T* p, int64_t zb)
 {
-    return p ? reinterpret_cast<P>(reinterpret_cast<uintptr_t>(p) + (uint64_t)blockIdx.z * (uint64_t)zb) : p;
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return p ? reinterpret_cast<T*>(reinterpret_cast<B*>(p) + (uint64_t)blockIdx.z * (uint64_t)zb) : p;
 }
 // Sets the calling thread's batch for a scope.
 struct BatchScope {

@@ -8,12 +8,16 @@
 // operands), with the next tile prefetched into registers while the current
 // one is multiplied.
 //
-// For alpha = +-1 (the LU's updates) the accumulators start from
-// (beta / alpha) C = alpha beta C, loaded in the MFMA output layout before the
-// K loop, so the epilogue is stores only: C = alpha (A B + alpha beta C), with
-// both scalings exact.  For any other alpha the accumulators start from zero
-// and the epilogue forms alpha (A B) + beta C, each term rounded once as in a
-// reference GEMM.  alpha == 0 is handled as C = beta C.
+// FP32: the accumulators start from zero and the epilogue forms
+// alpha (A B) + beta C, each term rounded once as in a reference GEMM.
+// (Starting them from C would round each of the K partial sums at the scale
+// of |C|: in the LU's trailing updates, whose products are small against C,
+// that multiplied the FP32 factorization's backward error about seventeenfold
+// at n = 6,144, 2^-24 |C| per step.)  FP64 with alpha = +-1 starts them from
+// (beta / alpha) C = alpha beta C, loaded before the K loop (which hides the
+// load), and the epilogue is stores only: C = alpha (A B + alpha beta C); its
+// per-step rounding is 2^-53 |C|.  Any other alpha takes the FP32 scheme.
+// alpha == 0 is handled as C = beta C.
 //
 // A workgroup reads all of K for its tile before it writes C, so the call is
 // safe in place when the C tile is the A tile (C = A B with N <= 128 and
@@ -134,8 +138,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
 
     typename P::acc_t acc[P::NB][P::NB];
     const bool use_c = g.beta != T(0) && g.alpha != T(0);
-    const bool pre_c = use_c && (g.alpha == T(1) || g.alpha == T(-1));   // (beta / alpha) = alpha beta exactly
-    const bool post_c = use_c && !pre_c;
+    const bool pre_c = use_c && sizeof(T) == 8 && (g.alpha == T(1) || g.alpha == T(-1));   // (beta / alpha) = alpha beta exactly
+    const bool post_c = use_c && !pre_c;   // C joins once, in the epilogue
     const T cscale = pre_c ? g.alpha * g.beta : T(0);
     // interior tiles take unpredicated accesses from one lane base pointer
     const bool full_c = m0 + TM <= g.M && n0 + TN <= g.N;
@@ -246,16 +250,26 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
     const T outscale = g.alpha != T(0) ? g.alpha : T(1);
     float local_max = 0.0f;
     if (full_c) {
+        // one row of blocks at a time, its loads of C issued before its stores
+        // (with a run-time ldc the compiler cannot tell C's entries apart and
+        // would otherwise wait for each load before the store after it)
 #pragma unroll
-        for (int i = 0; i < P::NB; ++i)
+        for (int i = 0; i < P::NB; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            T cv[P::NB][P::NACC];
+            if (post_c)
+#pragma unroll
+                for (int j = 0; j < P::NB; ++j)
+#pragma unroll
+                    for (int v = 0; v < P::NACC; ++v) cv[j][v] = cbase[coff(i, j, v)];
 #pragma unroll
             for (int j = 0; j < P::NB; ++j)
 #pragma unroll
                 for (int v = 0; v < P::NACC; ++v) {
-                    const int64_t o = coff(i, j, v);
-                    acc[i][j][v] = post_c ? outscale * acc[i][j][v] + g.beta * cbase[o] : outscale * acc[i][j][v];
-                    cbase[o] = acc[i][j][v];
+                    acc[i][j][v] = post_c ? outscale * acc[i][j][v] + g.beta * cv[j][v] : outscale * acc[i][j][v];
+                    cbase[coff(i, j, v)] = acc[i][j][v];
                 }
+        }
         if (g.absmax_bits) {
 #pragma unroll
             for (int i = 0; i < P::NB; ++i)

@@ -20,8 +20,10 @@
 // without bounds checks.
 //
 // Product kernel: four waves in a 2 x 2 grid, each wave a (32 WM) x (32 WN)
-// tile of C held in accumulators, which start from -C (exact) and are stored
-// negated: C - A B.  Each k block's fragments are loaded once per workgroup
+// tile of A B held in accumulators that start from zero; the epilogue stores
+// C - A B, so C enters one rounding instead of one per k step (starting the
+// sums from C would round every partial sum at the scale of |C|, which in the
+// LU's updates is far above the products').  Each k block's fragments are loaded once per workgroup
 // through a register ring into an LDS double buffer (below).  Workgroups are
 // dealt to the 8 XCDs round-robin, and tiles are ordered so that each XCD's
 // L2 sees a compact block of C tiles (shared A row panels and B column
@@ -175,6 +177,7 @@ struct Bf3Args {
     int tiles_m, tiles_n;
     int lower;       // only tiles that reach the diagonal or below
     int ntiles;      // tiles in the launch (all, or the lower ones)
+    int64_t awords;  // 4-byte words of the packed A operand
 };
 
 constexpr int NXCD = 8, GROUP = 8;
@@ -213,52 +216,53 @@ __device__ inline bool bf3_tile(const Bf3Args& g, int id, int q, int& tm, int& t
 
 __device__ inline int d_row(int l, int i) { return 8 * (i / 4) + 4 * (l / 32) + (i % 4); }
 
-// acc = -C over a wave's (32 WM) x (32 WN) tile at (r0, c0), in the MFMA
-// output layout (lane l, value v: row 8 (v / 4) + 4 (l / 32) + v % 4, column
-// l % 32 of each 32 x 32 block).  Interior tiles load unpredicated; edge
-// tiles load from clamped addresses and select zero outside C, without
-// branches (a branch per element would serialize the loads).
+// acc = 0, written as 0 * (words of the packed A operand): every word read
+// there is a finite float (an FP32 fragment, or two finite bf16 halves), and
+// accumulators that start from loaded values keep the loop's register
+// allocation (from a literal zero, or any value not loaded, the compiler
+// copies the accumulators across the loop's back edge and spills them).  All
+// workgroups read the same first 128 x 64 words, which stay in L2.
 template <int WM, int WN>
-__device__ __forceinline__ void load_neg_c(const Bf3Args& g, int64_t r0, int64_t c0, int lane, bool full,
-                                           floatx16 (&acc)[WM][WN])
+__device__ __forceinline__ void load_zero(const Bf3Args& g, int lane, floatx16 (&acc)[WM][WN])
 {
-    if (full) {
-        const float* cb = g.C + (r0 + 4 * (lane / 32)) * g.ldc + c0 + (lane % 32);
+    const float* z = reinterpret_cast<const float*>(g.Apk);
 #pragma unroll
-        for (int i = 0; i < WM; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-            for (int j = 0; j < WN; ++j)
+        for (int j = 0; j < WN; ++j)
 #pragma unroll
-                for (int v = 0; v < 16; ++v)
-                    acc[i][j][v] = -cb[(int64_t)(i * FE + 8 * (v / 4) + (v % 4)) * g.ldc + j * FE];
-    } else {
-#pragma unroll
-        for (int i = 0; i < WM; ++i)
-#pragma unroll
-            for (int j = 0; j < WN; ++j)
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int64_t r = r0 + i * FE + d_row(lane, v), c = c0 + j * FE + (lane % 32);
-                    const float x = g.C[min(r, g.M - 1) * g.ldc + min(c, g.N - 1)];
-                    acc[i][j][v] = (r < g.M && c < g.N) ? -x : 0.0f;
-                }
-    }
+            for (int v = 0; v < 16; ++v)
+                acc[i][j][v] = 0.0f * z[min((int64_t)((i * WN + j) * 16 + v) * 64 + lane, g.awords - 1)];
 }
 
-// C = -acc over the tile (edge tiles store inside C only).
+// The MFMA output layout of a wave's tile: lane l, value v of each 32 x 32
+// block is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+// C = C - acc over the tile (edge tiles inside C only).  One row of 32 x 32
+// blocks at a time: all its loads are issued before any of its stores (the
+// compiler cannot tell C's entries apart with a run-time ldc, so left alone
+// it would wait for each load before the store after it), and a scheduler
+// barrier keeps the next row's loads below this row's stores, which bounds
+// the registers live beside the accumulators.
 template <int WM, int WN>
-__device__ __forceinline__ void store_neg_c(const Bf3Args& g, int64_t r0, int64_t c0, int lane, bool full,
+__device__ __forceinline__ void store_sub_c(const Bf3Args& g, int64_t r0, int64_t c0, int lane, bool full,
                                             const floatx16 (&acc)[WM][WN])
 {
     if (full) {
         float* cb = g.C + (r0 + 4 * (lane / 32)) * g.ldc + c0 + (lane % 32);
 #pragma unroll
-        for (int i = 0; i < WM; ++i)
+        for (int i = 0; i < WM; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            float cv[WN][16];
+#pragma unroll
+            for (int j = 0; j < WN; ++j)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) cv[j][v] = cb[(int64_t)(i * FE + 8 * (v / 4) + (v % 4)) * g.ldc + j * FE];
 #pragma unroll
             for (int j = 0; j < WN; ++j)
 #pragma unroll
                 for (int v = 0; v < 16; ++v)
-                    cb[(int64_t)(i * FE + 8 * (v / 4) + (v % 4)) * g.ldc + j * FE] = -acc[i][j][v];
+                    cb[(int64_t)(i * FE + 8 * (v / 4) + (v % 4)) * g.ldc + j * FE] = cv[j][v] - acc[i][j][v];
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < WM; ++i)
@@ -267,7 +271,7 @@ __device__ __forceinline__ void store_neg_c(const Bf3Args& g, int64_t r0, int64_
 #pragma unroll
                 for (int v = 0; v < 16; ++v) {
                     const int64_t r = r0 + i * FE + d_row(lane, v), c = c0 + j * FE + (lane % 32);
-                    if (r < g.M && c < g.N) g.C[r * g.ldc + c] = -acc[i][j][v];
+                    if (r < g.M && c < g.N) g.C[r * g.ldc + c] -= acc[i][j][v];
                 }
     }
 }
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
     };
 
     floatx16 acc[WM][WN];
-    if (live) load_neg_c<WM, WN>(g, r0, c0, lane, full, acc);
+    if (live) load_zero<WM, WN>(g, lane, acc);
 
     auto mul = [&](int buf) {
         if constexpr (F32) {
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(WGT, (WM * WN <= 8) ? 2 : 1) void gemm_bf16x3_lds_k
     }
 
     if (!live) return;
-    store_neg_c<WM, WN>(g, r0, c0, lane, full, acc);
+    store_sub_c<WM, WN>(g, r0, c0, lane, full, acc);
 }
 
 // Wave tile 128 x 64 (WM = 4, WN = 2 blocks of 32 x 32), workgroup tile
@@ -444,6 +448,7 @@ int split3_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, 
     Bf3Args g;
     g.Apk = Apk;
     g.Bpk = Bpk;
+    g.awords = MBp * KB * 2 * FRAG / 2;
     g.C = C;
     g.M = M;
     g.N = N;
@@ -496,6 +501,7 @@ int f32_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, con
     Bf3Args g;
     g.Apk = reinterpret_cast<const uint16_t*>(Apk);
     g.Bpk = reinterpret_cast<const uint16_t*>(Bpk);
+    g.awords = MBp * KB * 64 * 4;
     g.C = C;
     g.M = M;
     g.N = N;

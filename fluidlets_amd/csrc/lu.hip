@@ -704,6 +704,12 @@ struct Acc32<float> {
 #pragma unroll
         for (int i = 0; i < 16; ++i) P[row(lane, i) * lp + col(lane)] = c[i];
     }
+    // P -= this block (one rounding per entry)
+    __device__ void sub_store(float* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[row(lane, i) * lp + col(lane)] -= c[i];
+    }
     __device__ float absmax() const
     {
         float m = 0.0f;
@@ -755,6 +761,15 @@ struct Acc32<double> {
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) P[row(lane, a, i) * lp + col(lane, b)] = c[a][b][i];
+    }
+    __device__ void sub_store(double* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) P[row(lane, a, i) * lp + col(lane, b)] -= c[a][b][i];
     }
     __device__ double absmax() const
     {
@@ -878,14 +893,15 @@ __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int 
             }
         }
         __syncthreads();
-        // trailing sub-blocks: A_ij -= L_ik U_kj
+        // trailing sub-blocks: A_ij -= L_ik U_kj (the product summed from zero,
+        // then subtracted: one rounding at the scale of A_ij)
         for (int p = wave; p < rest * rest; p += 4) {
             const int ib = kb + 1 + p / rest, jb = kb + 1 + p % rest;
             T* Cb = S + ib * SUBB * LS + jb * SUBB;
             Acc32<T> acc;
-            acc.load(Cb, LS, lane);
-            acc.mac(S + ib * SUBB * LS + o, LS, S + o * LS + jb * SUBB, LS, T(-1), lane);
-            acc.store(Cb, LS, lane);
+            acc.zero();
+            acc.mac(S + ib * SUBB * LS + o, LS, S + o * LS + jb * SUBB, LS, T(1), lane);
+            acc.sub_store(Cb, LS, lane);
         }
         __syncthreads();
     }
@@ -1927,10 +1943,11 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         const int v = e && *e ? std::atoi(e) : 0;
         return (int64_t)((v == 256 || v == 512) ? v : 0);
     }();
-    // FP64 at large orders: 512-column outer panels (the trailing updates'
-    // K = 512 halves the passes over A22; 7 % faster at n = 24,576, equal
-    // below 12,000)
-    const int64_t nbo_def = std::is_same_v<T, double> && n >= 16384 ? 512 : NBO;
+    // Large orders: 512-column outer panels (the trailing updates' K = 512
+    // halves the passes over A22 and the FP32 updates' epilogues, which read C
+    // back; FP64 7 % and FP32 products 2 % faster at n = 24,576, equal below
+    // 12,000)
+    const int64_t nbo_def = n >= 16384 ? 512 : NBO;
     const int64_t nbo_w = split ? NBO_SPLIT : (nbo_env ? nbo_env : nbo_def);
     // Look-ahead (asynchronous path, n >= LOOKAHEAD_MIN): while the caller's
     // stream updates the trailing matrix beyond the next outer panel, the next

@@ -482,3 +482,43 @@ def test_symmetric_factorization_falls_back_to_pivoting():
     lu, piv = sl.lu_factor(S)
     assert np.array_equal(F.ipiv.cpu().numpy(), piv)
     assert np.allclose(F.lu.cpu().numpy(), lu, rtol=0, atol=1e-11 * np.abs(lu).max())
+
+
+@pytest.mark.parametrize("sym,products", [(False, "fp32"), (True, "fp32"), (True, "bf16x3")])
+def test_fp32_factor_backward_error_is_lapack_level(sym, products):
+    """The FP32 factors of the 2,048-point wall sphere (n = 6,144; every large
+    trailing update on the packed MFMA kernels) have LAPACK sgetrf's backward
+    error on the same FP32 matrix, within a factor of 2 (split-bf16 products,
+    good to about 2^-16 each, within a factor of 4).  The updates sum their
+    products from zero and subtract the sum from C once: starting the sums
+    from C rounded every partial sum at the scale of |C| and left the FP32
+    factor's backward error 17 times LAPACK's here (and 30 times at n =
+    24,576, a third refinement step)."""
+    import torch
+    from fluidlets_amd import linalg
+    from fluidlets_amd.core.assemble import assemble_device
+    from fluidlets_amd import _device as D
+    print(f"case: FP32 factor backward error sym={sym} products={products}", flush=True)
+    N = 2048
+    X = fibonacci_sphere(N, 1.0, (0, 0, 1.5))
+    xs = D.soa(X, D.device())
+    eps_t, stride = D.eps_arg(0.5 * np.sqrt(4 * np.pi / N), N, D.device())
+    K64, K32 = assemble_device(fl.WallStokeslet3D(), xs, xs, eps_t, stride, 1.0, fp64=True, fp32=True)
+    A = K32.clone()
+    F = linalg.getrf(A, refill=lambda T: T.copy_(K32), products=products, sym=sym)
+    assert F.extra.get("async") and F.info == 0
+    n = 3 * N
+
+    def backward_error(lu):
+        L = torch.tril(lu, -1) + torch.eye(n, dtype=torch.float64, device=lu.device)
+        return float((L @ torch.triu(lu) - K32.double()).abs().max() / K32.double().abs().max())
+
+    err_gpu = backward_error(F.lu.double())
+    lu_ref, piv_ref = sl.lu_factor(K32.cpu().numpy())
+    assert np.array_equal(piv_ref, np.arange(n))     # no interchanges: the same elimination
+    err_ref = backward_error(torch.from_numpy(lu_ref).cuda().double())
+    assert err_gpu <= (2.0 if products == "fp32" else 4.0) * err_ref, (err_gpu, err_ref)
+    # and FP64 refinement from FP32-product factors converges in two corrections to 1e-12
+    rhs = torch.from_numpy(np.random.default_rng(7).standard_normal((n, 6))).cuda()
+    x, steps, res = linalg.refine(K64, F, rhs, tol=1e-12)
+    assert steps <= (2 if products == "fp32" else 3) and res.max() <= 1e-12
