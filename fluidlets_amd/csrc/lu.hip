@@ -1107,11 +1107,13 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 // Forward (LOWER: L z = y, unit diagonal) or backward (U x = z) substitution
 // by NB-row blocks in one launch, for NR-padded right-hand sides stored
 // row-major with ld nrhs <= NR.  Block row p needs every block solved before
-// it in the sweep, written with its nearest block q = p -/+ 1 apart:
-//   c_p = inv(T_pp) (b_p - sum_{j != q} T_pj x_j)   (j < p forward, j > p backward),
-//   x_p = c_p - M_p x_q,   M_p = inv(T_pp) T_pq      (inverse and link kept in aux),
-// so c_p is formed while block q is still being solved, and the step from
-// x_q to x_p is one NB x NB product.
+// it in the sweep, written with its two nearest blocks q1 = p -/+ 1 and
+// q2 = p -/+ 2 apart:
+//   c_p = inv(T_pp) (b_p - sum_{j != q1, q2} T_pj x_j)   (j < p forward, j > p backward),
+//   x_p = c_p - M2_p x_q2 - M_p x_q1,  M_p = inv(T_pp) T_pq1, M2_p = inv(T_pp) T_pq2
+// (inverse and links kept in aux), so c_p and its far sums are formed while
+// the two blocks before it are still being solved, and the step from x_q1 to
+// x_p is one NB x NB product.
 // Workgroups take tickets in dispatch order (ticket t solves block t forward,
 // nblk - 1 - t backward), so a workgroup waits only on workgroups dispatched
 // before it.  The output x is filled with a signalling-NaN sentinel before
@@ -1124,6 +1126,13 @@ __global__ __launch_bounds__(256) void panel_update_kernel(T* A, int64_t lda, in
 // up sets sync[1] and writes quiet NaN for its block, so every workgroup ends.
 // sync = [ticket, error], zeroed before the launch.
 constexpr int CH_SPIN = 1 << 22;
+// NB x NB blocks per diagonal block in the factor's aux (FactorAux): FP32
+// factors keep the two-step links ML2, MU2 as well (trsm_chain_kernel)
+template <typename T>
+constexpr int aux_blocks()
+{
+    return sizeof(T) == 4 ? 6 : 4;
+}
 
 __global__ void zero_u32_kernel(unsigned* p, int64_t n, int64_t zb)
 {
@@ -1246,9 +1255,10 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
     };
     // the diagonal block's inverse row and the link row M_p (always NB x NB,
     // zero or identity beyond the block; rows of acc beyond it are zero)
-    const T* blk = invs + p * 4 * NB * NB + (int64_t)r * NB + hf * H;
+    const T* blk = invs + p * aux_blocks<T>() * NB * NB + (int64_t)r * NB + hf * H;
     const T* inv = blk + (LOWER ? 0 : NB * NB);
     const T* lnk = blk + (LOWER ? 2 : 3) * NB * NB;
+    const T* lnk2 = blk + (LOWER ? 4 : 5) * NB * NB;
     auto load_nb = [&](const T* src) {
 #pragma unroll
         for (int k = 0; k < H; k += V) {
@@ -1293,8 +1303,12 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
             xs[idx] = (c < nbj && q < nrhs) ? chain_read(&x[(c0 + c) * nrhs + q], gave_up) : T(0);
         }
     };
-    // every block of the sweep before this one but the nearest: acc -= T_pj x_j
-    const int64_t far = nsteps > 0 ? nsteps - 1 : 0;
+    // every block of the sweep before this one but the nearest (FP32: but the
+    // two nearest, which gives the far sums and c_p two block steps of the
+    // chain to finish in instead of one; FP64 has no registers left for the
+    // second link's row): acc -= T_pj x_j
+    constexpr bool TWO = sizeof(T) == 4;
+    const int64_t far = TWO ? (nsteps > 1 ? nsteps - 2 : 0) : (nsteps > 0 ? nsteps - 1 : 0);
     if (far > 0) load_row(LOWER ? 0 : nblk - 1);
     else load_nb(inv);
 #pragma unroll 1
@@ -1317,18 +1331,33 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
             xs[r * NR + q] = (r < nbp && q < nrhs) ? bq[q] - part[q] : T(0);
     stage_row();
     __syncthreads();
-    if (nsteps > 0) load_nb(lnk);
+    // the links' rows, loaded while c_p is formed: M2_p into trow and M_p
+    // into mrow when the second nearest block joins through its link
+    const bool two = TWO && nsteps > 1;
+    T mrow[TWO ? H : 1];
+    if (two) {
+        load_nb(lnk2);
+#pragma unroll
+        for (int k = 0; k < (TWO ? H : 1); k += V) {
+            const VT u = *reinterpret_cast<const VT*>(lnk + k);
+            const T* up = reinterpret_cast<const T*>(&u);
+#pragma unroll
+            for (int q = 0; q < V; ++q) mrow[k + q] = up[q];
+        }
+    } else if (nsteps > 0) {
+        load_nb(lnk);
+    }
     T y[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) y[q] = T(0);
     row_product(y);
 #pragma unroll
     for (int q = 0; q < NR; ++q) y[q] = row_sum<TPR>(y[q]);
-    if (nsteps > 0) {
-        // x_p = c_p - M_p x_nearest: the only work after the nearest block arrives
+    // y = c_p - M2_p x_second - M_p x_nearest
+    auto link_step = [&](int64_t j) {
         __syncthreads();
         stage_row();
-        read_x(LOWER ? p - 1 : p + 1);
+        read_x(j);
         __syncthreads();
         T z[NR];
 #pragma unroll
@@ -1336,7 +1365,14 @@ __global__ __launch_bounds__(NB* TPR) void trsm_chain_kernel(const T* LU, int64_
         row_product(z);
 #pragma unroll
         for (int q = 0; q < NR; ++q) y[q] -= row_sum<TPR>(z[q]);
+    };
+    if (two) {
+        link_step(LOWER ? p - 2 : p + 2);
+#pragma unroll
+        for (int k = 0; k < (TWO ? H : 1); ++k) trow[k] = mrow[k];
     }
+    // the only work after the nearest block arrives: one NB x NB product
+    if (nsteps > 0) link_step(LOWER ? p - 1 : p + 1);
     const int bad = __syncthreads_or(gave_up);
     if (tid == 0 && bad) atomicOr(&sync[1], 1u);
     if (hf == 0 && r < nbp)
@@ -1638,10 +1674,11 @@ int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 template <typename T>
 struct FactorAux {
-    // aux layout: perm (n int32, padded to 256 B), then per panel p four
-    // NB x NB blocks: invL, invU, and the chained solves' links
-    // ML = inv(L_pp) L_p,p-1 and MU = inv(U_pp) U_p,p+1 (link_kernel)
-    static constexpr int BLOCKS = 4;
+    // aux layout: perm (n int32, padded to 256 B), then per panel p
+    // aux_blocks<T>() NB x NB blocks: invL, invU, and the chained solves'
+    // links ML = inv(L_pp) L_p,p-1, MU = inv(U_pp) U_p,p+1, and for FP32
+    // ML2 = inv(L_pp) L_p,p-2 and MU2 = inv(U_pp) U_p,p+2 (link_kernel)
+    static constexpr int BLOCKS = aux_blocks<T>();
     static size_t perm_bytes(int64_t n) { return align_up((size_t)n * sizeof(int32_t), 256); }
     static size_t bytes(int64_t n)
     {
@@ -1658,13 +1695,16 @@ struct FactorAux {
 };
 
 // The links of the chained triangular solves: for block row p,
-//   ML_p = inv(L_pp) L_p,p-1  (p >= 1),   MU_p = inv(U_pp) U_p,p+1  (p <= nblk - 2),
+//   ML_p  = inv(L_pp) L_p,p-1  (p >= 1),   MU_p  = inv(U_pp) U_p,p+1  (p <= nblk - 2),
+//   ML2_p = inv(L_pp) L_p,p-2  (p >= 2),   MU2_p = inv(U_pp) U_p,p+2  (p <= nblk - 3),
 // NB x NB (ld NB), zero beyond the matrix and where the neighbour block does
-// not exist.  With them a sweep's block step depends on its neighbour only
-// through x_p = c_p - M_p x_(p-/+1), c_p being formed before the neighbour
-// arrives.  grid (nblk, 2): y = 0 forms ML, y = 1 MU; 256 lanes, each an
-// 8 x 8 tile of the product, K staged through LDS in slices of 32.  blocks =
-// FactorAux::invL(aux, n, 0) (the four blocks of panel p start at p 4 NB^2).
+// not exist.  With them a sweep's block step depends on its two nearest
+// blocks only through x_p = c_p - M2_p x_(p-/+2) - M_p x_(p-/+1), c_p being
+// formed before either arrives.  grid (nblk, 4): y = 0 forms ML, 1 MU, 2 ML2,
+// 3 MU2; 256 lanes, each an 8 x 8 tile of the product, K staged through LDS
+// in slices of 32 (FP64 factors: y = 0, 1 only).  blocks =
+// FactorAux::invL(aux, n, 0) (the blocks of panel p start at
+// p aux_blocks<T>() NB^2).
 template <typename T>
 __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int64_t n, T* blocks, int64_t zb)
 {
@@ -1675,12 +1715,13 @@ __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int
     __shared__ T Is[NB * (KC + 1)];
     __shared__ T Bs[KC * NB];
     const int64_t p = blockIdx.x;
-    const bool upper = blockIdx.y == 1;
+    const bool upper = blockIdx.y % 2 == 1;
+    const int dist = blockIdx.y < 2 ? 1 : 2;
     const int64_t nblk = (n + NB - 1) / NB;
     T* blk = blocks + p * FactorAux<T>::BLOCKS * NB * NB;
     const T* inv = blk + (upper ? NB * NB : 0);
-    T* out = blk + (upper ? 3 : 2) * NB * NB;
-    const bool exists = upper ? p + 1 < nblk : p >= 1;
+    T* out = blk + (2 + blockIdx.y) * NB * NB;
+    const bool exists = upper ? p + dist < nblk : p >= dist;
     const int tid = threadIdx.x, ti = tid / 16, tj = tid % 16;
     T acc[8][8];
 #pragma unroll
@@ -1688,7 +1729,7 @@ __global__ __launch_bounds__(256) void link_kernel(const T* LU, int64_t lda, int
 #pragma unroll
         for (int v = 0; v < 8; ++v) acc[u][v] = T(0);
     if (exists) {
-        const int64_t r0 = p * NB, c0 = (upper ? p + 1 : p - 1) * NB;
+        const int64_t r0 = p * NB, c0 = (upper ? p + dist : p - dist) * NB;
         const int64_t nbr = min((int64_t)NB, n - r0), nbc = min((int64_t)NB, n - c0);
         for (int k0 = 0; k0 < NB; k0 += KC) {
             for (int idx = tid; idx < NB * KC; idx += 256) {
@@ -2137,7 +2178,8 @@ int getrf_impl(T* A, int64_t n, int64_t lda, int32_t* ipiv, void* aux, int* info
         }
     }
     // the chained solves' links, from the finished factors
-    hipLaunchKernelGGL(link_kernel<T>, zgrid(dim3((unsigned)div_up(n, NB), 2)), dim3(256), 0, st, (const T*)A, lda, n,
+    hipLaunchKernelGGL(link_kernel<T>, zgrid(dim3((unsigned)div_up(n, NB), aux_blocks<T>() - 2)), dim3(256), 0, st,
+                       (const T*)A, lda, n,
                        FactorAux<T>::invL(aux, n, 0), batch().zb);
     FL_LAUNCHED();
     if (spec) {

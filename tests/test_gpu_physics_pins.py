@@ -23,7 +23,13 @@ from fluidlets_amd.geometry import cube_sphere, fibonacci_sphere, gs_biflagellat
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("variant,published,tol", [("paper", 5.75e-2, 5e-5), ("authors_code", 5.529e-2, 1e-5)])
+# The published figures are what the reference's procedure gives: RK45 at
+# rtol 1e-6 over one beat.  At that tolerance the step choices alone move the
+# distance per beat by a few 1e-5 (solves that differ in the 13th digit gave
+# 5.5288e-2 and 5.5313e-2 for the authors' shape), so the test follows the
+# same procedure and allows 5e-5.  Converged in time (rtol 1e-8) the two
+# shapes swim 5.737e-2 and 5.528e-2 per beat.
+@pytest.mark.parametrize("variant,published,tol", [("paper", 5.75e-2, 5e-5), ("authors_code", 5.529e-2, 5e-5)])
 def test_biflagellate_swim_speed(variant, published, tol):
     print(f"case: biflagellate one beat, {variant}", flush=True)
     from fluidlets_amd.swimming import swim
