@@ -22,7 +22,8 @@ The default run also carries the other legs of the workloads: the icosphere
 solve, the 8,192-point sphere resistance solves (WallStokeslet3D and
 Brinkmanlet3D at alpha = 1, 5, 10; mixed precision against this library's
 FP64 LU and rocSOLVER's), the 131,072-point rod-carpet GMRES (FP64, flexible
-with FP32C inner cycles, block-Jacobi) and the 2,000-request serving mix.
+with FP32C inner cycles, block-Jacobi), the 2,000-request serving mix and the
+C2 kernel sum at 2,097,152 points by the treecode against the direct sum.
 `bench_wall_s` is the whole run's wall time.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-solve] [--no-serve] [--no-gmres]
@@ -494,6 +495,49 @@ def gmres_section(torch, fl, D, precision_list=("fp64", "mixed", "fp64_block_jac
     return out
 
 
+def treecode_section(torch, fl, D, n_points=2097152, degree=8, theta=0.7, leaf=2048, check_targets=8192):
+    """The C2 kernel sum (WallStokeslet3D, eps = 0.02, points uniform in
+    [-4,4]^2 x [0.05,4], N(0,1) forces; seed 3) at 2,097,152 points, by the
+    barycentric Lagrange treecode (degree 8, theta 0.7, leaves of 2,048) and by
+    one direct FP64 launch (the general path: above about 428,000 points the
+    reciprocal path's workspace passes its 4 GiB cap).  The treecode's error is
+    measured against the direct sum on 8,192 of the targets; its setup (trees
+    and lists on the host, uploads) is reported apart from the evaluation."""
+    from fluidlets_amd.core.matvec import evaluate_device
+    from fluidlets_amd.treecode import Treecode
+
+    rng = np.random.default_rng(3)
+    X = rng.uniform([-4, -4, 0.05], [4, 4, 4], (n_points, 3))
+    F = rng.standard_normal((n_points, 3))
+    K = fl.WallStokeslet3D()
+    dev = D.device()
+    x, f = D.soa(X, dev), D.soa(F, dev)
+    t0 = time.perf_counter()
+    tc = Treecode(K, X, X, EPS, degree=degree, theta=theta, leaf_size=leaf)
+    setup = time.perf_counter() - t0
+    tc.apply_device(f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, _ = tc.apply_device(f)
+    torch.cuda.synchronize()
+    t_tc = time.perf_counter() - t0
+    eps_t, stride = D.eps_arg(EPS, n_points, dev)
+    t0 = time.perf_counter()
+    ud_full, _ = evaluate_device(K, x, x, eps_t, stride, f)
+    torch.cuda.synchronize()
+    t_dir = time.perf_counter() - t0
+    sub = np.arange(0, n_points, n_points // check_targets)[:check_targets]
+    ud = ud_full[:, torch.from_numpy(sub).to(dev)].cpu().numpy()
+    ut = u[:, torch.from_numpy(sub).to(dev)].cpu().numpy()
+    del ud_full
+    return {"points": n_points, "kernel": "WallStokeslet3D", "degree": degree, "theta": theta, "leaf_size": leaf,
+            "setup_s": setup, "treecode_s": t_tc, "direct_s": t_dir, "speedup_vs_direct": t_dir / t_tc,
+            "rel_err_vs_direct": float(np.linalg.norm(ut - ud) / np.linalg.norm(ud)),
+            "checked_targets": int(len(sub)), "clusters_with_proxies": tc.ncl,
+            "direct_pairs_frac": tc.n_direct_pairs / n_points**2, "proxy_pairs_frac": tc.n_proxy_pairs / n_points**2,
+            "pair_evals_per_s": (tc.n_direct_pairs + tc.n_proxy_pairs) / t_tc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -518,6 +562,7 @@ def main():
     ap.add_argument("--small-solves-only", action="store_true",
                     help="of the serving section, run only the small-solve comparison")
     ap.add_argument("--no-gmres", action="store_true", help="skip the 131,072-point GMRES rod-carpet section")
+    ap.add_argument("--no-treecode", action="store_true", help="skip the 2,097,152-point treecode section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (the reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -653,6 +698,11 @@ def main():
     if not args.no_gmres:
         result["rod_carpet_gmres"] = gmres_section(torch, fl, D)
         progress("rod-carpet GMRES done")
+
+    if not args.no_treecode:
+        result["treecode_2m"] = treecode_section(torch, fl, D)
+        progress(f"treecode at 2,097,152 points: {result['treecode_2m']['treecode_s']:.3f} s against "
+                 f"{result['treecode_2m']['direct_s']:.3f} s direct, error {result['treecode_2m']['rel_err_vs_direct']:.1e}")
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(X, F, args.cpu_targets)

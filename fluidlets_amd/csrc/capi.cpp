@@ -65,6 +65,13 @@ int assemble_nearest_launch(int kind, double alpha, int64_t M, int64_t Q, int64_
 
 int nearest_launch(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* out,
                    hipStream_t stream);
+int bltc_weights_launch(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t N, const double* src,
+                        const double* f, const double* fi, int64_t P, double* px, double* pf, double* pfi,
+                        hipStream_t stream);
+int bltc_eval_launch(int kind, double alpha, int64_t M, const double* tgt, const int64_t* tidx, int64_t nblk,
+                     const int64_t* blk, const int64_t* seg, int64_t N, const double* src, const double* f,
+                     const double* fi, int64_t P, const double* px, const double* pf, const double* pfi, double eps,
+                     double w_re, double w_im, double* u_re, double* u_im, hipStream_t stream);
 
 size_t getrf_aux_bytes(int64_t n, int precision);
 size_t getrf_workspace_bytes(int64_t n, int precision);
@@ -170,6 +177,36 @@ int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N, const double* tgt, co
     }
     return eval_launch(kernel->kind, kernel->alpha, M, N, tgt, src, eps, eps_stride, f_re, f_im, w_re, w_im, u_re,
                        u_im, precision, workspace, workspace_bytes, as_stream(stream));
+}
+
+int fl_bltc_weights(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t N, const double* src,
+                    const double* f_re, const double* f_im, int64_t P, double* px, double* pf_re, double* pf_im,
+                    void* stream)
+{
+    clear_error();
+    return bltc_weights_launch(ncl, cl, box, degree, N, src, f_re, f_im, P, px, pf_re, pf_im, as_stream(stream));
+}
+
+int fl_bltc_eval(const fl_kernel* kernel, int64_t M, const double* tgt, const int64_t* tidx, int64_t nblk,
+                 const int64_t* blk, const int64_t* seg, int64_t N, const double* src, const double* f_re,
+                 const double* f_im, int64_t P, const double* px, const double* pf_re, const double* pf_im,
+                 double eps, double mu_re, double mu_im, double* u_re, double* u_im, void* stream)
+{
+    clear_error();
+    if (int s = check_kernel(kernel)) return s;
+    FL_CHECK(mu_re != 0.0 || mu_im != 0.0, "mu must be nonzero");
+    const double s = kernel_scale(kernel->kind);
+    double w_re, w_im;
+    if (mu_im == 0.0) {
+        w_re = s / mu_re;
+        w_im = 0.0;
+    } else {
+        const double den = mu_re * mu_re + mu_im * mu_im;
+        w_re = s * (mu_re / den);
+        w_im = -s * (mu_im / den);
+    }
+    return bltc_eval_launch(kernel->kind, kernel->alpha, M, tgt, tidx, nblk, blk, seg, N, src, f_re, f_im, P, px,
+                            pf_re, pf_im, eps, w_re, w_im, u_re, u_im, as_stream(stream));
 }
 
 int fl_assemble(const fl_kernel* kernel, int64_t M, int64_t N, const double* tgt, const double* src,

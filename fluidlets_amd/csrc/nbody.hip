@@ -1030,6 +1030,244 @@ void launch_kind(int kind, const NbArgs& a, dim3 grid, int precision, hipStream_
     else launch_main<FL_BRINKMANLET3D, NF, TPL>(a, grid, precision, stream);
 }
 
+
+// ---------------------------------------------------------------- barycentric Lagrange treecode
+// Fast summation of the same kernel sums (a particle-cluster treecode with
+// barycentric Lagrange interpolation: Wang, Tlupova & Krasny, SIAM J. Sci.
+// Comput. 42 (2020) A2993; kernel-independent, so the regularized kernels and
+// the wall's image system are interpolated as they are).  For a cluster C of
+// sources with box [lo, hi]^3 and Chebyshev points of the second kind
+// s_k = (lo + hi)/2 + (hi - lo)/2 cos(k pi / n), k = 0 .. n, in each
+// dimension, the (n + 1)^3 proxy sources at s_(k1,k2,k3) carry the strengths
+//   F_k = sum_{y_j in C} L_k1(y_j1) L_k2(y_j2) L_k3(y_j3) f_j,
+//   L_k(x) = (w_k / (x - s_k)) / sum_l (w_l / (x - s_l)),   w_k = (-1)^k (1/2 at k = 0, n),
+// (L_k(x) = 1 when x is s_k), and a target far from C in the tree's
+// acceptance criterion sums the kernel over the proxies instead of over C.
+// The host builds the tree and each target block's list of segments (runs of
+// real sources or of one cluster's proxies); the kernels below compute the
+// proxy strengths and then walk the lists with the direct sum's pair code.
+
+constexpr int BLTC_NMAX = 12;   // largest interpolation degree
+
+struct BltcWeightArgs {
+    const int64_t* cl;    // per cluster: source begin, source end, proxy base
+    const double* box;    // per cluster: lo x, y, z, hi x, y, z
+    const double* src;    // (3, N) sorted sources
+    const double* f;      // (3, N) strengths (real part)
+    const double* fi;     // (3, N) imaginary part or null
+    double* px;           // (3, P) proxy coordinates
+    double* pf;           // (3, P) proxy strengths
+    double* pfi;          // (3, P) imaginary part or null
+    int64_t N, P;
+    int degree;
+};
+
+// One workgroup per cluster: the proxies' coordinates and strengths.  The
+// cluster's Chebyshev points and barycentric weights are staged in LDS once;
+// sources are staged 256 at a time with their basis values L_k(y_d) (the
+// barycentric formula above, exact 1 / 0 where y_d is a node); each lane owns
+// proxies q = tid, tid + 256, ... and sums over the sources in order
+// (deterministic).
+template <int NF>
+__global__ __launch_bounds__(WG) void bltc_weights_kernel(BltcWeightArgs a)
+{
+    constexpr int NB1 = BLTC_NMAX + 1;
+    constexpr int QMAX = (NB1 * NB1 * NB1 + WG - 1) / WG;
+    constexpr int RW = 3 * NB1 + 3 * NF;   // LDS record of a source: basis values, strengths
+    __shared__ double Ls[WG * RW];
+    __shared__ double nodes[3 * NB1], bw[NB1];
+    const int tid = threadIdx.x;
+    const int64_t c = blockIdx.x;
+    const int n = a.degree, n1 = n + 1, Q = n1 * n1 * n1;
+    const int64_t b = a.cl[3 * c], e = a.cl[3 * c + 1], pb = a.cl[3 * c + 2];
+    const double* bx = a.box + 6 * c;
+    if (tid < 3 * n1) {
+        const int d = tid / n1, k = tid % n1;
+        nodes[d * NB1 + k] = 0.5 * (bx[d] + bx[3 + d]) + 0.5 * (bx[3 + d] - bx[d]) * cos(M_PI * k / n);
+    }
+    if (tid < n1) bw[tid] = ((tid & 1) ? -1.0 : 1.0) * ((tid == 0 || tid == n) ? 0.5 : 1.0);
+    int k1[QMAX], k2[QMAX], k3[QMAX];
+    bool own[QMAX];
+    double acc[QMAX][3 * NF];
+#pragma unroll
+    for (int u = 0; u < QMAX; ++u) {
+        const int q = tid + u * WG;
+        own[u] = q < Q;
+        k1[u] = own[u] ? q / (n1 * n1) : 0;
+        k2[u] = own[u] ? (q / n1) % n1 : 0;
+        k3[u] = own[u] ? q % n1 : 0;
+#pragma unroll
+        for (int k = 0; k < 3 * NF; ++k) acc[u][k] = 0.0;
+    }
+    for (int64_t j0 = b; j0 < e; j0 += WG) {
+        const int64_t j = j0 + tid;
+        __syncthreads();   // the previous tile is consumed (and the nodes are staged)
+        if (j < e) {
+            double* rec = Ls + tid * RW;
+            for (int d = 0; d < 3; ++d) {
+                const double x = a.src[d * a.N + j];
+                double sum = 0.0;
+                int hit = -1;
+                for (int k = 0; k < n1; ++k) {
+                    const double dx = x - nodes[d * NB1 + k];
+                    if (dx == 0.0 && hit < 0) hit = k;
+                    const double t = dx != 0.0 ? bw[k] / dx : 0.0;
+                    rec[d * NB1 + k] = t;
+                    sum += t;
+                }
+                const double inv = hit < 0 ? 1.0 / sum : 0.0;
+                for (int k = 0; k < n1; ++k)
+                    rec[d * NB1 + k] = hit < 0 ? rec[d * NB1 + k] * inv : (k == hit ? 1.0 : 0.0);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                rec[3 * NB1 + k] = a.f[k * a.N + j];
+                if (NF == 2) rec[3 * NB1 + 3 + k] = a.fi[k * a.N + j];
+            }
+        }
+        __syncthreads();
+        const int cnt = (int)min((int64_t)WG, e - j0);
+        for (int sidx = 0; sidx < cnt; ++sidx) {
+            const double* r = Ls + sidx * RW;
+#pragma unroll
+            for (int u = 0; u < QMAX; ++u) {
+                if (own[u]) {
+                    const double w = r[k1[u]] * r[NB1 + k2[u]] * r[2 * NB1 + k3[u]];
+#pragma unroll
+                    for (int k = 0; k < 3 * NF; ++k) acc[u][k] = fma(w, r[3 * NB1 + k], acc[u][k]);
+                }
+            }
+        }
+    }
+    __syncthreads();   // the nodes, for a cluster without sources too
+#pragma unroll
+    for (int u = 0; u < QMAX; ++u) {
+        if (own[u]) {
+            const int64_t o = pb + tid + u * WG;
+            const int kk[3] = {k1[u], k2[u], k3[u]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                a.px[d * a.P + o] = nodes[d * NB1 + kk[d]];
+                a.pf[d * a.P + o] = acc[u][d];
+                if (NF == 2) a.pfi[d * a.P + o] = acc[u][3 + d];
+            }
+        }
+    }
+}
+
+struct BltcEvalArgs {
+    const double* tgt;     // (3, M) sorted targets
+    const int64_t* tidx;   // original index of each sorted target
+    const int64_t* blk;    // per target block: target begin, count (<= WG BLTC_TPL = 256), segment begin, segment end
+    const int64_t* seg;    // per segment: kind (0 sources, 1 proxies), begin, count
+    const double* src;     // (3, N) sorted sources, strengths f / fi
+    const double* f;
+    const double* fi;
+    const double* px;      // (3, P) proxies, strengths pf / pfi
+    const double* pf;
+    const double* pfi;
+    double* u_re;          // (3, M) in the original target order
+    double* u_im;
+    int64_t M, N, P;
+    double e2;             // (scaled) eps^2, one blob width
+    double sc;             // coordinate scale (alpha for the Brinkmanlet)
+    double w_re, w_im;
+};
+
+// One workgroup per target block (BLTC_TPL targets per lane, lane-strided as
+// in nbody64): its segments in list order, each staged through LDS a tile at
+// a time and summed with the direct kernel's pair code, so a list that holds
+// every source directly gives the direct sum's terms in another order.  One
+// target per lane: with two, the blocks (and workgroups) halve and the
+// uneven lists balance worse (90 against 79 ms at 262,144 points, degree 8).
+constexpr int BLTC_TPL = 1;
+
+template <int KIND, int NF>
+__global__ __launch_bounds__(WG) void bltc_eval_kernel(BltcEvalArgs a)
+{
+    constexpr int W = Src64<KIND, NF>::W;
+    constexpr int FO = Src64<KIND, NF>::FOFF;
+    constexpr int TPL = BLTC_TPL;
+    __shared__ __attribute__((aligned(16))) double lds[TILE * W];
+    __shared__ double bl[BRINK_LDS];
+    const int tid = threadIdx.x;
+    if (KIND == FL_BRINKMANLET3D) stage_brinkman(bl);
+    const int64_t* bk = a.blk + 4 * (int64_t)blockIdx.x;
+    const int64_t t0 = bk[0], tc = bk[1], s0 = bk[2], s1 = bk[3];
+    double tx[TPL], ty[TPL], tz[TPL], acc[TPL][3 * NF];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const int64_t i = t0 + min((int64_t)(t * WG + tid), tc - 1);
+        tx[t] = a.sc * a.tgt[i];
+        ty[t] = a.sc * a.tgt[a.M + i];
+        tz[t] = a.sc * a.tgt[2 * a.M + i];
+#pragma unroll
+        for (int k = 0; k < 3 * NF; ++k) acc[t][k] = 0.0;
+    }
+    for (int64_t sg = s0; sg < s1; ++sg) {
+        const bool prox = a.seg[3 * sg] != 0;
+        const int64_t jb = a.seg[3 * sg + 1], cnt = a.seg[3 * sg + 2];
+        const double* X = prox ? a.px : a.src;
+        const double* F = prox ? a.pf : a.f;
+        const double* FI = prox ? a.pfi : a.fi;
+        const int64_t ld = prox ? a.P : a.N;
+        for (int64_t j0 = 0; j0 < cnt; j0 += TILE) {
+            double rec[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) rec[k] = 0.0;
+            if (j0 + tid < cnt) {
+                const int64_t j = jb + j0 + tid;
+                fill_src64<KIND, NF>(rec, a.sc * X[j], a.sc * X[ld + j], a.sc * X[2 * ld + j], a.e2);
+#pragma unroll
+                for (int k = 0; k < 3 * NF; ++k) rec[FO + k] = k < 3 ? F[(int64_t)k * ld + j] : FI[(int64_t)(k - 3) * ld + j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < W; k += 2)
+                *reinterpret_cast<double2*>(&lds[tid * W + k]) = make_double2(rec[k], rec[k + 1]);
+            __syncthreads();
+            const int nt = (int)min((int64_t)TILE, cnt - j0);
+            for (int s = 0; s < nt; ++s) {
+                double r[W];
+#pragma unroll
+                for (int k = 0; k < W; k += 2) {
+                    double2 v = *reinterpret_cast<const double2*>(&lds[s * W + k]);
+                    r[k] = v.x;
+                    r[k + 1] = v.y;
+                }
+                if constexpr (KIND == FL_BRINKMANLET3D) {
+                    pairs_brinkman<NF, TPL>(tx, ty, tz, r, bl, r + FO, acc);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < TPL; ++t) {
+                        if (KIND == FL_STOKESLET3D)
+                            pair_stokeslet<NF>(tx[t] - r[0], ty[t] - r[1], tz[t] - r[2], r[3], r + FO, acc[t]);
+                        else
+                            pair_wall<NF>(tx[t] - r[0], ty[t] - r[1], tz[t], r[2], r[3], r[4], r[5], r[6], r + FO,
+                                          acc[t]);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        if (t * WG + tid < tc) {
+            const int64_t o = a.tidx[t0 + t * WG + tid], M = a.M;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (NF == 1) {
+                    a.u_re[c * M + o] = a.w_re * acc[t][c];
+                    if (a.u_im) a.u_im[c * M + o] = a.w_im * acc[t][c];
+                } else {
+                    a.u_re[c * M + o] = a.w_re * acc[t][c] - a.w_im * acc[t][3 + c];
+                    a.u_im[c * M + o] = a.w_re * acc[t][3 + c] + a.w_im * acc[t][c];
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 double kernel_scale(int kind)
@@ -1164,6 +1402,67 @@ int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt,
         else hipLaunchKernelGGL(nbody_reduce<2>, rg, dim3(WG), 0, stream, a);
         FL_LAUNCHED();
     }
+    return FL_OK;
+}
+
+
+// Proxy strengths of ncl clusters (cl: source begin, end, proxy base; box:
+// lo xyz, hi xyz) at interpolation degree n (1 .. BLTC_NMAX).
+int bltc_weights_launch(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t N, const double* src,
+                        const double* f, const double* fi, int64_t P, double* px, double* pf, double* pfi,
+                        hipStream_t stream)
+{
+    FL_CHECK(degree >= 1 && degree <= BLTC_NMAX, "treecode degree must be 1 .. %d", BLTC_NMAX);
+    FL_CHECK(ncl >= 0 && N >= 0 && P >= 0, "negative size");
+    FL_CHECK(!fi == !pfi, "imaginary strengths need imaginary proxy strengths");
+    if (ncl == 0) return FL_OK;
+    FL_CHECK(cl && box && src && f && px && pf, "null pointer argument");
+    BltcWeightArgs a{cl, box, src, f, fi, px, pf, pfi, N, P, degree};
+    if (fi) hipLaunchKernelGGL(bltc_weights_kernel<2>, dim3((unsigned)ncl), dim3(WG), 0, stream, a);
+    else hipLaunchKernelGGL(bltc_weights_kernel<1>, dim3((unsigned)ncl), dim3(WG), 0, stream, a);
+    FL_LAUNCHED();
+    return FL_OK;
+}
+
+// The treecode's kernel sums over the target blocks' segment lists.
+int bltc_eval_launch(int kind, double alpha, int64_t M, const double* tgt, const int64_t* tidx, int64_t nblk,
+                     const int64_t* blk, const int64_t* seg, int64_t N, const double* src, const double* f,
+                     const double* fi, int64_t P, const double* px, const double* pf, const double* pfi, double eps,
+                     double w_re, double w_im, double* u_re, double* u_im, hipStream_t stream)
+{
+    const int nf = fi ? 2 : 1;
+    FL_CHECK(kind >= 0 && kind <= 2, "unknown kernel kind %d", kind);
+    FL_CHECK(M >= 0 && N >= 0 && P >= 0 && nblk >= 0, "negative size");
+    FL_CHECK(!(nf == 2 || w_im != 0.0) || u_im != nullptr, "complex result needs u_im");
+    FL_CHECK(!fi == !pfi, "imaginary strengths need imaginary proxy strengths");
+    FL_CHECK(eps > 0.0 && std::isfinite(eps), "eps must be finite and > 0");
+    if (M == 0 || nblk == 0) return FL_OK;
+    FL_CHECK(tgt && tidx && blk && seg && u_re, "null pointer argument");
+    FL_CHECK(kind != FL_BRINKMANLET3D || (alpha >= 0.0 && std::isfinite(alpha)), "alpha must be finite and >= 0");
+    double sc = 1.0;
+    if (kind == FL_BRINKMANLET3D) {
+        if (alpha < 1e-150) {   // the Stokeslet, as in eval_launch
+            kind = FL_STOKESLET3D;
+            w_re *= 0.5;
+            w_im *= 0.5;
+        } else {
+            sc = alpha;
+            w_re *= alpha;
+            w_im *= alpha;
+        }
+    }
+    BltcEvalArgs a{tgt, tidx, blk, seg, src, f, fi, px, pf, pfi, u_re, u_im, M, N, P, sc * sc * eps * eps, sc, w_re, w_im};
+    const dim3 grid((unsigned)nblk);
+    if (nf == 1) {
+        if (kind == FL_STOKESLET3D) hipLaunchKernelGGL((bltc_eval_kernel<FL_STOKESLET3D, 1>), grid, dim3(WG), 0, stream, a);
+        else if (kind == FL_WALL_STOKESLET3D) hipLaunchKernelGGL((bltc_eval_kernel<FL_WALL_STOKESLET3D, 1>), grid, dim3(WG), 0, stream, a);
+        else hipLaunchKernelGGL((bltc_eval_kernel<FL_BRINKMANLET3D, 1>), grid, dim3(WG), 0, stream, a);
+    } else {
+        if (kind == FL_STOKESLET3D) hipLaunchKernelGGL((bltc_eval_kernel<FL_STOKESLET3D, 2>), grid, dim3(WG), 0, stream, a);
+        else if (kind == FL_WALL_STOKESLET3D) hipLaunchKernelGGL((bltc_eval_kernel<FL_WALL_STOKESLET3D, 2>), grid, dim3(WG), 0, stream, a);
+        else hipLaunchKernelGGL((bltc_eval_kernel<FL_BRINKMANLET3D, 2>), grid, dim3(WG), 0, stream, a);
+    }
+    FL_LAUNCHED();
     return FL_OK;
 }
 

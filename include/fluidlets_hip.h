@@ -128,6 +128,38 @@ int fl_eval_noalloc(const fl_kernel* kernel, int64_t M, int64_t N,
                     const double* f_re, const double* f_im, double mu_re, double mu_im,
                     double* u_re, double* u_im, int precision, void* stream);
 
+/* ------------------------------------------------------------------ fast summation (treecode)
+ * Barycentric Lagrange treecode for the kernel sums of fl_eval: no
+ * counterpart in the reference, which defers fast summation
+ * (DESIGN.md:18, 381); its evaluate (core/matvec.py:24-49) is the direct sum.
+ * The host (fluidlets_amd.treecode) builds a source tree of clusters and, per
+ * block of at most 256 sorted targets, a list of segments: runs of sorted
+ * sources summed directly, or one cluster's (degree+1)^3 proxy sources.  All
+ * pointers are device memory; coordinates and strengths are (3, count)
+ * structure-of-arrays, FP64.
+ *
+ * fl_bltc_weights: the proxies of ncl clusters.  cl holds per cluster the
+ * source range [begin, end) in the sorted sources and the cluster's first
+ * proxy index (3 int64); box its lo x, y, z, hi x, y, z (6 doubles).  Writes
+ * px (3, P) (the Chebyshev points of the box) and pf (3, P) (the proxy
+ * strengths; pfi for imaginary strengths fi, both or neither).  degree 1 .. 12.
+ *
+ * fl_bltc_eval: u = (1/mu) sum over each block's segments of G(x, y) f, with
+ * one blob width eps (the proxies carry it too).  blk holds per block the
+ * first sorted target, the count (<= 256) and the segment range [s0, s1) (4
+ * int64); seg per segment the kind (0: sources, 1: proxies), the first index
+ * and the count (3 int64).  tidx maps a sorted target to its row of u
+ * (3, M), which is written in the caller's order.  Kernel, mu and complex
+ * results as fl_eval. */
+int fl_bltc_weights(int64_t ncl, const int64_t* cl, const double* box, int degree,
+                    int64_t N, const double* src, const double* f_re, const double* f_im,
+                    int64_t P, double* px, double* pf_re, double* pf_im, void* stream);
+int fl_bltc_eval(const fl_kernel* kernel, int64_t M, const double* tgt, const int64_t* tidx,
+                 int64_t nblk, const int64_t* blk, const int64_t* seg,
+                 int64_t N, const double* src, const double* f_re, const double* f_im,
+                 int64_t P, const double* px, const double* pf_re, const double* pf_im,
+                 double eps, double mu_re, double mu_im, double* u_re, double* u_im, void* stream);
+
 /* ------------------------------------------------------------------ assembly
  * K[3i+a, 3j+b] = G_ab(x_i, y_j; eps_j) / mu,  i < M, j < N.
  * Replaces _fill called at src/fluidlets/core/assemble.py:45 (loop
