@@ -495,7 +495,7 @@ def gmres_section(torch, fl, D, precision_list=("fp64", "mixed", "fp64_block_jac
     return out
 
 
-def treecode_section(torch, fl, D, n_points=2097152, degree=8, theta=0.7, leaf=2048, check_targets=8192):
+def treecode_section(torch, fl, D, n_points=2097152, degree=8, theta=0.7, leaf=1024, check_targets=8192):
     """The C2 kernel sum (WallStokeslet3D, eps = 0.02, points uniform in
     [-4,4]^2 x [0.05,4], N(0,1) forces; seed 3) at 2,097,152 points, by the
     barycentric Lagrange treecode (degree 8, theta 0.7, leaves of 2,048) and by

@@ -33,8 +33,8 @@ _SIGNATURES = {
                         c_void_p, c_double, c_double, c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
     "fl_eval_noalloc": (c_int, [POINTER(fl_kernel), c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
                                 c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_int, c_void_p]),
-    "fl_bltc_weights": (c_int, [c_int64, c_void_p, c_void_p, c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int64,
-                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fl_bltc_weights": (c_int, [c_int64, c_void_p, c_void_p, c_int, c_int64, c_void_p, c_int64, c_void_p, c_void_p,
+                                c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fl_bltc_eval": (c_int, [POINTER(fl_kernel), c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
                              c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_double, c_double,
                              c_double, c_void_p, c_void_p, c_void_p]),

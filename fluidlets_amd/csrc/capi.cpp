@@ -65,9 +65,9 @@ int assemble_nearest_launch(int kind, double alpha, int64_t M, int64_t Q, int64_
 
 int nearest_launch(int64_t Q, int64_t N, const double* quad, const double* force, int64_t* out,
                    hipStream_t stream);
-int bltc_weights_launch(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t N, const double* src,
-                        const double* f, const double* fi, int64_t P, double* px, double* pf, double* pfi,
-                        hipStream_t stream);
+int bltc_weights_launch(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t nchunk,
+                        const int64_t* chunk, int64_t N, const double* src, const double* f, const double* fi,
+                        int64_t P, double* px, double* pf, double* pfi, double* ws, hipStream_t stream);
 int bltc_eval_launch(int kind, double alpha, int64_t M, const double* tgt, const int64_t* tidx, int64_t nblk,
                      const int64_t* blk, const int64_t* seg, int64_t N, const double* src, const double* f,
                      const double* fi, int64_t P, const double* px, const double* pf, const double* pfi, double eps,
@@ -179,12 +179,13 @@ int fl_eval(const fl_kernel* kernel, int64_t M, int64_t N, const double* tgt, co
                        u_im, precision, workspace, workspace_bytes, as_stream(stream));
 }
 
-int fl_bltc_weights(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t N, const double* src,
-                    const double* f_re, const double* f_im, int64_t P, double* px, double* pf_re, double* pf_im,
-                    void* stream)
+int fl_bltc_weights(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t nchunk,
+                    const int64_t* chunk, int64_t N, const double* src, const double* f_re, const double* f_im,
+                    int64_t P, double* px, double* pf_re, double* pf_im, double* ws, void* stream)
 {
     clear_error();
-    return bltc_weights_launch(ncl, cl, box, degree, N, src, f_re, f_im, P, px, pf_re, pf_im, as_stream(stream));
+    return bltc_weights_launch(ncl, cl, box, degree, nchunk, chunk, N, src, f_re, f_im, P, px, pf_re, pf_im, ws,
+                               as_stream(stream));
 }
 
 int fl_bltc_eval(const fl_kernel* kernel, int64_t M, const double* tgt, const int64_t* tidx, int64_t nblk,

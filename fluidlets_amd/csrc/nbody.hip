@@ -1049,108 +1049,183 @@ void launch_kind(int kind, const NbArgs& a, dim3 grid, int precision, hipStream_
 
 constexpr int BLTC_NMAX = 12;   // largest interpolation degree
 
+constexpr int BLTC_TS = 64;    // sources staged per tile of the moments kernel
+
+// LDS of the moments kernel at n + 1 = N1 points per dimension: a tile of
+// source records (f in 6 slots, then L_k(y_1), L_k(y_2), L_k(y_3), N1 each;
+// an even count, for 16-byte reads), reused for the pairwise group sums.
+template <int N1>
+constexpr int bltc_rw() { return (6 + 3 * N1 + 1) / 2 * 2; }
+template <int NF, int N1>
+constexpr size_t bltc_moments_lds()
+{
+    constexpr size_t tile = (size_t)BLTC_TS * bltc_rw<N1>();
+    constexpr size_t sums = (size_t)(WG / (N1 * N1) / 2) * N1 * N1 * N1 * 3 * NF;
+    return (tile > sums ? tile : sums) * sizeof(double);
+}
+
 struct BltcWeightArgs {
-    const int64_t* cl;    // per cluster: source begin, source end, proxy base
-    const double* box;    // per cluster: lo x, y, z, hi x, y, z
-    const double* src;    // (3, N) sorted sources
-    const double* f;      // (3, N) strengths (real part)
-    const double* fi;     // (3, N) imaginary part or null
-    double* px;           // (3, P) proxy coordinates
-    double* pf;           // (3, P) proxy strengths
-    double* pfi;          // (3, P) imaginary part or null
+    const int64_t* cl;     // per cluster: source begin, source end, proxy base, first chunk, chunk count
+    const int64_t* chunk;  // per chunk: cluster, source begin, source end
+    const double* box;     // per cluster: lo x, y, z, hi x, y, z
+    const double* src;     // (3, N) sorted sources
+    const double* f;       // (3, N) strengths (real part)
+    const double* fi;      // (3, N) imaginary part or null
+    double* px;            // (3, P) proxy coordinates
+    double* pf;            // (3, P) proxy strengths
+    double* pfi;           // (3, P) imaginary part or null
+    double* ws;            // per chunk: (3 NF, (n + 1)^3) partial strengths
     int64_t N, P;
     int degree;
 };
 
-// One workgroup per cluster: the proxies' coordinates and strengths.  The
-// cluster's Chebyshev points and barycentric weights are staged in LDS once;
-// sources are staged 256 at a time with their basis values L_k(y_d) (the
-// barycentric formula above, exact 1 / 0 where y_d is a node); each lane owns
-// proxies q = tid, tid + 256, ... and sums over the sources in order
-// (deterministic).
-template <int NF>
-__global__ __launch_bounds__(WG) void bltc_weights_kernel(BltcWeightArgs a)
+// Chebyshev point k (of the second kind, degree n) of dimension d of a box.
+__device__ __forceinline__ double bltc_node(const double* bx, int d, int k, int n)
 {
-    constexpr int NB1 = BLTC_NMAX + 1;
-    constexpr int QMAX = (NB1 * NB1 * NB1 + WG - 1) / WG;
-    constexpr int RW = 3 * NB1 + 3 * NF;   // LDS record of a source: basis values, strengths
-    __shared__ double Ls[WG * RW];
-    __shared__ double nodes[3 * NB1], bw[NB1];
+    return 0.5 * (bx[d] + bx[3 + d]) + 0.5 * (bx[3 + d] - bx[d]) * cos(M_PI * k / n);
+}
+
+// Partial proxy strengths of one chunk of a cluster's sources (one workgroup
+// per chunk, so a cluster of half a million sources is summed by a hundred
+// workgroups rather than one).  The strengths are a product
+//   F[k1][k2 k3][c] = sum_j L_k1(y_j1) (L_k2(y_j2) L_k3(y_j3) f_jc),
+// so each lane owns one (k2, k3) and all n + 1 values of k1: per source it
+// forms the bracket once (1 + 3 NF multiplies) and adds (n + 1) 3 NF FMAs,
+// the L_k1 and f values being LDS broadcasts.  The (n + 1)^2 lanes of a
+// (k2, k3) set make a group; the WG / (n + 1)^2 groups take the tile's
+// sources in turn and are summed pairwise through LDS in a fixed tree, so
+// the result does not depend on timing.  Sources are staged 64 at a time:
+// 192 lanes compute the basis values of one dimension of one source (the
+// barycentric formula, exact 1 / 0 where the coordinate is a node), the last
+// 64 load the strengths.
+template <int NF, int N1>
+__global__ __launch_bounds__(WG) void bltc_moments_kernel(BltcWeightArgs a)
+{
+    constexpr int TS = BLTC_TS, RW = bltc_rw<N1>(), NC = 3 * NF, n = N1 - 1;
+    constexpr int G = N1 * N1, NG = WG / G;
+    static_assert(4 * TS == WG, "staging: three dimensions and the strengths, one source per lane each");
+    static_assert(NG >= 1, "a group of (n + 1)^2 lanes fits a workgroup");
+    extern __shared__ __attribute__((aligned(16))) double lds[];   // tile records; then the group sums
+    __shared__ double nodes[3 * N1], bw[N1];
     const int tid = threadIdx.x;
-    const int64_t c = blockIdx.x;
-    const int n = a.degree, n1 = n + 1, Q = n1 * n1 * n1;
-    const int64_t b = a.cl[3 * c], e = a.cl[3 * c + 1], pb = a.cl[3 * c + 2];
+    const int64_t ch = blockIdx.x;
+    const int64_t c = a.chunk[3 * ch], jb = a.chunk[3 * ch + 1], je = a.chunk[3 * ch + 2];
     const double* bx = a.box + 6 * c;
-    if (tid < 3 * n1) {
-        const int d = tid / n1, k = tid % n1;
-        nodes[d * NB1 + k] = 0.5 * (bx[d] + bx[3 + d]) + 0.5 * (bx[3 + d] - bx[d]) * cos(M_PI * k / n);
-    }
-    if (tid < n1) bw[tid] = ((tid & 1) ? -1.0 : 1.0) * ((tid == 0 || tid == n) ? 0.5 : 1.0);
-    int k1[QMAX], k2[QMAX], k3[QMAX];
-    bool own[QMAX];
-    double acc[QMAX][3 * NF];
+    if (tid < 3 * N1) nodes[tid] = bltc_node(bx, tid / N1, tid % N1, n);
+    if (tid < N1) bw[tid] = ((tid & 1) ? -1.0 : 1.0) * ((tid == 0 || tid == n) ? 0.5 : 1.0);
+    const int g = tid / G, p = tid % G, k2 = p / N1, k3 = p % N1;
+    const bool act = g < NG;
+    double acc[N1][NC];
 #pragma unroll
-    for (int u = 0; u < QMAX; ++u) {
-        const int q = tid + u * WG;
-        own[u] = q < Q;
-        k1[u] = own[u] ? q / (n1 * n1) : 0;
-        k2[u] = own[u] ? (q / n1) % n1 : 0;
-        k3[u] = own[u] ? q % n1 : 0;
+    for (int k1 = 0; k1 < N1; ++k1)
 #pragma unroll
-        for (int k = 0; k < 3 * NF; ++k) acc[u][k] = 0.0;
-    }
-    for (int64_t j0 = b; j0 < e; j0 += WG) {
-        const int64_t j = j0 + tid;
+        for (int k = 0; k < NC; ++k) acc[k1][k] = 0.0;
+    for (int64_t j0 = jb; j0 < je; j0 += TS) {
+        const int cnt = (int)min((int64_t)TS, je - j0);
         __syncthreads();   // the previous tile is consumed (and the nodes are staged)
-        if (j < e) {
-            double* rec = Ls + tid * RW;
-            for (int d = 0; d < 3; ++d) {
-                const double x = a.src[d * a.N + j];
-                double sum = 0.0;
+        if (tid < 3 * TS) {
+            const int d = tid / TS, i = tid % TS;
+            if (i < cnt) {
+                double* L = lds + i * RW + 6 + d * N1;
+                const double x = a.src[d * a.N + j0 + i];
+                double t[N1], sum = 0.0;
                 int hit = -1;
-                for (int k = 0; k < n1; ++k) {
-                    const double dx = x - nodes[d * NB1 + k];
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    const double dx = x - nodes[d * N1 + k];
                     if (dx == 0.0 && hit < 0) hit = k;
-                    const double t = dx != 0.0 ? bw[k] / dx : 0.0;
-                    rec[d * NB1 + k] = t;
-                    sum += t;
+                    t[k] = dx != 0.0 ? bw[k] / dx : 0.0;
+                    sum += t[k];
                 }
                 const double inv = hit < 0 ? 1.0 / sum : 0.0;
-                for (int k = 0; k < n1; ++k)
-                    rec[d * NB1 + k] = hit < 0 ? rec[d * NB1 + k] * inv : (k == hit ? 1.0 : 0.0);
-            }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                rec[3 * NB1 + k] = a.f[k * a.N + j];
-                if (NF == 2) rec[3 * NB1 + 3 + k] = a.fi[k * a.N + j];
+                for (int k = 0; k < N1; ++k) L[k] = hit < 0 ? t[k] * inv : (k == hit ? 1.0 : 0.0);
+            }
+        } else {
+            const int i = tid - 3 * TS;
+            if (i < cnt) {
+                double* r = lds + i * RW;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    r[k] = a.f[k * a.N + j0 + i];
+                    if (NF == 2) r[3 + k] = a.fi[k * a.N + j0 + i];
+                }
             }
         }
         __syncthreads();
-        const int cnt = (int)min((int64_t)WG, e - j0);
-        for (int sidx = 0; sidx < cnt; ++sidx) {
-            const double* r = Ls + sidx * RW;
+        if (act) {
+            for (int i = g; i < cnt; i += NG) {
+                const double* r = lds + i * RW;
+                const double w = r[6 + N1 + k2] * r[6 + 2 * N1 + k3];
+                double fw[NC];
 #pragma unroll
-            for (int u = 0; u < QMAX; ++u) {
-                if (own[u]) {
-                    const double w = r[k1[u]] * r[NB1 + k2[u]] * r[2 * NB1 + k3[u]];
+                for (int k = 0; k < NC; ++k) fw[k] = w * r[k];
 #pragma unroll
-                    for (int k = 0; k < 3 * NF; ++k) acc[u][k] = fma(w, r[3 * NB1 + k], acc[u][k]);
+                for (int k1 = 0; k1 < N1; ++k1) {
+                    const double l = r[6 + k1];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k) acc[k1][k] = fma(l, fw[k], acc[k1][k]);
                 }
             }
         }
     }
-    __syncthreads();   // the nodes, for a cluster without sources too
+    // groups g and g + s (g a multiple of 2 s) are summed into g, s = 1, 2, 4 ...
+    constexpr int SLOT = G * N1 * NC;
 #pragma unroll
-    for (int u = 0; u < QMAX; ++u) {
-        if (own[u]) {
-            const int64_t o = pb + tid + u * WG;
-            const int kk[3] = {k1[u], k2[u], k3[u]};
+    for (int s = 1; s < NG; s *= 2) {
+        __syncthreads();
+        if (act && g % (2 * s) == s) {
+            double* o = lds + (g / (2 * s)) * SLOT + p;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                a.px[d * a.P + o] = nodes[d * NB1 + kk[d]];
-                a.pf[d * a.P + o] = acc[u][d];
-                if (NF == 2) a.pfi[d * a.P + o] = acc[u][3 + d];
-            }
+            for (int k1 = 0; k1 < N1; ++k1)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) o[(k1 * NC + k) * G] = acc[k1][k];
+        }
+        __syncthreads();
+        if (act && g % (2 * s) == 0 && g + s < NG) {
+            const double* o = lds + (g / (2 * s)) * SLOT + p;
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1)
+#pragma unroll
+                for (int k = 0; k < NC; ++k) acc[k1][k] += o[(k1 * NC + k) * G];
+        }
+    }
+    if (g == 0) {
+        constexpr int Q = G * N1;
+        double* o = a.ws + ch * NC * Q + p;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+#pragma unroll
+            for (int k = 0; k < NC; ++k) o[k * Q + k1 * G] = acc[k1][k];
+    }
+}
+
+// One workgroup per cluster: the proxies' coordinates (the Chebyshev points
+// of its box, proxy q = (k1 (n + 1) + k2) (n + 1) + k3) and strengths, the
+// sum of its chunks' partial strengths in chunk order.
+template <int NF>
+__global__ __launch_bounds__(WG) void bltc_proxy_kernel(BltcWeightArgs a)
+{
+    constexpr int NC = 3 * NF;
+    const int64_t c = blockIdx.x;
+    const int n = a.degree, n1 = n + 1, Q = n1 * n1 * n1;
+    const int64_t pb = a.cl[5 * c + 2], c0 = a.cl[5 * c + 3], nch = a.cl[5 * c + 4];
+    const double* bx = a.box + 6 * c;
+    for (int q = threadIdx.x; q < Q; q += WG) {
+        const int kk[3] = {q / (n1 * n1), (q / n1) % n1, q % n1};
+        double s[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) s[k] = 0.0;
+        for (int64_t h = c0; h < c0 + nch; ++h) {
+            const double* w = a.ws + h * NC * Q + q;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) s[k] += w[k * Q];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a.px[d * a.P + pb + q] = bltc_node(bx, d, kk[d], n);
+            a.pf[d * a.P + pb + q] = s[d];
+            if (NF == 2) a.pfi[d * a.P + pb + q] = s[3 + d];
         }
     }
 }
@@ -1406,20 +1481,50 @@ int eval_launch(int kind, double alpha, int64_t M, int64_t N, const double* tgt,
 }
 
 
-// Proxy strengths of ncl clusters (cl: source begin, end, proxy base; box:
-// lo xyz, hi xyz) at interpolation degree n (1 .. BLTC_NMAX).
-int bltc_weights_launch(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t N, const double* src,
-                        const double* f, const double* fi, int64_t P, double* px, double* pf, double* pfi,
-                        hipStream_t stream)
+template <int N1>
+void bltc_moments_launch(int nf, int64_t nchunk, const BltcWeightArgs& a, hipStream_t stream)
+{
+    constexpr size_t lds1 = bltc_moments_lds<1, N1>(), lds2 = bltc_moments_lds<2, N1>();
+    static_assert(lds2 <= 65536, "moments kernel LDS");
+    if (nf == 2) hipLaunchKernelGGL((bltc_moments_kernel<2, N1>), dim3((unsigned)nchunk), dim3(WG), lds2, stream, a);
+    else hipLaunchKernelGGL((bltc_moments_kernel<1, N1>), dim3((unsigned)nchunk), dim3(WG), lds1, stream, a);
+}
+
+// Proxy strengths of ncl clusters at interpolation degree n (1 .. BLTC_NMAX):
+// cl per cluster (source begin, end, proxy base, first chunk, chunk count),
+// box (lo xyz, hi xyz); chunk per chunk (cluster, source begin, end), the
+// chunks of a cluster consecutive; ws holds nchunk 3 nf (n + 1)^3 doubles.
+int bltc_weights_launch(int64_t ncl, const int64_t* cl, const double* box, int degree, int64_t nchunk,
+                        const int64_t* chunk, int64_t N, const double* src, const double* f, const double* fi,
+                        int64_t P, double* px, double* pf, double* pfi, double* ws, hipStream_t stream)
 {
     FL_CHECK(degree >= 1 && degree <= BLTC_NMAX, "treecode degree must be 1 .. %d", BLTC_NMAX);
-    FL_CHECK(ncl >= 0 && N >= 0 && P >= 0, "negative size");
+    FL_CHECK(ncl >= 0 && nchunk >= 0 && N >= 0 && P >= 0, "negative size");
     FL_CHECK(!fi == !pfi, "imaginary strengths need imaginary proxy strengths");
     if (ncl == 0) return FL_OK;
-    FL_CHECK(cl && box && src && f && px && pf, "null pointer argument");
-    BltcWeightArgs a{cl, box, src, f, fi, px, pf, pfi, N, P, degree};
-    if (fi) hipLaunchKernelGGL(bltc_weights_kernel<2>, dim3((unsigned)ncl), dim3(WG), 0, stream, a);
-    else hipLaunchKernelGGL(bltc_weights_kernel<1>, dim3((unsigned)ncl), dim3(WG), 0, stream, a);
+    FL_CHECK(cl && box && px && pf, "null pointer argument");
+    FL_CHECK(nchunk == 0 || (chunk && src && f && ws), "null pointer argument");
+    BltcWeightArgs a{cl, chunk, box, src, f, fi, px, pf, pfi, ws, N, P, degree};
+    const int nf = fi ? 2 : 1;
+    if (nchunk > 0) {
+        switch (degree) {
+        case 1: bltc_moments_launch<2>(nf, nchunk, a, stream); break;
+        case 2: bltc_moments_launch<3>(nf, nchunk, a, stream); break;
+        case 3: bltc_moments_launch<4>(nf, nchunk, a, stream); break;
+        case 4: bltc_moments_launch<5>(nf, nchunk, a, stream); break;
+        case 5: bltc_moments_launch<6>(nf, nchunk, a, stream); break;
+        case 6: bltc_moments_launch<7>(nf, nchunk, a, stream); break;
+        case 7: bltc_moments_launch<8>(nf, nchunk, a, stream); break;
+        case 8: bltc_moments_launch<9>(nf, nchunk, a, stream); break;
+        case 9: bltc_moments_launch<10>(nf, nchunk, a, stream); break;
+        case 10: bltc_moments_launch<11>(nf, nchunk, a, stream); break;
+        case 11: bltc_moments_launch<12>(nf, nchunk, a, stream); break;
+        default: bltc_moments_launch<13>(nf, nchunk, a, stream); break;
+        }
+        FL_LAUNCHED();
+    }
+    if (nf == 2) hipLaunchKernelGGL(bltc_proxy_kernel<2>, dim3((unsigned)ncl), dim3(WG), 0, stream, a);
+    else hipLaunchKernelGGL(bltc_proxy_kernel<1>, dim3((unsigned)ncl), dim3(WG), 0, stream, a);
     FL_LAUNCHED();
     return FL_OK;
 }

@@ -124,13 +124,35 @@ def interaction_lists(tt: Tree, st: Tree, theta: float, nproxy: int):
     return (cat(dirB), cat(dirC)), (cat(appB), cat(appC))
 
 
+def chunk_table(begin: np.ndarray, end: np.ndarray, target: int = 16384):
+    """Splits each cluster's source range [begin, end) into chunks of at most
+    ``size`` sources, ``size`` = 4096 doubled until there are no more than
+    ``target`` chunks or every cluster is one chunk (enough workgroups to
+    fill the GPU several times, a bounded partial-strength workspace).
+    Returns per cluster the chunk count and first chunk, and the chunk rows
+    (cluster, begin, end)."""
+    begin = np.asarray(begin, dtype=np.int64)
+    end = np.asarray(end, dtype=np.int64)
+    n = end - begin
+    size = 4096
+    while np.sum(-(-n // size)) > target and size < n.max():
+        size *= 2
+    nch = -(-n // size)
+    first = (np.cumsum(nch) - nch).astype(np.int64)
+    cl = np.repeat(np.arange(n.size, dtype=np.int64), nch)
+    k = np.arange(cl.size, dtype=np.int64) - first[cl]
+    b = begin[cl] + k * size
+    chunk = np.stack([cl, b, np.minimum(b + size, end[cl])], axis=1).astype(np.int64)
+    return nch.astype(np.int64), first, chunk.reshape(-1, 3)
+
+
 class Treecode:
     """A treecode for fixed targets, sources and blob width: ``apply`` sums
     the kernel over new strengths with the same trees and lists (the GMRES
     matvec's use), ``evaluate_treecode`` builds one for a single sum."""
 
     def __init__(self, kernel, targets, sources, eps, degree: int = 8, theta: float = 0.7,
-                 leaf_size: int = 2048, device=None):
+                 leaf_size: int = 1024, device=None):
         kernel_struct(kernel)
         if not 1 <= int(degree) <= 12:
             raise ValueError("degree must be 1 .. 12")
@@ -151,12 +173,15 @@ class Treecode:
         (dB, dC), (aB, aC) = interaction_lists(self.tt, self.st, self.theta, nq)
         self.n_direct_pairs = int(np.sum((self.st.end - self.st.begin)[dC] * (self.tt.end - self.tt.begin)[dB]))
         self.n_proxy_pairs = int(np.sum((self.tt.end - self.tt.begin)[aB]) * nq)
-        # clusters with proxies and their first proxy
+        # clusters with proxies, their first proxy and their chunks (the
+        # moments kernel sums each chunk in a workgroup of its own)
         pc = np.unique(aC)
         self.P = int(pc.size * nq)
         base = np.full(self.st.begin.size, -1, np.int64)
         base[pc] = np.arange(pc.size, dtype=np.int64) * nq
-        cl = np.stack([self.st.begin[pc], self.st.end[pc], base[pc]], axis=1).astype(np.int64)
+        cb, ce = self.st.begin[pc], self.st.end[pc]
+        nch, first, chunk = chunk_table(cb, ce)
+        cl = np.stack([cb, ce, base[pc], first, nch], axis=1).astype(np.int64)
         box = np.concatenate([self.st.lo[pc], self.st.hi[pc]], axis=1)
         # segments per target leaf: direct runs (merged where contiguous), then proxies
         leaves = self.tt.leaves
@@ -194,6 +219,9 @@ class Treecode:
         t = D.torch()
         as64 = lambda a: t.from_numpy(np.ascontiguousarray(a, dtype=np.int64).reshape(-1)).to(dev)
         self._cl = as64(cl)
+        self.nchunk = int(chunk.shape[0])
+        self._chunk = as64(chunk)
+        self._ws = None
         self._box = t.from_numpy(np.ascontiguousarray(box, dtype=np.float64).reshape(-1)).to(dev)
         self._blk = as64(np.array(blk_rows, dtype=np.int64).reshape(-1, 4))
         self._seg = as64(np.array(seg_rows, dtype=np.int64).reshape(-1, 3))
@@ -222,9 +250,13 @@ class Treecode:
         if fi is not None and self._pfi is None:
             self._pfi = t.empty_like(self._pf)
         pfi = self._pfi if fi is not None else None
+        nws = max(self.nchunk, 1) * 3 * (2 if fi is not None else 1) * self.nproxy
+        if self._ws is None or self._ws.numel() < nws:
+            self._ws = t.empty(nws, dtype=t.float64, device=dev)
         sh = D.stream_handle(stream)
-        check(L.fl_bltc_weights(self.ncl, D.ptr(self._cl), D.ptr(self._box), self.degree, self.N, D.ptr(self._src),
-                                D.ptr(fr), D.ptr(fi), self.P, D.ptr(self._px), D.ptr(self._pf), D.ptr(pfi), sh),
+        check(L.fl_bltc_weights(self.ncl, D.ptr(self._cl), D.ptr(self._box), self.degree, self.nchunk,
+                                D.ptr(self._chunk), self.N, D.ptr(self._src), D.ptr(fr), D.ptr(fi), self.P,
+                                D.ptr(self._px), D.ptr(self._pf), D.ptr(pfi), D.ptr(self._ws), sh),
               "fl_bltc_weights")
         k = kernel_struct(self.kernel)
         check(L.fl_bltc_eval(k, self.M, D.ptr(self._tgt), D.ptr(self._tidx), self.nblk, D.ptr(self._blk),
@@ -235,7 +267,7 @@ class Treecode:
 
 
 def evaluate_treecode(kernel, targets, sources, strengths, eps, mu=1.0, degree: int = 8, theta: float = 0.7,
-                      leaf_size: int = 2048) -> np.ndarray:
+                      leaf_size: int = 1024) -> np.ndarray:
     """``evaluate`` (the reference's core/matvec.py:24-49 contract: shapes,
     dtypes, complex promotion) by the treecode: the far field of each source
     cluster through (degree+1)^3 proxies where (r_B + r_C) < theta |c_B - c_C|.

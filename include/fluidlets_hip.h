@@ -139,10 +139,16 @@ int fl_eval_noalloc(const fl_kernel* kernel, int64_t M, int64_t N,
  * structure-of-arrays, FP64.
  *
  * fl_bltc_weights: the proxies of ncl clusters.  cl holds per cluster the
- * source range [begin, end) in the sorted sources and the cluster's first
- * proxy index (3 int64); box its lo x, y, z, hi x, y, z (6 doubles).  Writes
- * px (3, P) (the Chebyshev points of the box) and pf (3, P) (the proxy
- * strengths; pfi for imaginary strengths fi, both or neither).  degree 1 .. 12.
+ * source range [begin, end) in the sorted sources, the cluster's first proxy
+ * index, its first chunk and its chunk count (5 int64); box its lo x, y, z,
+ * hi x, y, z (6 doubles).  chunk holds per chunk the cluster and a source
+ * range [begin, end) within it (3 int64), a cluster's chunks consecutive and
+ * covering its range; each chunk is summed by one workgroup into ws, which
+ * holds nchunk * 3 * nf * (degree + 1)^3 doubles (nf = 2 with f_im, else 1),
+ * and the chunks are then added in order (the result does not depend on
+ * timing).  Writes px (3, P) (the Chebyshev points of the box) and pf (3, P)
+ * (the proxy strengths; pf_im for imaginary strengths f_im, both or
+ * neither).  degree 1 .. 12.
  *
  * fl_bltc_eval: u = (1/mu) sum over each block's segments of G(x, y) f, with
  * one blob width eps (the proxies carry it too).  blk holds per block the
@@ -152,8 +158,9 @@ int fl_eval_noalloc(const fl_kernel* kernel, int64_t M, int64_t N,
  * (3, M), which is written in the caller's order.  Kernel, mu and complex
  * results as fl_eval. */
 int fl_bltc_weights(int64_t ncl, const int64_t* cl, const double* box, int degree,
+                    int64_t nchunk, const int64_t* chunk,
                     int64_t N, const double* src, const double* f_re, const double* f_im,
-                    int64_t P, double* px, double* pf_re, double* pf_im, void* stream);
+                    int64_t P, double* px, double* pf_re, double* pf_im, double* ws, void* stream);
 int fl_bltc_eval(const fl_kernel* kernel, int64_t M, const double* tgt, const int64_t* tidx,
                  int64_t nblk, const int64_t* blk, const int64_t* seg,
                  int64_t N, const double* src, const double* f_re, const double* f_im,

@@ -41,8 +41,11 @@ def test_theta_zero_is_the_direct_sum(kernel):
         assert rel_err(v, u) < 1e-13
 
 
-@pytest.mark.parametrize("degree", [1, 3, 6])
+@pytest.mark.parametrize("degree", [1, 3, 6, 8, 10, 12])
 def test_proxy_strengths_reproduce_the_cluster_moments(degree):
+    """Degrees 1 .. 12 take 64, 16, 5, 3, 2 and 1 groups of lanes per
+    workgroup (summed pairwise at the end); the largest clusters with
+    proxies (5,000 sources) are summed in two chunks of up to 4,096."""
     import torch
     print(f"case: treecode proxy moments degree {degree}", flush=True)
     X, F = slab(20000, 2)
@@ -53,10 +56,11 @@ def test_proxy_strengths_reproduce_the_cluster_moments(degree):
     torch.cuda.synchronize()
     px = tc._px.cpu().numpy()
     pf = tc._pf.cpu().numpy()
-    cl = tc._cl.cpu().numpy().reshape(-1, 3)
+    cl = tc._cl.cpu().numpy().reshape(-1, 5)
+    assert cl[:, 4].max() == 2                        # clusters of 5,000 sources in two chunks
     Xs, Fs = X[tc.st.order], F[tc.st.order]
     nq = (degree + 1) ** 3
-    for b, e, base in cl[:: max(1, len(cl) // 20)]:
+    for b, e, base, _, _ in cl[:: max(1, len(cl) // (20 if degree <= 6 else 4))]:
         y, fy = Xs[b:e], Fs[b:e]
         s, fs = px[:, base:base + nq].T, pf[:, base:base + nq].T
         c = 0.5 * (y.min(0) + y.max(0))
@@ -81,12 +85,12 @@ def test_error_falls_with_the_degree():
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=lambda k: type(k).__name__)
 def test_defaults_meet_the_stated_error(kernel):
-    """Degree 8, theta 0.7: relative error below 1e-5 against the direct sum,
-    for targets apart from the sources too."""
+    """Degree 8, theta 0.7, leaves of 1,024: relative error below 1e-5
+    against the direct sum, for targets apart from the sources too."""
     print(f"case: treecode defaults {type(kernel).__name__}", flush=True)
     X, F = slab(80000, 4)
     T = np.random.default_rng(6).uniform([-6, -6, 0.05], [6, 6, 6], (20000, 3))
     for tg in (X, T):
         u = fl.evaluate(kernel, tg, X, F, 0.02)
-        v = evaluate_treecode(kernel, tg, X, F, 0.02, leaf_size=1024)
+        v = evaluate_treecode(kernel, tg, X, F, 0.02)
         assert rel_err(v, u) < 1e-5

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import fluidlets_amd as fl
-from fluidlets_amd.treecode import Treecode, build_tree, interaction_lists
+from fluidlets_amd.treecode import Treecode, build_tree, chunk_table, interaction_lists
 
 
 def slab(n, seed):
@@ -73,3 +73,30 @@ def test_treecode_needs_one_blob_width_and_a_valid_degree():
         Treecode(fl.Stokeslet3D(), X, X, 0.01, degree=13)
     with pytest.raises(ValueError, match="theta"):
         interaction_lists(build_tree(X, 10), build_tree(X, 10), 1.0, 8)
+
+
+@pytest.mark.parametrize("target", [16384, 8])
+def test_chunks_cover_each_cluster_in_order(target):
+    """The moments kernel's chunks: consecutive per cluster, covering its
+    range once, 4,096 sources (or a doubling of it) at most, and no more than
+    ``target`` of them once the chunk size has grown."""
+    rng = np.random.default_rng(7)
+    begin = np.sort(rng.integers(0, 200000, 60))
+    end = begin + rng.integers(730, 70000, 60)
+    nch, first, ch = chunk_table(begin, end, target)
+    ln = ch[:, 2] - ch[:, 1]
+    last = first + nch - 1
+    inner = np.setdiff1d(np.arange(ch.shape[0]), last)            # every chunk but a cluster's last
+    size = int(ln[inner].max()) if inner.size else 1 << int(np.ceil(np.log2(max(ln.max(), 4096))))
+    assert size >= 4096 and size & (size - 1) == 0 and np.all(ln[inner] == size) and np.all(ln <= size)
+    assert ch.shape[0] == nch.sum() and (ch.shape[0] <= target or np.all(nch == 1))
+    for c in range(begin.size):
+        rows = ch[first[c]:first[c] + nch[c]]
+        assert np.all(rows[:, 0] == c)
+        assert rows[0, 1] == begin[c] and rows[-1, 2] == end[c]
+        assert np.all(rows[1:, 1] == rows[:-1, 2]) and np.all(rows[:, 2] > rows[:, 1])
+
+
+def test_no_clusters_no_chunks():
+    nch, first, ch = chunk_table(np.zeros(0, np.int64), np.zeros(0, np.int64))
+    assert nch.shape == first.shape == (0,) and ch.shape == (0, 3)
