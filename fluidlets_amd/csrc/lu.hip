@@ -663,17 +663,22 @@ __global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int 
 // four waves.  The 128 x 128 block is a 4 x 4 grid of 32 x 32 sub-blocks;
 // for kb = 0 .. 3:
 //   leaf    wave 0 factors sub-block (kb, kb), one row per lane, the pivot
-//           row broadcast with readlane; then wave 0 inverts its unit lower
-//           triangle and wave 1 its upper triangle (one column per lane, by
-//           substitution), IL_kb and IU_kb;
+//           row broadcast with readlane; then the four waves invert the
+//           16 x 16 diagonal triangles of its factors (one column per lane,
+//           by substitution) and waves 0, 1 form the off-diagonal 16 x 16
+//           blocks on MFMA: IL_kb = inv(L_kk), IU_kb = inv(U_kk);
 //   panels  L_ib = A_ib IU_kb (i > kb) and U_kj = IL_kb A_kj (j > kb), one
 //           32 x 32 x 32 MFMA product per wave;
 //   update  A_ij -= L_ik U_kj for i, j > kb.
-// The off-diagonal blocks of inv(L) and inv(U) then follow from products
-// only: with M_im = -IL_i L_im (i > m) and N_mj = -U_mj IU_j (m < j),
-//   inv(L)_ij = sum_{m = j}^{i - 1} M_im inv(L)_mj,  inv(U)_ij = sum_{m = i}^{j - 1} inv(U)_im N_mj,
-// block diagonals of inv(L) and inv(U) being IL_i and IU_i, level by level in
-// the distance from the diagonal (a barrier between levels).
+// The off-diagonal blocks of inv(L) and inv(U) follow block row by block
+// row (column by column for inv(U)), from L X = I and Y U = I:
+//   X_ij = -IL_i sum_{m = j}^{i - 1} L_im X_mj,  Y_ji = -(sum_{m = j}^{i - 1} Y_jm U_mi) IU_i,
+// with X_mm = IL_m, Y_mm = IU_m, so row i needs only rows above it.
+// Schedule: wave 0 carries the chain (update of diagonal block kb + 1 by
+// step kb, leaf LU kb + 1); while it factors a leaf, waves 1 .. 3 do the
+// other trailing updates of step kb and block row kb of X, Y; at the last
+// leaf they also form the sums of row 3 that need no IL_3 and store the
+// finished factor blocks.
 //
 // One wave's 32 x 32 MFMA accumulator (FP32: one v_mfma_f32_32x32x2_f32
 // block; FP64: 2 x 2 v_mfma_f64_16x16x4_f64 blocks).
@@ -784,6 +789,114 @@ struct Acc32<double> {
     }
 };
 
+// LU without interchanges of the 32 x 32 diagonal sub-block at (o, o) of S
+// (pitch ls) by one wave: row o + r on lane r (lanes 32 .. 63 repeat row
+// lane % 32 and store nothing), each pivot row broadcast with readlane (no
+// barrier per column).  Returns the lane's largest |multiplier| (infinite
+// for a zero pivot, left to the pivoting tiers).
+template <typename T>
+__device__ __forceinline__ T leaf_lu32(T* S, int ls, int o, int lane)
+{
+    T lmax = T(0);
+    T a[SUBB];
+    const int r = o + (lane & (SUBB - 1));
+#pragma unroll
+    for (int j = 0; j < SUBB; ++j) a[j] = S[r * ls + o + j];
+#pragma unroll
+    for (int c = 0; c < SUBB; ++c) {
+        const T pv = read_lane(a[c], c);
+        if (pv == T(0)) lmax = T(INFINITY);
+        const T rp = pv != T(0) ? recip(pv) : T(1);
+        const bool below = (lane & (SUBB - 1)) > c && lane < SUBB;
+        const T l = below ? a[c] * rp : T(0);
+        if (below) {
+            a[c] = l;
+            lmax = fmax(lmax, l < T(0) ? -l : l);
+        }
+        // the pivot row's values first, then the updates, sixteen columns at a
+        // time: a readlane right after the FMA that wrote its source, or an
+        // FMA right after the readlane it reads, costs wait states
+        T pr[SUBB];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int j = c + 1 + h * 16; j < SUBB && j < c + 17 + h * 16; ++j) pr[j] = read_lane(a[j], c);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = c + 1 + h * 16; j < SUBB && j < c + 17 + h * 16; ++j) a[j] = fma(-l, pr[j], a[j]);
+        }
+    }
+    if (lane < SUBB)
+#pragma unroll
+        for (int j = 0; j < SUBB; ++j) S[r * ls + o + j] = a[j];
+    return lmax;
+}
+
+// One wave's 16 x 16 MFMA accumulator over K = 16 (v_mfma_f32_16x16x4_f32 /
+// v_mfma_f64_16x16x4_f64: A[i][k] on lane i + 16 (k % 4), B[k][j] on lane
+// j + 16 (k % 4); the result's row i, column j is on lane j + 16 (i / 4),
+// register i % 4 in FP32, lane j + 16 (i % 4), register i / 4 in FP64), for
+// the 16 x 16 blocks of the leaf inverses.
+template <typename T>
+struct Acc16;
+
+template <>
+struct Acc16<float> {
+    typedef float floatx4 __attribute__((ext_vector_type(4)));
+    floatx4 c;
+    static __device__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+    __device__ void zero() { c = floatx4{0}; }
+    __device__ void mac(const float* A, int la, const float* B, int lb, float s, int lane)
+    {
+        const int r = lane & 15, kk = lane >> 4;
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(s * A[r * la + k0 + kk], B[(k0 + kk) * lb + r], c, 0, 0, 0);
+    }
+    __device__ void store(float* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[row(lane, i) * lp + (lane & 15)] = c[i];
+    }
+    __device__ void sub_store(float* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[row(lane, i) * lp + (lane & 15)] -= c[i];
+    }
+    __device__ float absmax() const
+    {
+        return fmaxf(fmaxf(fabsf(c[0]), fabsf(c[1])), fmaxf(fabsf(c[2]), fabsf(c[3])));
+    }
+};
+
+template <>
+struct Acc16<double> {
+    mfma::doublex4 c;
+    static __device__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
+    __device__ void zero() { c = mfma::doublex4{0}; }
+    __device__ void mac(const double* A, int la, const double* B, int lb, double s, int lane)
+    {
+        const int r = lane & 15, kk = lane >> 4;
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4)
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(s * A[r * la + k0 + kk], B[(k0 + kk) * lb + r], c, 0, 0, 0);
+    }
+    __device__ void store(double* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[row(lane, i) * lp + (lane & 15)] = c[i];
+    }
+    __device__ void sub_store(double* P, int lp, int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[row(lane, i) * lp + (lane & 15)] -= c[i];
+    }
+    __device__ double absmax() const
+    {
+        return fmax(fmax(fabs(c[0]), fabs(c[1])), fmax(fabs(c[2]), fabs(c[3])));
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int nb, int64_t k0, int32_t* ipiv,
                                                          unsigned int* lmax_bits, T* invL, T* invU, int64_t zb)
@@ -809,170 +922,231 @@ __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int 
     for (int c = tid; c < nb; c += 256) ipiv[k0 + c] = (int32_t)(k0 + c);
     __syncthreads();
     T lmax = T(0);
-    for (int kb = 0; kb < NBB; ++kb) {
-        const int o = kb * SUBB;
-        T* IL = ILs[KEEP ? kb : 0];
-        T* IU = IUs[KEEP ? kb : 0];
-        if (wave == 0) {
-            // leaf LU: rows o .. o + 31 by lanes 0 .. 31
-            T a[SUBB];
-            const int r = o + (lane & (SUBB - 1));
-#pragma unroll
-            for (int j = 0; j < SUBB; ++j) a[j] = S[r * LS + o + j];
-#pragma unroll
-            for (int c = 0; c < SUBB; ++c) {
-                const T pv = read_lane(a[c], c);
-                if (pv == T(0)) lmax = T(INFINITY);  // a zero pivot: leave it to the pivoting tiers
-                const T rp = pv != T(0) ? recip(pv) : T(1);
-                const bool below = (lane & (SUBB - 1)) > c && lane < SUBB;
-                const T l = below ? a[c] * rp : T(0);
-                if (below) {
-                    a[c] = l;
-                    lmax = fmax(lmax, l < T(0) ? -l : l);
-                }
-#pragma unroll
-                for (int j = c + 1; j < SUBB; ++j) a[j] = fma(-l, read_lane(a[j], c), a[j]);
-            }
-            if (lane < SUBB)
-#pragma unroll
-                for (int j = 0; j < SUBB; ++j) S[r * LS + o + j] = a[j];
+    // The inverses' off-diagonal blocks: X = inv(L) block (i, j), j < i, and
+    // Y = inv(U) block (j, i), have index q(i, j) = i (i - 1) / 2 + j (FP32:
+    // XLs / YUs; FP64: in place in invL / invU); the leaf inverses IL_i, IU_i
+    // are their diagonal blocks.
+    auto qi = [](int i, int j) { return i * (i - 1) / 2 + j; };
+    auto ilp = [&](int i) -> const T* { return KEEP ? ILs[i] : invL + i * SUBB * NB + i * SUBB; };
+    auto iup = [&](int i) -> const T* { return KEEP ? IUs[i] : invU + i * SUBB * NB + i * SUBB; };
+    constexpr int LP = KEEP ? LB : NB;   // pitch of those blocks
+    auto xp = [&](int i, int j) -> T* { return KEEP ? XLs[qi(i, j)] : invL + i * SUBB * NB + j * SUBB; };
+    auto yp = [&](int j, int i) -> T* { return KEEP ? YUs[qi(i, j)] : invU + j * SUBB * NB + i * SUBB; };
+    // A_ib -= L_ik U_kb at step k (the product summed from zero, then
+    // subtracted: one rounding at the scale of A_ib)
+    auto update = [&](int k, int ib, int jb) {
+        Acc32<T> acc;
+        acc.zero();
+        acc.mac(S + ib * SUBB * LS + k * SUBB, LS, S + k * SUBB * LS + jb * SUBB, LS, T(1), lane);
+        acc.sub_store(S + ib * SUBB * LS + jb * SUBB, LS, lane);
+    };
+    // Block row i of inv(L) and block column i of inv(U), task t < i:
+    //   X_it = -IL_i sum_{m = t}^{i - 1} L_im X_mt,
+    // task t >= i (j = t - i):
+    //   Y_ji = -(sum_{m = j}^{i - 1} Y_jm U_mi) IU_i,
+    // with X_mm = IL_m, Y_mm = IU_m: the sum into the output's place, then the
+    // outer product in place (the wave reads all its operands before it stores).
+    // (a wave's own earlier stores are read back by other lanes: fenced)
+    auto wave_fence = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
+    auto inv_sum = [&](int i, int t) {
+        wave_fence();
+        Acc32<T> acc;
+        acc.zero();
+        const bool lo = t < i;
+        const int j = lo ? t : t - i;
+        for (int m = j; m < i; ++m) {
+            if (lo) acc.mac(S + i * SUBB * LS + m * SUBB, LS, m == j ? ilp(j) : xp(m, j), LP, T(1), lane);
+            else acc.mac(m == j ? iup(j) : yp(j, m), LP, S + m * SUBB * LS + i * SUBB, LS, T(1), lane);
         }
-        __syncthreads();
-        const T* F = S + o * LS + o;
-        if (wave == 0 && lane < SUBB) {
-            // column `lane` of IL = inv(L_kk), unit lower
-            const int j = lane;
-            T x[SUBB];
-#pragma unroll
-            for (int i = 0; i < SUBB; ++i) {
-                T acc = T(0);
-#pragma unroll
-                for (int m = 0; m < i; ++m) acc = fma(F[i * LS + m], x[m], acc);
-                x[i] = i < j ? T(0) : (i == j ? T(1) : -acc);
-            }
-#pragma unroll
-            for (int i = 0; i < SUBB; ++i) {
-                IL[i * LB + j] = x[i];
-                if (!KEEP) invL[(o + i) * NB + o + j] = x[i];
-            }
-        } else if (wave == 1 && lane < SUBB) {
-            // column `lane` of IU = inv(U_kk); y[q] holds row SUBB - 1 - q
-            const int j = lane;
-            T y[SUBB];
-#pragma unroll
-            for (int q = 0; q < SUBB; ++q) {
-                const int i = SUBB - 1 - q;
-                T acc = T(0);
-#pragma unroll
-                for (int m = 0; m < q; ++m) acc = fma(F[i * LS + SUBB - 1 - m], y[m], acc);
-                const T rd = recip(F[i * LS + i]);
-                y[q] = i > j ? T(0) : (i == j ? rd : -acc * rd);
-            }
-#pragma unroll
-            for (int q = 0; q < SUBB; ++q) {
-                const int i = SUBB - 1 - q;
-                IU[i * LB + j] = y[q];
-                if (!KEEP) invU[(o + i) * NB + o + j] = y[q];
-            }
-        }
-        __syncthreads();
-        const int rest = NBB - 1 - kb;
-        // panels: L_ib = A_ib IU (p < rest), U_kj = IL A_kj (p >= rest)
-        for (int p = wave; p < 2 * rest; p += 4) {
-            Acc32<T> acc;
-            acc.zero();
-            if (p < rest) {
-                T* blk = S + (kb + 1 + p) * SUBB * LS + o;
-                acc.mac(blk, LS, IU, LB, T(1), lane);
-                acc.store(blk, LS, lane);
-                lmax = fmax(lmax, acc.absmax());
-            } else {
-                T* blk = S + o * LS + (kb + 1 + p - rest) * SUBB;
-                acc.mac(IL, LB, blk, LS, T(1), lane);
-                acc.store(blk, LS, lane);
-            }
-        }
-        __syncthreads();
-        // trailing sub-blocks: A_ij -= L_ik U_kj (the product summed from zero,
-        // then subtracted: one rounding at the scale of A_ij)
-        for (int p = wave; p < rest * rest; p += 4) {
-            const int ib = kb + 1 + p / rest, jb = kb + 1 + p % rest;
-            T* Cb = S + ib * SUBB * LS + jb * SUBB;
-            Acc32<T> acc;
-            acc.zero();
-            acc.mac(S + ib * SUBB * LS + o, LS, S + o * LS + jb * SUBB, LS, T(1), lane);
-            acc.sub_store(Cb, LS, lane);
-        }
-        __syncthreads();
-    }
-    // largest multiplier of the block -> *lmax_bits; the factors -> A
-    const T wl = wave_max(lmax);
-    if (lane == 0) atomicMax(lmax_bits, __float_as_uint((float)wl));
+        acc.store(lo ? xp(i, j) : yp(j, i), LP, lane);
+    };
+    auto inv_outer = [&](int i, int t) {
+        wave_fence();
+        Acc32<T> acc;
+        acc.zero();
+        const bool lo = t < i;
+        const int j = lo ? t : t - i;
+        T* out = lo ? xp(i, j) : yp(j, i);
+        if (lo) acc.mac(ilp(i), LP, out, LP, T(-1), lane);
+        else acc.mac(out, LP, iup(i), LP, T(-1), lane);
+        acc.store(out, LP, lane);
+    };
+    // The factors -> A, rows [r0, r1) x columns [c0, c1) of the block, by
+    // thread t0 of nt (16-byte stores when the block is whole and aligned; the
+    // LDS copy's rows are not 16-byte aligned: scalar reads)
     using VT = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n;
-    if (nb == NB && lda % VW == 0 && ((uintptr_t)A % 16) == 0) {
-        // 16-byte stores (the LDS copy's rows are not 16-byte aligned: scalar reads)
-        for (int idx = tid; idx < NB * NB / VW; idx += 256) {
-            const int rr = idx / (NB / VW), c = (idx % (NB / VW)) * VW;
+    const bool vec_store = nb == NB && lda % VW == 0 && ((uintptr_t)A % 16) == 0;
+    auto store_factors = [&](int r0, int r1, int c0, int c1, int t0, int nt) {
+        const int rv = (c1 - c0) / VW;
+        for (int idx = t0; idx < (r1 - r0) * rv; idx += nt) {
+            const int rr = r0 + idx / rv, c = c0 + (idx % rv) * VW;
             VT v;
             T* vp = reinterpret_cast<T*>(&v);
 #pragma unroll
             for (int q = 0; q < VW; ++q) vp[q] = S[rr * LS + c + q];
             *reinterpret_cast<VT*>(A + rr * lda + c) = v;
         }
+    };
+    // Each step kb opens with wave 0 updating diagonal block kb by step kb - 1
+    // and factoring its leaf, while waves 1 .. 3 do the rest of step kb - 1:
+    // its other trailing updates and block row / column kb - 1 of the
+    // inverses (at the last leaf also the sums of row / column NBB - 1 that
+    // need no IL_(NBB-1), and the finished factor blocks).  Then all four waves
+    // invert the leaf and form the panels of step kb.
+    static_assert(NBB == 4, "the last step's work lists are laid out for four leaves");
+    for (int kb = 0;; ++kb) {
+        const int o = kb * SUBB;
+        T* IL = ILs[KEEP ? kb : 0];
+        T* IU = IUs[KEEP ? kb : 0];
+        if (wave == 0) {
+            if (kb > 0) update(kb - 1, kb, kb);
+            lmax = fmax(lmax, leaf_lu32<T>(S, LS, o, lane));
+        } else if (kb > 0 && kb < NBB - 1) {
+            const int k = kb - 1, rest = NBB - kb;
+            const int nu = rest * rest - 1;
+#pragma unroll 1
+            for (int q = wave - 1; q < nu + 2 * k; q += 3) {
+                if (q < nu) {
+                    update(k, kb + (q + 1) / rest, kb + (q + 1) % rest);
+                } else {
+                    inv_sum(k, q - nu);
+                    inv_outer(k, q - nu);
+                }
+            }
+        } else if (kb == NBB - 1) {
+            // X_20, X_21, Y_02, Y_12 and the sums of X_3j, Y_j3 (each after the
+            // row-2 block it reads, on the same wave; (whole task?, i, t) per
+            // entry, 7 / 8 / 7 products); then the factors but block (3, 3)
+            constexpr signed char work[3][4][3] = {{{1, 2, 0}, {0, 3, 0}, {0, 3, 2}, {-1, 0, 0}},
+                                                   {{1, 2, 1}, {0, 3, 1}, {1, 2, 3}, {0, 3, 4}},
+                                                   {{1, 2, 2}, {0, 3, 3}, {0, 3, 5}, {-1, 0, 0}}};
+#pragma unroll 1
+            for (int e = 0; e < 4; ++e) {
+                const int whole = work[wave - 1][e][0], i = work[wave - 1][e][1], t = work[wave - 1][e][2];
+                if (whole < 0) break;
+                inv_sum(i, t);
+                if (whole) inv_outer(i, t);
+            }
+            if (vec_store) {
+                store_factors(0, o, 0, NB, tid - 64, 192);
+                store_factors(o, NB, 0, o, tid - 64, 192);
+            }
+        }
+        __syncthreads();
+        const T* F = S + o * LS + o;
+        {
+            // The leaf's triangles in 16 x 16 halves: wave 0 inverts L11, wave 1
+            // L22, wave 2 U11, wave 3 U22 (one column per lane, by
+            // substitution: chains of 120 dependent steps instead of 496);
+            // waves 0 and 2 also zero IL12 and IU21.
+            constexpr int H = SUBB / 2;
+            const int h = (wave & 1) * H;
+            const T* Fh = F + h * LS + h;
+            if (lane < H) {
+                const int j = lane;
+                if (wave < 2) {
+                    T x[H];
+#pragma unroll
+                    for (int i = 0; i < H; ++i) {
+                        T acc = T(0);
+#pragma unroll
+                        for (int m = 0; m < i; ++m) acc = fma(Fh[i * LS + m], x[m], acc);
+                        x[i] = i < j ? T(0) : (i == j ? T(1) : -acc);
+                    }
+#pragma unroll
+                    for (int i = 0; i < H; ++i) IL[(h + i) * LB + h + j] = x[i];
+                } else {
+                    // y[q] holds row H - 1 - q
+                    T y[H];
+#pragma unroll
+                    for (int q = 0; q < H; ++q) {
+                        const int i = H - 1 - q;
+                        T acc = T(0);
+#pragma unroll
+                        for (int m = 0; m < q; ++m) acc = fma(Fh[i * LS + H - 1 - m], y[m], acc);
+                        const T rd = recip(Fh[i * LS + i]);
+                        y[q] = i > j ? T(0) : (i == j ? rd : -acc * rd);
+                    }
+#pragma unroll
+                    for (int q = 0; q < H; ++q) IU[(h + H - 1 - q) * LB + h + j] = y[q];
+                }
+            } else if (lane < 2 * H && (wave & 1) == 0) {
+                const int j = lane - H;
+#pragma unroll
+                for (int i = 0; i < H; ++i) {
+                    if (wave == 0) IL[i * LB + H + j] = T(0);
+                    else IU[(H + i) * LB + j] = T(0);
+                }
+            }
+            __syncthreads();
+            // IL21 = -IL22 (L21 IL11) on wave 0, IU12 = -(IU11 U12) IU22 on
+            // wave 1: the inner product into the output's place, then the outer
+            // one in place (a wave reads all its operands before it stores)
+            if (wave < 2) {
+                Acc16<T> p;
+                p.zero();
+                if (wave == 0) p.mac(F + H * LS, LS, IL, LB, T(1), lane);
+                else p.mac(IU, LB, F + H, LS, T(1), lane);
+                T* out = wave == 0 ? IL + H * LB : IU + H;
+                p.store(out, LB, lane);
+                p.zero();
+                if (wave == 0) p.mac(IL + H * LB + H, LB, out, LB, T(-1), lane);
+                else p.mac(out, LB, IU + H * LB + H, LB, T(-1), lane);
+                p.store(out, LB, lane);
+            }
+        }
+        __syncthreads();
+        if (!KEEP) {
+            // FP64: the leaf inverses are the diagonal blocks of invL and invU
+            for (int e = tid; e < SUBB * SUBB; e += 256) {
+                const int i = e / SUBB, j = e % SUBB;
+                invL[(o + i) * NB + o + j] = IL[i * LB + j];
+                invU[(o + i) * NB + o + j] = IU[i * LB + j];
+            }
+        }
+        if (kb == NBB - 1) break;
+        const int rest = NBB - 1 - kb;
+        // panels L_bk = A_bk IU (task p even, b = kb + 1 + p / 2) and
+        // U_kb = IL A_kb (p odd): those of block kb + 1 first, on waves 0 and 1
+        for (int p = wave; p < 2 * rest; p += 4) {
+            const int b = kb + 1 + p / 2;
+            Acc32<T> acc;
+            acc.zero();
+            if ((p & 1) == 0) {
+                T* blk = S + b * SUBB * LS + o;
+                acc.mac(blk, LS, IU, LB, T(1), lane);
+                acc.store(blk, LS, lane);
+                lmax = fmax(lmax, acc.absmax());
+            } else {
+                T* blk = S + o * LS + b * SUBB;
+                acc.mac(IL, LB, blk, LS, T(1), lane);
+                acc.store(blk, LS, lane);
+            }
+        }
+        __syncthreads();
+    }
+    // the products by IL_3 and IU_3 of row / column 3 of the inverses (FP64
+    // reads IL_3, IU_3 from the copies in invL, invU); the factors of block
+    // (3, 3) (all of them when the block is partial)
+    if (!KEEP) __syncthreads();
+#pragma unroll 1
+    for (int t = wave; t < 2 * (NBB - 1); t += 4) inv_outer(NBB - 1, t);
+    if (vec_store) {
+        store_factors((NBB - 1) * SUBB, NB, (NBB - 1) * SUBB, NB, tid, 256);
     } else if (nb == NB) {
         for (int r = tid / NB; r < NB; r += 256 / NB) A[r * lda + tid % NB] = S[r * LS + tid % NB];
     } else {
         for (int r = 0; r < nb; ++r)
             for (int c = tid; c < nb; c += 256) A[r * lda + c] = S[r * LS + c];
     }
-    // The off-diagonal blocks of the inverses: block (i, j), 0 <= j < i < 4,
-    // has index q(i, j) = i (i - 1) / 2 + j; X (inv(L)) in XLs / invL, Y
-    // (inv(U), block (j, i)) in YUs / invU.
-    auto qi = [](int i, int j) { return i * (i - 1) / 2 + j; };
-    auto ilp = [&](int i) -> const T* { return KEEP ? ILs[i] : invL + i * SUBB * NB + i * SUBB; };
-    auto iup = [&](int i) -> const T* { return KEEP ? IUs[i] : invU + i * SUBB * NB + i * SUBB; };
-    constexpr int LP = KEEP ? LB : NB;   // pitch of those blocks
-    auto xp = [&](int i, int j) -> T* { return KEEP ? XLs[qi(i, j)] : invL + i * SUBB * NB + j * SUBB; };
-    auto yp = [&](int i, int j) -> T* { return KEEP ? YUs[qi(j, i)] : invU + i * SUBB * NB + j * SUBB; };
     __syncthreads();
-    // level 0: M_im = -IL_i L_im (i > m), N_mj = -U_mj IU_j (m < j), in place of L_im, U_mj
-    for (int p = wave; p < 12; p += 4) {
-        const int q = p % 6;
-        const int hi = q < 1 ? 1 : (q < 3 ? 2 : 3), lo = q < 1 ? 0 : (q < 3 ? q - 1 : q - 3);   // (1,0) (2,0) (2,1) (3,0) (3,1) (3,2)
-        Acc32<T> acc;
-        acc.zero();
-        if (p < 6) {
-            T* blk = S + hi * SUBB * LS + lo * SUBB;
-            acc.mac(ilp(hi), LP, blk, LS, T(-1), lane);
-            acc.store(blk, LS, lane);
-        } else {
-            T* blk = S + lo * SUBB * LS + hi * SUBB;
-            acc.mac(blk, LS, iup(hi), LP, T(-1), lane);
-            acc.store(blk, LS, lane);
-        }
-    }
-    __syncthreads();
-    // levels 1 .. 3: inv(L)_ij (i - j = d) and inv(U)_ji, one output block per wave
-    for (int d = 1; d < NBB; ++d) {
-        const int outs = NBB - d;
-        for (int p = wave; p < 2 * outs; p += 4) {
-            Acc32<T> acc;
-            acc.zero();
-            if (p < outs) {
-                const int j = p, i = p + d;
-                for (int m = j; m < i; ++m)
-                    acc.mac(S + i * SUBB * LS + m * SUBB, LS, m == j ? ilp(j) : xp(m, j), LP, T(1), lane);
-                acc.store(xp(i, j), LP, lane);
-            } else {
-                const int i = p - outs, j = i + d;
-                for (int m = i; m < j; ++m)
-                    acc.mac(m == i ? iup(i) : yp(i, m), LP, S + m * SUBB * LS + j * SUBB, LS, T(1), lane);
-                acc.store(yp(i, j), LP, lane);
-            }
-        }
-        __syncthreads();
-    }
+    // largest multiplier of the block -> *lmax_bits
+    const T wl = wave_max(lmax);
+    if (lane == 0) atomicMax(lmax_bits, __float_as_uint((float)wl));
     // the inverses -> invL, invU, block by block (FP32 from LDS; FP64's
     // nonzero blocks are already in place, only its zero blocks are written)
     for (int blk = 0; blk < NBB * NBB; ++blk) {
