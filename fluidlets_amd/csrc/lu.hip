@@ -691,13 +691,20 @@ struct Acc32<float> {
     static __device__ int row(int lane, int i) { return 8 * (i / 4) + 4 * (lane / 32) + (i % 4); }
     static __device__ int col(int lane) { return lane & 31; }
     __device__ void zero() { c = mfma::floatx16{0}; }
-    // c += s A B over K = 32: A (32 x 32, pitch la), B (32 x 32, pitch lb)
+    // c += s A B over K = 32: A (32 x 32, pitch la), B (32 x 32, pitch lb).
+    // Every operand is loaded before the first product: loaded one step
+    // ahead, each pair of products waited out a full LDS latency.
     __device__ void mac(const float* A, int la, const float* B, int lb, float s, int lane)
     {
         const int r = lane & 31, kk = lane >> 5;
+        float av[16], bv[16];
 #pragma unroll
-        for (int k0 = 0; k0 < 32; k0 += 2)
-            c = __builtin_amdgcn_mfma_f32_32x32x2f32(s * A[r * la + k0 + kk], B[(k0 + kk) * lb + r], c, 0, 0, 0);
+        for (int k = 0; k < 16; ++k) {
+            av[k] = A[r * la + 2 * k + kk];
+            bv[k] = B[(2 * k + kk) * lb + r];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) c = __builtin_amdgcn_mfma_f32_32x32x2f32(s * av[k], bv[k], c, 0, 0, 0);
     }
     __device__ void load(const float* P, int lp, int lane)
     {
@@ -736,17 +743,28 @@ struct Acc32<double> {
 #pragma unroll
             for (int b = 0; b < 2; ++b) c[a][b] = mfma::doublex4{0};
     }
+    // operands loaded a half (K = 16) at a time ahead of its products
     __device__ void mac(const double* A, int la, const double* B, int lb, double s, int lane)
     {
         const int r = lane & 15, kk = lane >> 4;
 #pragma unroll
-        for (int k0 = 0; k0 < 32; k0 += 4) {
-            const double a0 = s * A[r * la + k0 + kk], a1 = s * A[(16 + r) * la + k0 + kk];
-            const double b0 = B[(k0 + kk) * lb + r], b1 = B[(k0 + kk) * lb + 16 + r];
-            c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c[0][0], 0, 0, 0);
-            c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c[0][1], 0, 0, 0);
-            c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c[1][0], 0, 0, 0);
-            c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c[1][1], 0, 0, 0);
+        for (int h = 0; h < 32; h += 16) {
+            double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = h + 4 * q + kk;
+                a0[q] = A[r * la + k];
+                a1[q] = A[(16 + r) * la + k];
+                b0[q] = B[k * lb + r];
+                b1[q] = B[k * lb + 16 + r];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(s * a0[q], b0[q], c[0][0], 0, 0, 0);
+                c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(s * a0[q], b1[q], c[0][1], 0, 0, 0);
+                c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(s * a1[q], b0[q], c[1][0], 0, 0, 0);
+                c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(s * a1[q], b1[q], c[1][1], 0, 0, 0);
+            }
         }
     }
     __device__ void load(const double* P, int lp, int lane)
@@ -1047,28 +1065,44 @@ __global__ __launch_bounds__(256) void diag_block_kernel(T* A, int64_t lda, int 
             const int h = (wave & 1) * H;
             const T* Fh = F + h * LS + h;
             if (lane < H) {
+                // the triangle's entries are loaded into registers before the
+                // substitution (read as it went, each row waited out an LDS
+                // latency); f holds row i's entries m < i (L) or row
+                // H - 1 - i's entries H - 1 - m, m < i, then its diagonal (U)
+                constexpr int NT = H * (H - 1) / 2;
                 const int j = lane;
+                T f[NT + H];
                 if (wave < 2) {
+#pragma unroll
+                    for (int i = 1, e = 0; i < H; ++i)
+#pragma unroll
+                        for (int m = 0; m < i; ++m, ++e) f[e] = Fh[i * LS + m];
                     T x[H];
 #pragma unroll
-                    for (int i = 0; i < H; ++i) {
+                    for (int i = 0, e = 0; i < H; ++i) {
                         T acc = T(0);
 #pragma unroll
-                        for (int m = 0; m < i; ++m) acc = fma(Fh[i * LS + m], x[m], acc);
+                        for (int m = 0; m < i; ++m, ++e) acc = fma(f[e], x[m], acc);
                         x[i] = i < j ? T(0) : (i == j ? T(1) : -acc);
                     }
 #pragma unroll
                     for (int i = 0; i < H; ++i) IL[(h + i) * LB + h + j] = x[i];
                 } else {
+#pragma unroll
+                    for (int q = 1, e = 0; q < H; ++q)
+#pragma unroll
+                        for (int m = 0; m < q; ++m, ++e) f[e] = Fh[(H - 1 - q) * LS + H - 1 - m];
+#pragma unroll
+                    for (int i = 0; i < H; ++i) f[NT + i] = Fh[i * LS + i];
                     // y[q] holds row H - 1 - q
                     T y[H];
 #pragma unroll
-                    for (int q = 0; q < H; ++q) {
+                    for (int q = 0, e = 0; q < H; ++q) {
                         const int i = H - 1 - q;
                         T acc = T(0);
 #pragma unroll
-                        for (int m = 0; m < q; ++m) acc = fma(Fh[i * LS + H - 1 - m], y[m], acc);
-                        const T rd = recip(Fh[i * LS + i]);
+                        for (int m = 0; m < q; ++m, ++e) acc = fma(f[e], y[m], acc);
+                        const T rd = recip(f[NT + i]);
                         y[q] = i > j ? T(0) : (i == j ? rd : -acc * rd);
                     }
 #pragma unroll
