@@ -65,43 +65,60 @@ __device__ __forceinline__ double rsqrt_pos(double x)
     return fma(y0 * e, fma(e, 0.375, 0.5), y0);
 }
 
-// 2^(j/64), j = 0..63, each the double nearest the exact value (computed to
+// 2^(j/128), j = 0..127, each the double nearest the exact value (computed to
 // 60 digits and rounded once).
-__constant__ double kExp2Frac[64] = {
-    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
-    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
-    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
-    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
-    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
-    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
-    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
-    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
-    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
-    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
-    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
-    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
-    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
-    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
-    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
-    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+__constant__ double kExp2Frac[128] = {
+    0x1.0000000000000p+0  , 0x1.0163da9fb3335p+0  , 0x1.02c9a3e778061p+0  , 0x1.04315e86e7f85p+0,
+    0x1.059b0d3158574p+0  , 0x1.0706b29ddf6dep+0  , 0x1.0874518759bc8p+0  , 0x1.09e3ecac6f383p+0,
+    0x1.0b5586cf9890fp+0  , 0x1.0cc922b7247f7p+0  , 0x1.0e3ec32d3d1a2p+0  , 0x1.0fb66affed31bp+0,
+    0x1.11301d0125b51p+0  , 0x1.12abdc06c31ccp+0  , 0x1.1429aaea92de0p+0  , 0x1.15a98c8a58e51p+0,
+    0x1.172b83c7d517bp+0  , 0x1.18af9388c8deap+0  , 0x1.1a35beb6fcb75p+0  , 0x1.1bbe084045cd4p+0,
+    0x1.1d4873168b9aap+0  , 0x1.1ed5022fcd91dp+0  , 0x1.2063b88628cd6p+0  , 0x1.21f49917ddc96p+0,
+    0x1.2387a6e756238p+0  , 0x1.251ce4fb2a63fp+0  , 0x1.26b4565e27cddp+0  , 0x1.284dfe1f56381p+0,
+    0x1.29e9df51fdee1p+0  , 0x1.2b87fd0dad990p+0  , 0x1.2d285a6e4030bp+0  , 0x1.2ecafa93e2f56p+0,
+    0x1.306fe0a31b715p+0  , 0x1.32170fc4cd831p+0  , 0x1.33c08b26416ffp+0  , 0x1.356c55f929ff1p+0,
+    0x1.371a7373aa9cbp+0  , 0x1.38cae6d05d866p+0  , 0x1.3a7db34e59ff7p+0  , 0x1.3c32dc313a8e5p+0,
+    0x1.3dea64c123422p+0  , 0x1.3fa4504ac801cp+0  , 0x1.4160a21f72e2ap+0  , 0x1.431f5d950a897p+0,
+    0x1.44e086061892dp+0  , 0x1.46a41ed1d0057p+0  , 0x1.486a2b5c13cd0p+0  , 0x1.4a32af0d7d3dep+0,
+    0x1.4bfdad5362a27p+0  , 0x1.4dcb299fddd0dp+0  , 0x1.4f9b2769d2ca7p+0  , 0x1.516daa2cf6642p+0,
+    0x1.5342b569d4f82p+0  , 0x1.551a4ca5d920fp+0  , 0x1.56f4736b527dap+0  , 0x1.58d12d497c7fdp+0,
+    0x1.5ab07dd485429p+0  , 0x1.5c9268a5946b7p+0  , 0x1.5e76f15ad2148p+0  , 0x1.605e1b976dc09p+0,
+    0x1.6247eb03a5585p+0  , 0x1.6434634ccc320p+0  , 0x1.6623882552225p+0  , 0x1.68155d44ca973p+0,
+    0x1.6a09e667f3bcdp+0  , 0x1.6c012750bdabfp+0  , 0x1.6dfb23c651a2fp+0  , 0x1.6ff7df9519484p+0,
+    0x1.71f75e8ec5f74p+0  , 0x1.73f9a48a58174p+0  , 0x1.75feb564267c9p+0  , 0x1.780694fde5d3fp+0,
+    0x1.7a11473eb0187p+0  , 0x1.7c1ed0130c132p+0  , 0x1.7e2f336cf4e62p+0  , 0x1.80427543e1a12p+0,
+    0x1.82589994cce13p+0  , 0x1.8471a4623c7adp+0  , 0x1.868d99b4492edp+0  , 0x1.88ac7d98a6699p+0,
+    0x1.8ace5422aa0dbp+0  , 0x1.8cf3216b5448cp+0  , 0x1.8f1ae99157736p+0  , 0x1.9145b0b91ffc6p+0,
+    0x1.93737b0cdc5e5p+0  , 0x1.95a44cbc8520fp+0  , 0x1.97d829fde4e50p+0  , 0x1.9a0f170ca07bap+0,
+    0x1.9c49182a3f090p+0  , 0x1.9e86319e32323p+0  , 0x1.a0c667b5de565p+0  , 0x1.a309bec4a2d33p+0,
+    0x1.a5503b23e255dp+0  , 0x1.a799e1330b358p+0  , 0x1.a9e6b5579fdbfp+0  , 0x1.ac36bbfd3f37ap+0,
+    0x1.ae89f995ad3adp+0  , 0x1.b0e07298db666p+0  , 0x1.b33a2b84f15fbp+0  , 0x1.b59728de5593ap+0,
+    0x1.b7f76f2fb5e47p+0  , 0x1.ba5b030a1064ap+0  , 0x1.bcc1e904bc1d2p+0  , 0x1.bf2c25bd71e09p+0,
+    0x1.c199bdd85529cp+0  , 0x1.c40ab5fffd07ap+0  , 0x1.c67f12e57d14bp+0  , 0x1.c8f6d9406e7b5p+0,
+    0x1.cb720dcef9069p+0  , 0x1.cdf0b555dc3fap+0  , 0x1.d072d4a07897cp+0  , 0x1.d2f87080d89f2p+0,
+    0x1.d5818dcfba487p+0  , 0x1.d80e316c98398p+0  , 0x1.da9e603db3285p+0  , 0x1.dd321f301b460p+0,
+    0x1.dfc97337b9b5fp+0  , 0x1.e264614f5a129p+0  , 0x1.e502ee78b3ff6p+0  , 0x1.e7a51fbc74c83p+0,
+    0x1.ea4afa2a490dap+0  , 0x1.ecf482d8e67f1p+0  , 0x1.efa1bee615a27p+0  , 0x1.f252b376bba97p+0,
+    0x1.f50765b6e4540p+0  , 0x1.f7bfdad9cbe14p+0  , 0x1.fa7c1819e90d8p+0  , 0x1.fd3c22b8f71f1p+0,
 };
 
-// LDS of a Brinkmanlet kernel: the table of exp_neg (64 doubles; the lanes
+// LDS of a Brinkmanlet kernel: the table of exp_neg (128 doubles; the lanes
 // of a wave index it at different j, which LDS serves in one pass) and the
 // series coefficients c1, c2 (read only on the rare near-pair path, so they
 // hold no registers across the pair loop).
-constexpr int BRINK_LDS = 64 + 2 * NTERMS + 1;
-constexpr int BRINK_C24 = 64 + 2 * NTERMS;   // 1/24, the exp polynomial's coefficient kept in a vector register
+constexpr int BRINK_TAB = 128;
+constexpr int BRINK_LDS = BRINK_TAB + 2 * NTERMS + 1;
+constexpr int BRINK_C3 = BRINK_TAB + 2 * NTERMS;   // c3 of the exp polynomial, kept in a vector register
 
 __device__ __forceinline__ void stage_brinkman(double* bl)
 {
     const int i = threadIdx.x;
-    if (i < 64) bl[i] = kExp2Frac[i];
+    if (i < BRINK_TAB) bl[i] = kExp2Frac[i];
     if (i < NTERMS) {
-        bl[64 + i] = kSeries.c1[i];
-        bl[64 + NTERMS + i] = kSeries.c2[i];
+        bl[BRINK_TAB + i] = kSeries.c1[i];
+        bl[BRINK_TAB + NTERMS + i] = kSeries.c2[i];
     }
-    if (i == 0) bl[BRINK_C24] = 1.0 / 24.0;
+    if (i == 0) bl[BRINK_C3] = 0x1.55555b7bae95fp-3;
 }
 
 // a * b + c, one v_fma_f64 with b from a scalar and c from a vector register
@@ -114,28 +131,25 @@ __device__ __forceinline__ double fma_sv(double a, double b, double c)
     return d;
 }
 
-__device__ __forceinline__ double fma_vs(double a, double b, double c)
-{
-    double d;
-    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
-    return d;
-}
-
 // e^{-z} for z >= 0, from the LDS block etab of a Brinkmanlet kernel
-// (etab[j] = 2^(j/64), etab[BRINK_C24] = 1/24): with
-// m = rint(-64 z / ln 2) = 64 n + j (0 <= j < 64, n = floor(m / 64)),
-//   e^{-z} = 2^n * 2^(j/64) * e^r,   r = -z - m ln2/64,  |r| <= ln2/128,
-// where m ln2/64 is taken with ln2/64 split in a high and a low part (the
-// two fused multiply-adds leave r exact to about 1e-19), and e^r is its
-// degree-5 Taylor polynomial (the first omitted term r^6/720 < 4e-17).  The
-// result is within about 2 ulp.  m is rounded by adding 1.5 * 2^52 (the sum's
-// low word is then m as a two's-complement integer).  z is clamped at 1000,
-// where e^{-z} underflows to 0 anyway.
+// (etab[j] = 2^(j/128), etab[BRINK_C3] = c3): with
+// m = rint(-128 z / ln 2) = 128 n + j (0 <= j < 128, n = floor(m / 128)),
+//   e^{-z} = 2^n * 2^(j/128) * e^r,   r = -z - m ln2/128,  |r| <= ln2/256,
+// where m ln2/128 is taken with ln2/128 split in a high and a low part (the
+// two fused multiply-adds leave r exact to about 1e-19), and e^r is
+//   1 + r + r^2 / 2 + c3 r^3 + c4 r^4,
+// c3 and c4 fitted at the three Chebyshev nodes of [-ln2/256, ln2/256] for
+// (e^r - 1 - r) / r^2 (relative error 3.0e-16 over the interval, against
+// 1.2e-15 for the Taylor coefficients 1/6, 1/24; the 64-entry table needed
+// degree 5, one more fused multiply-add).  The result is within about 2 ulp.
+// m is rounded by adding 1.5 * 2^52 (the
+// sum's low word is then m as a two's-complement integer).  z is clamped at
+// 1000, where e^{-z} underflows to 0 anyway.
 __device__ __forceinline__ double exp_neg(double z, const double* etab)
 {
-    constexpr double kScale = -92.33248261689366;       // -64 / ln 2
-    constexpr double kLhi = 0.010830424696249145;       // ln 2 / 64, high part
-    constexpr double kLlo = 3.623510646634843e-19;      // ln 2 / 64 - kLhi
+    constexpr double kScale = -184.6649652337873;      // -128 / ln 2
+    constexpr double kLhi = 0.0054152123481245725;      // ln 2 / 128, high part
+    constexpr double kLlo = 1.8117553233174215e-19;     // ln 2 / 128 - kLhi
     constexpr double kShift = 6755399441055744.0;       // 1.5 * 2^52
     asm("v_min_f64 %0, %1, %2" : "=v"(z) : "v"(z), "s"(1000.0));   // z is never NaN here
     const double ms = fma(z, kScale, kShift);
@@ -143,13 +157,12 @@ __device__ __forceinline__ double exp_neg(double z, const double* etab)
     const int mi = __double2loint(ms);
     double r = fma(m, -kLhi, -z);
     r = fma(m, -kLlo, r);
-    const double T = etab[mi & 63];
-    double p = fma_sv(r, 1.0 / 120.0, etab[BRINK_C24]);
-    p = fma_vs(p, r, 1.0 / 6.0);
+    const double T = etab[mi & (BRINK_TAB - 1)];
+    double p = fma_sv(r, 0x1.5555596ee628dp-5, etab[BRINK_C3]);   // c4 r + c3
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    return __builtin_amdgcn_ldexp(T * p, mi >> 6);
+    return __builtin_amdgcn_ldexp(T * p, mi >> 7);
 }
 
 // ---------------------------------------------------------------- FP64 pairs
@@ -271,7 +284,7 @@ __device__ __forceinline__ void brinkman_series_lds(double z, const double* bl, 
 {
     int k = 0;
     asm volatile("" : "+s"(k));   // the coefficient reads stay inside the calling branch
-    const double* c1 = bl + 64 + k;
+    const double* c1 = bl + BRINK_TAB + k;
     const double* c2 = c1 + NTERMS;
     b1 = 0.0;
     b2 = 0.0;
