@@ -699,7 +699,7 @@ def main():
         result["rod_carpet_gmres"] = gmres_section(torch, fl, D)
         progress("rod-carpet GMRES done")
 
-    if not args.no_treecode:
+    if not args.no_treecode and not args.small_solves_only:
         result["treecode_2m"] = treecode_section(torch, fl, D)
         progress(f"treecode at 2,097,152 points: {result['treecode_2m']['treecode_s']:.3f} s against "
                  f"{result['treecode_2m']['direct_s']:.3f} s direct, error {result['treecode_2m']['rel_err_vs_direct']:.1e}")
