@@ -54,6 +54,7 @@ struct GemmArgs {
     int lower;    // only the tiles on or below the diagonal (M == N), enumerated densely
     int ntiles;   // tiles of the product (all, or the lower ones)
     int64_t zb;   // grouped launches: problem blockIdx.z's pointers are z zb bytes on
+    int tm;       // tile height: 128, or 64 for narrow products (gemm_tile's WMT = 1)
 };
 
 // This workgroup's problem of a grouped launch (blockIdx.z).
@@ -111,61 +112,66 @@ __device__ __forceinline__ bool tile_of(const GemmArgs<T>& g, int id, int& tm, i
     return true;
 }
 
-// One 128 x 128 tile of C for workgroup-local tile index `id` of problem g.
-template <typename T>
-__device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
+// One TMW x 128 tile of C for workgroup-local tile index `id` of problem g:
+// WMT = 2 waves along M (TMW = 128, each wave 64 x 64) or WMT = 1 (TMW = 64,
+// the four waves side by side, each 64 x 32: twice the workgroups for a
+// product too narrow to fill the GPU with 128-row tiles).
+template <typename T, int WMT = 2>
+__device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id, T* As, T* Bs)
 {
     using P = Mma<T>;
+    constexpr int TMW = 64 * WMT;        // tile height
+    constexpr int WNT = 4 / WMT;         // waves along N
+    constexpr int WCOL = TN / WNT;       // columns per wave
+    constexpr int NBI = 64 / P::B, NBJ = WCOL / P::B;   // MFMA blocks per wave
     constexpr int KT = P::KT;
     constexpr int VW = P::VW;
-    constexpr int LA = TM + 1;         // padded LDS row (A is stored k-major)
-    constexpr int EA = TM * KT / WG;   // A elements per lane per tile
+    constexpr int LA = TMW + 1;        // padded LDS row (A is stored k-major)
+    constexpr int EA = TMW * KT / WG;  // A elements per lane per tile
     constexpr int EB = TN * KT / WG;
     constexpr int VA = EA / VW;        // 16-byte loads per lane per tile
     constexpr int VB = EB / VW;
     constexpr int AV_ROW = KT / VW;    // vectors per A tile row
     constexpr int BV_ROW = TN / VW;    // vectors per B tile row
-    __shared__ T As[KT * LA];
-    __shared__ __attribute__((aligned(16))) T Bs[KT * TN];
 
     const int tid = threadIdx.x;
     const int lane = tid % 64, wave = tid / 64;
-    const int wm = wave / 2, wn = wave % 2;
+    const int wm = wave / WNT, wn = wave % WNT;
     int tm, tn;
     if (!tile_of(g, id, tm, tn)) return;
-    const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
-    const bool full_mn = g.aligned && m0 + TM <= g.M && n0 + TN <= g.N;
+    const int64_t m0 = (int64_t)tm * TMW, n0 = (int64_t)tn * TN;
+    const bool full_mn = g.aligned && m0 + TMW <= g.M && n0 + TN <= g.N;
 
-    typename P::acc_t acc[P::NB][P::NB];
+    typename P::acc_t acc[NBI][NBJ];
     const bool use_c = g.beta != T(0) && g.alpha != T(0);
     const bool pre_c = use_c && sizeof(T) == 8 && (g.alpha == T(1) || g.alpha == T(-1));   // (beta / alpha) = alpha beta exactly
     const bool post_c = use_c && !pre_c;   // C joins once, in the epilogue
     const T cscale = pre_c ? g.alpha * g.beta : T(0);
     // interior tiles take unpredicated accesses from one lane base pointer
-    const bool full_c = m0 + TM <= g.M && n0 + TN <= g.N;
-    T* cbase = g.C + (m0 + wm * 64 + P::row_lane(lane)) * g.ldc + n0 + wn * 64 + P::d_col(lane);
+    const bool full_c = m0 + TMW <= g.M && n0 + TN <= g.N;
+    T* cbase = g.C + (m0 + wm * 64 + P::row_lane(lane)) * g.ldc + n0 + wn * WCOL + P::d_col(lane);
     auto coff = [&](int i, int j, int v) -> int64_t { return (int64_t)(i * P::B + P::row_acc(v)) * g.ldc + j * P::B; };
 #pragma unroll
-    for (int i = 0; i < P::NB; ++i)
+    for (int i = 0; i < NBI; ++i)
 #pragma unroll
-        for (int j = 0; j < P::NB; ++j) acc[i][j] = P::zero();
+        for (int j = 0; j < NBJ; ++j) acc[i][j] = P::zero();
     if (pre_c) {
         if (full_c) {
 #pragma unroll
-            for (int i = 0; i < P::NB; ++i)
+            for (int i = 0; i < NBI; ++i)
 #pragma unroll
-                for (int j = 0; j < P::NB; ++j)
+                for (int j = 0; j < NBJ; ++j)
 #pragma unroll
                     for (int v = 0; v < P::NACC; ++v) acc[i][j][v] = cscale * cbase[coff(i, j, v)];
         } else {
 #pragma unroll
-            for (int i = 0; i < P::NB; ++i)
+            for (int i = 0; i < NBI; ++i)
 #pragma unroll
-                for (int j = 0; j < P::NB; ++j)
+                for (int j = 0; j < NBJ; ++j)
 #pragma unroll
                     for (int v = 0; v < P::NACC; ++v) {
                         int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
-                        int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
+                        int64_t c = n0 + wn * WCOL + j * P::B + P::d_col(lane);
                         acc[i][j][v] = (r < g.M && c < g.N) ? cscale * g.C[r * g.ldc + c] : T(0);
                     }
         }
@@ -235,15 +241,15 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
 #pragma unroll
         for (int kk = 0; kk < KT; kk += P::KS) {
             const int k = kk + P::a_k(lane);
-            T a[P::NB], b[P::NB];
+            T a[NBI], b[NBJ];
 #pragma unroll
-            for (int i = 0; i < P::NB; ++i) a[i] = As[k * LA + wm * 64 + i * P::B + P::a_row(lane)];
+            for (int i = 0; i < NBI; ++i) a[i] = As[k * LA + wm * 64 + i * P::B + P::a_row(lane)];
 #pragma unroll
-            for (int j = 0; j < P::NB; ++j) b[j] = Bs[k * TN + wn * 64 + j * P::B + P::a_row(lane)];
+            for (int j = 0; j < NBJ; ++j) b[j] = Bs[k * TN + wn * WCOL + j * P::B + P::a_row(lane)];
 #pragma unroll
-            for (int i = 0; i < P::NB; ++i)
+            for (int i = 0; i < NBI; ++i)
 #pragma unroll
-                for (int j = 0; j < P::NB; ++j) acc[i][j] = P::mma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < NBJ; ++j) acc[i][j] = P::mma(a[i], b[j], acc[i][j]);
         }
     }
 
@@ -254,16 +260,16 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
         // (with a run-time ldc the compiler cannot tell C's entries apart and
         // would otherwise wait for each load before the store after it)
 #pragma unroll
-        for (int i = 0; i < P::NB; ++i) {
+        for (int i = 0; i < NBI; ++i) {
             __builtin_amdgcn_sched_barrier(0);
-            T cv[P::NB][P::NACC];
+            T cv[NBJ][P::NACC];
             if (post_c)
 #pragma unroll
-                for (int j = 0; j < P::NB; ++j)
+                for (int j = 0; j < NBJ; ++j)
 #pragma unroll
                     for (int v = 0; v < P::NACC; ++v) cv[j][v] = cbase[coff(i, j, v)];
 #pragma unroll
-            for (int j = 0; j < P::NB; ++j)
+            for (int j = 0; j < NBJ; ++j)
 #pragma unroll
                 for (int v = 0; v < P::NACC; ++v) {
                     acc[i][j][v] = post_c ? outscale * acc[i][j][v] + g.beta * cv[j][v] : outscale * acc[i][j][v];
@@ -272,21 +278,21 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
         }
         if (g.absmax_bits) {
 #pragma unroll
-            for (int i = 0; i < P::NB; ++i)
+            for (int i = 0; i < NBI; ++i)
 #pragma unroll
-                for (int j = 0; j < P::NB; ++j)
+                for (int j = 0; j < NBJ; ++j)
 #pragma unroll
                     for (int v = 0; v < P::NACC; ++v) local_max = fmaxf(local_max, (float)fabs((double)acc[i][j][v]));
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < P::NB; ++i)
+        for (int i = 0; i < NBI; ++i)
 #pragma unroll
-            for (int j = 0; j < P::NB; ++j)
+            for (int j = 0; j < NBJ; ++j)
 #pragma unroll
                 for (int v = 0; v < P::NACC; ++v) {
                     int64_t r = m0 + wm * 64 + i * P::B + P::d_row(lane, v);
-                    int64_t c = n0 + wn * 64 + j * P::B + P::d_col(lane);
+                    int64_t c = n0 + wn * WCOL + j * P::B + P::d_col(lane);
                     if (r < g.M && c < g.N) {
                         T val = outscale * acc[i][j][v];
                         if (post_c) val += g.beta * g.C[r * g.ldc + c];
@@ -302,11 +308,18 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int id)
     }
 }
 
+// LDS of a tile (either height; As k-major with a padded row, Bs 16-byte aligned)
+#define GEMM_TILE_LDS(T)                                                                  \
+    __shared__ T As[Mma<T>::KT * (TM + 1)];                                               \
+    __shared__ __attribute__((aligned(16))) T Bs[Mma<T>::KT * TN]
+
 template <typename T>
 __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
 {
+    GEMM_TILE_LDS(T);
     to_problem(g);
-    gemm_tile(g, (int)blockIdx.x);
+    if (g.tm == 64) gemm_tile<T, 1>(g, (int)blockIdx.x, As, Bs);
+    else gemm_tile<T, 2>(g, (int)blockIdx.x, As, Bs);
 }
 
 // Two independent products in one launch (a grouped launch): workgroups
@@ -316,11 +329,17 @@ __global__ __launch_bounds__(WG, 2) void gemm_kernel(GemmArgs<T> g)
 template <typename T>
 __global__ __launch_bounds__(WG, 2) void gemm_pair_kernel(GemmArgs<T> g1, GemmArgs<T> g2, int tiles1)
 {
+    GEMM_TILE_LDS(T);
     const int id = (int)blockIdx.x;
     to_problem(g1);
     to_problem(g2);
-    if (id < tiles1) gemm_tile(g1, id);
-    else gemm_tile(g2, id - tiles1);
+    if (id < tiles1) {
+        if (g1.tm == 64) gemm_tile<T, 1>(g1, id, As, Bs);
+        else gemm_tile<T, 2>(g1, id, As, Bs);
+    } else {
+        if (g2.tm == 64) gemm_tile<T, 1>(g2, id - tiles1, As, Bs);
+        else gemm_tile<T, 2>(g2, id - tiles1, As, Bs);
+    }
 }
 
 // C = beta C (the alpha == 0 case)
@@ -333,6 +352,16 @@ __global__ __launch_bounds__(WG) void scale_kernel(T* C, int64_t ldc, int64_t M,
     *p = beta == T(0) ? T(0) : beta * *p;
 }
 
+// FL_GEMM_NARROW=0 keeps 128-row tiles for every product (a measurement switch).
+bool narrow_tiles()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FL_GEMM_NARROW");
+        return !(e && *e == '0');
+    }();
+    return on;
+}
+
 template <typename T>
 GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
                       T beta, T* C, int64_t ldc, unsigned int* absmax_bits)
@@ -342,10 +371,15 @@ GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int6
                         (lda % vw == 0) && (ldb % vw == 0);
     static const char* sw_env = std::getenv("FL_GEMM_SWIZZLE");
     const int swizzle = sw_env && *sw_env ? std::atoi(sw_env) : 1;
-    const int tiles_m = (int)((M + TM - 1) / TM), tiles_n = (int)((N + TN - 1) / TN);
+    // 64-row tiles for a single product whose 128-row tiles would leave most
+    // CUs idle (the panel products of a large factorization); grouped launches
+    // of many small problems keep 128-row tiles, which measured faster there
+    const int64_t t128 = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+    const int tm = narrow_tiles() && batch().count == 1 && t128 < 2 * device_cus() && M > 64 ? 64 : TM;
+    const int tiles_m = (int)((M + tm - 1) / tm), tiles_n = (int)((N + TN - 1) / TN);
     return GemmArgs<T>{M,     N,          K,           alpha,   beta,    A,       lda,   B,
                        ldb,   C,          ldc,         absmax_bits, aligned, tiles_m, tiles_n, swizzle,
-                       0,     tiles_m * tiles_n, batch().zb};
+                       0,     tiles_m * tiles_n, batch().zb, tm};
 }
 
 }  // namespace
@@ -412,6 +446,8 @@ int gemm_lower_update(int64_t n, int64_t K, const T* A, int64_t lda, const T* B,
     FL_CHECK(n >= 0 && K >= 0, "negative GEMM size");
     if (n == 0) return FL_OK;
     GemmArgs<T> g = gemm_args<T>(n, n, K, T(-1), A, lda, B, ldb, T(1), C, ldc, nullptr);
+    g.tm = TM;   // the lower enumeration is of square tiles
+    g.tiles_m = (int)((n + TM - 1) / TM);
     g.lower = 1;
     g.ntiles = g.tiles_m * (g.tiles_m + 1) / 2;
     const int launch = (g.ntiles + NXCD - 1) / NXCD * NXCD;

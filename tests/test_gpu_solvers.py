@@ -397,10 +397,11 @@ def test_grand_resistance_six_modes():
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (33, 17, 5), (128, 128, 128), (129, 1, 128), (300, 200, 130),
-                                   (64, 513, 40)])
+                                   (64, 513, 40), (2100, 4100, 24)])
 def test_mfma_gemm_matches_numpy(dtype, M, N, K):
     """C = alpha A B + beta C through fl_sgemm / fl_dgemm (pins the MFMA lane
-    layouts and the tile edges)."""
+    layouts and the tile edges).  Products of fewer than two 128 x 128 tiles
+    per CU take 64-row tiles; 2100 x 4100 (561 tiles) takes 128-row ones."""
     import torch
     from fluidlets_amd import _lib
     print(f"case: gemm {dtype} {M}x{N}x{K}", flush=True)
