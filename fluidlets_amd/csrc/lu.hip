@@ -1679,79 +1679,92 @@ __global__ __launch_bounds__(256) void copy_rows_kernel(const TS* src, TD* dst, 
 constexpr int RES_ROWS = 8;
 // PAIR: two adjacent columns per lane and step, read as one 16-byte load per
 // row (A 16-byte aligned, lda even), which doubles the bytes each load keeps
-// in flight.  Either variant sums in a fixed order for a given shape.
-template <bool PAIR>
-__global__ __launch_bounds__(256) void residual_kernel(const double* A, int64_t lda, int64_t n, const double* b,
-                                                       int64_t ldb, const double* x, double* r, int nrhs, int64_t zb)
+// in flight.  NQ = nrhs (1 .. 8) sizes the accumulators: with 6 right-hand
+// sides the kernel stays under 256 VGPRs (two waves per SIMD, twice the
+// loads in flight of the 8-column form, which would spill under that
+// bound).  Either variant sums in a fixed order for a given shape.
+template <bool PAIR, int NQ>
+__global__ __launch_bounds__(256, NQ < 8 ? 2 : 1) void residual_kernel(const double* A, int64_t lda, int64_t n, const double* b,
+                                                       int64_t ldb, const double* x, double* r, int64_t zb)
 {
     A = zoff(A, zb);
     b = zoff(b, zb);
     x = zoff(x, zb);
     r = zoff(r, zb);
 
-    __shared__ double part[4][RES_ROWS * 8];
+    __shared__ double part[4][RES_ROWS * NQ];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int64_t row0 = (int64_t)blockIdx.x * RES_ROWS;
-    double acc[RES_ROWS][8];
+    double acc[RES_ROWS][NQ];
 #pragma unroll
     for (int i = 0; i < RES_ROWS; ++i)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc[i][q] = 0.0;
+        for (int q = 0; q < NQ; ++q) acc[i][q] = 0.0;
     int64_t c = PAIR ? 2 * (int64_t)threadIdx.x : (int64_t)threadIdx.x;
     if (PAIR) {
         for (; c + 1 < n; c += 512) {
-            double x0[8], x1[8];
+            double x0[NQ], x1[NQ];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                x0[q] = q < nrhs ? x[c * nrhs + q] : 0.0;
-                x1[q] = q < nrhs ? x[(c + 1) * nrhs + q] : 0.0;
+            for (int q = 0; q < NQ; ++q) {
+                x0[q] = x[c * NQ + q];
+                x1[q] = x[(c + 1) * NQ + q];
             }
 #pragma unroll
             for (int i = 0; i < RES_ROWS; ++i) {
                 const double2 a = row0 + i < n ? *reinterpret_cast<const double2*>(A + (row0 + i) * lda + c)
                                                : make_double2(0.0, 0.0);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc[i][q] += a.x * x0[q] + a.y * x1[q];
+                for (int q = 0; q < NQ; ++q) acc[i][q] += a.x * x0[q] + a.y * x1[q];
             }
         }
     }
     for (; c < n; c += PAIR ? 512 : 256) {   // PAIR: at most the odd last column
-        double xv[8];
+        double xv[NQ];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) xv[q] = q < nrhs ? x[c * nrhs + q] : 0.0;
+        for (int q = 0; q < NQ; ++q) xv[q] = x[c * NQ + q];
 #pragma unroll
         for (int i = 0; i < RES_ROWS; ++i) {
             const double a = row0 + i < n ? A[(row0 + i) * lda + c] : 0.0;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) acc[i][q] += a * xv[q];
+            for (int q = 0; q < NQ; ++q) acc[i][q] += a * xv[q];
         }
     }
 #pragma unroll
     for (int i = 0; i < RES_ROWS; ++i)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             double s = acc[i][q];
 #pragma unroll
             for (int off = 32; off > 0; off /= 2) s += __shfl_xor(s, off);
             acc[i][q] = s;
         }
-    if (lane < RES_ROWS * 8) {
-        const int i = lane / 8, q = lane % 8;
+    if (lane < RES_ROWS * NQ) {
         double s = 0.0;
 #pragma unroll
         for (int ii = 0; ii < RES_ROWS; ++ii)
 #pragma unroll
-            for (int qq = 0; qq < 8; ++qq)
-                if (ii == i && qq == q) s = acc[ii][qq];
+            for (int qq = 0; qq < NQ; ++qq)
+                if (ii * NQ + qq == lane) s = acc[ii][qq];
         part[wave][lane] = s;
     }
     __syncthreads();
     const int t = threadIdx.x;
-    if (t < RES_ROWS * 8) {
-        const int i = t / 8, q = t % 8;
+    if (t < RES_ROWS * NQ) {
+        const int i = t / NQ, q = t % NQ;
         const double s = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
-        if (q < nrhs && row0 + i < n) r[(row0 + i) * nrhs + q] = b[(row0 + i) * ldb + q] - s;
+        if (row0 + i < n) r[(row0 + i) * NQ + q] = b[(row0 + i) * ldb + q] - s;
     }
+}
+
+// r = b - A x through residual_kernel<PAIR, nrhs> (x and r with ld nrhs)
+template <bool PAIR, int NQ = 1>
+void residual_launch(int nrhs, dim3 grid, hipStream_t st, const double* A, int64_t lda, int64_t n, const double* b,
+                     int64_t ldb, const double* x, double* r, int64_t zb)
+{
+    if constexpr (NQ < 8) {
+        if (nrhs != NQ) return residual_launch<PAIR, NQ + 1>(nrhs, grid, st, A, lda, n, b, ldb, x, r, zb);
+    }
+    hipLaunchKernelGGL((residual_kernel<PAIR, NQ>), grid, dim3(256), 0, st, A, lda, n, b, ldb, x, r, zb);
 }
 
 // y = X t for an explicit FP32 inverse X (n x n, ldx) and nrhs <= 8 FP32
@@ -2644,11 +2657,11 @@ int refine_impl(const double* A64, int64_t lda64, const float* LU32, int64_t lda
         FL_LAUNCHED();
         ++it;
         if (pair)
-            hipLaunchKernelGGL(residual_kernel<true>, zgrid(dim3((unsigned)div_up(n, RES_ROWS))), dim3(256), 0, st, A64, lda64,
-                               n, b, ldb, (const double*)xc, r, q, batch().zb);
+            residual_launch<true>(q, zgrid(dim3((unsigned)div_up(n, RES_ROWS))), st, A64, lda64, n, b, ldb,
+                                  (const double*)xc, r, batch().zb);
         else
-            hipLaunchKernelGGL(residual_kernel<false>, zgrid(dim3((unsigned)div_up(n, RES_ROWS))), dim3(256), 0, st, A64,
-                               lda64, n, b, ldb, (const double*)xc, r, q, batch().zb);
+            residual_launch<false>(q, zgrid(dim3((unsigned)div_up(n, RES_ROWS))), st, A64, lda64, n, b, ldb,
+                                   (const double*)xc, r, batch().zb);
         FL_LAUNCHED();
     }
     hipLaunchKernelGGL((copy_rows_kernel<double, double>), zgrid(dim3(g)), dim3(256), 0, st, (const double*)xc, x, ldb, n, q, batch().zb);
