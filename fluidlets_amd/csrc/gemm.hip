@@ -22,6 +22,9 @@
 // A workgroup reads all of K for its tile before it writes C, so the call is
 // safe in place when the C tile is the A tile (C = A B with N <= 128 and
 // ldc = lda on the same memory) or the B tile (M <= 128, C on B's memory).
+// A product whose C overlaps B therefore always takes 128-row tiles: with
+// 64-row ones two workgroups would share B's column block, and one could
+// overwrite rows of B that the other has still to read.
 //
 // Optional epilogue: the largest |C| written is folded into *absmax_bits
 // (IEEE float bits, atomic max), which the LU uses to verify its pivots.
@@ -375,7 +378,11 @@ GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, T alpha, const T* A, int6
     // CUs idle (the panel products of a large factorization); grouped launches
     // of many small problems keep 128-row tiles, which measured faster there
     const int64_t t128 = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
-    const int tm = narrow_tiles() && batch().count == 1 && t128 < 2 * device_cus() && M > 64 ? 64 : TM;
+    const char* bb = reinterpret_cast<const char*>(B);
+    const char* cb = reinterpret_cast<const char*>(C);
+    const bool c_on_b = cb < bb + ((K > 0 ? K - 1 : 0) * ldb + N) * (int64_t)sizeof(T) &&
+                        bb < cb + ((M - 1) * ldc + N) * (int64_t)sizeof(T);
+    const int tm = narrow_tiles() && batch().count == 1 && !c_on_b && t128 < 2 * device_cus() && M > 64 ? 64 : TM;
     const int tiles_m = (int)((M + tm - 1) / tm), tiles_n = (int)((N + TN - 1) / TN);
     return GemmArgs<T>{M,     N,          K,           alpha,   beta,    A,       lda,   B,
                        ldb,   C,          ldc,         absmax_bits, aligned, tiles_m, tiles_n, swizzle,

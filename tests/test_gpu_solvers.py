@@ -419,6 +419,35 @@ def test_mfma_gemm_matches_numpy(dtype, M, N, K):
     assert rel_err(dC.cpu().numpy().astype(np.float64), ref) < tol
 
 
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_mfma_gemm_in_place_on_b(dtype):
+    """C = A B written over B (M = K = 128, N = 60,000), as the LU's U12 =
+    L11^-1 A12 products are: one tile row, so no workgroup overwrites rows
+    of B that another has still to read.  A product this narrow (469 tiles
+    of 128 rows) would otherwise take 64-row tiles, 938 workgroups, more
+    than run at once, and the later ones would read rows already written.
+    Twice, both results equal."""
+    import torch
+    from fluidlets_amd import _lib
+    print(f"case: gemm in place on B {dtype}", flush=True)
+    rng = np.random.default_rng(11)
+    M, N = 128, 60000
+    A = rng.standard_normal((M, M)).astype(dtype)
+    B = rng.standard_normal((M, N)).astype(dtype)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    fn = _lib.lib().fl_sgemm if dtype == "float32" else _lib.lib().fl_dgemm
+    outs = []
+    for _ in range(2):
+        dA, dB = dev(A), dev(B)
+        st = fn(M, N, M, 1.0, dA.data_ptr(), M, dB.data_ptr(), N, 0.0, dB.data_ptr(), N,
+                torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        outs.append(dB.cpu().numpy())
+    tol = 1e-5 if dtype == "float32" else 1e-14
+    assert rel_err(outs[0].astype(np.float64), ref) < tol
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("n_points,kernel,products", [
     (300, "stokeslet", "fp32"),          # n = 900: every update on the FP32 lower-tile GEMM
     (1200, "brinkman:3.0", "fp32"),      # n = 3,600: FP32 products, lower tiles of 128
