@@ -341,6 +341,8 @@ int fl_sgetrf_async_ex(float* A, int64_t n, int64_t lda, int32_t* ipiv, void* au
  * (fl_sgemm) or FP64 (fl_dgemm) accumulation; C is not read when beta == 0.
  * alpha (A B) and beta C are each formed once and added, as in a reference
  * GEMM (for alpha = +-1 the sum starts from alpha beta C, exact as well).
+ * C may be written over A (N <= 128, ldc = lda, same memory) or over B
+ * (M <= 128, C on B's memory); no other overlap is allowed.
  * The LU's update kernel.  It has no counterpart call in the reference (there
  * it is the gemm inside LAPACK getrf); exported for tests and for callers
  * composing their own blocked algorithms. */
