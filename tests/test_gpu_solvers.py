@@ -168,6 +168,24 @@ def test_mixed_precision_refinement_reaches_1e_10():
     assert rel_err(x.cpu().numpy(), xref) < 1e-10
 
 
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_refinement_with_each_right_hand_side_count(nrhs):
+    """The FP64 residual has one kernel per right-hand-side count (1 .. 8,
+    accumulators sized to it); each refines to the LAPACK solution."""
+    from fluidlets_amd import linalg
+    import torch
+    print(f"case: refine nrhs {nrhs}", flush=True)
+    n = 300
+    X = fibonacci_sphere(n, 1.0, (0, 0, 1.5))
+    K = O.assemble("wall", X, X, 0.5 * np.sqrt(4 * np.pi / n))
+    b = np.random.default_rng(40 + nrhs).standard_normal((3 * n, nrhs))
+    F32 = linalg.getrf(dev(K).to(torch.float32))
+    x, steps, res = linalg.refine(dev(K), F32, dev(b), tol=1e-12)
+    assert x.shape == (3 * n, nrhs) and res.shape == (nrhs,)
+    assert res.max() <= 1e-12
+    assert rel_err(x.cpu().numpy(), sl.solve(K, b)) < 1e-10
+
+
 @pytest.mark.parametrize("n_points", [1024, 1500])
 def test_bf16x3_products_lu_refines_to_the_fp64_solution(n_points):
     """FP32 LU whose trailing updates (>= 2048 on a side) take split-bf16
